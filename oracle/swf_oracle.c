@@ -1,0 +1,895 @@
+/*
+ * swf_oracle.c - CPU restatement of the reference sweeping FD-WENO hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): see swf_oracle.h.
+ *
+ * Conventions used throughout, chosen to reproduce the reference build
+ * (gfortran/flang, `-O3 -fdefault-real-8`, x86-64 SSE2, no FMA):
+ *   - every double expression is written in the reference's left-to-right order,
+ *     structural zeros and ones of the eigenvector matrices included, so that the
+ *     sign of zero and every rounding match;
+ *   - every `d`-literal site is evaluated in __float128 (REAL(16)) and rounded once
+ *     to double on assignment (`D()` below);
+ *   - this file is compiled with -ffp-contract=off (no FMA).
+ */
+#include "swf_oracle.h"
+
+#include <math.h>
+#include <quadmath.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef __float128 quad;
+#define Q(x) ((quad)(x))
+static inline double D(quad x) { return (double)x; }
+
+/* Fortran MAX / MIN on two reals as lowered for non-NaN operands. */
+static inline double fmax2(double a, double b) { return (b > a) ? b : a; }
+static inline double fmin2(double a, double b) { return (b < a) ? b : a; }
+
+/* ------------------------------------------------------------------------- */
+/* weno5.f90 (ex4_sedov_2d/weno5.f90:3-15, 86-146)                            */
+/* ------------------------------------------------------------------------- */
+
+/* c(-1:2, 0:2), RESHAPE column-major of the 12 decimal literals (weno5.f90:3-14).
+ * The literals are default REAL = REAL(8) under -fdefault-real-8. */
+static const double WC[4][3] = {
+    /* c(-1,0..2) */ {1.83333333333333333333333333333333, -1.16666666666666666666666666666667,
+                      0.333333333333333333333333333333333},
+    /* c( 0,0..2) */ {0.333333333333333333333333333333333, 0.833333333333333333333333333333333,
+                      -0.166666666666666666666666666666667},
+    /* c( 1,0..2) */ {-0.166666666666666666666666666666667, 0.833333333333333333333333333333333,
+                      0.333333333333333333333333333333333},
+    /* c( 2,0..2) */ {0.333333333333333333333333333333333, -1.16666666666666666666666666666667,
+                      1.83333333333333333333333333333333},
+};
+#define CW(r, col) WC[(r) + 1][(col)]
+
+/* beta(1..3), weno5.f90:100-102 and :134-136.  `a` points at average(0). */
+static void weno_beta(const double *a, double beta[3]) {
+    const quad q13_12 = Q(13) / Q(12); /* 13d0/12d0 folded in REAL(16) */
+    const quad q025 = Q(0.25);
+    quad x, y;
+    /* beta(1) = 13/12*(a0-2a1+a2)**2 + 0.25*(3a0-4a1+a2)**2, all REAL(16) */
+    x = (Q(a[0]) - Q(2) * Q(a[1])) + Q(a[2]);
+    y = (Q(3) * Q(a[0]) - Q(4) * Q(a[1])) + Q(a[2]);
+    beta[0] = D(q13_12 * (x * x) + q025 * (y * y));
+    /* beta(2) = 13/12*(a-1-2a0+a1)**2 + 0.25*(a-1 - a1)**2 ; the second
+     * square has no d-literal inside, so it is REAL(8) then promoted. */
+    x = (Q(a[-1]) - Q(2) * Q(a[0])) + Q(a[1]);
+    {
+        double yd = a[-1] - a[1];
+        double y2 = yd * yd;
+        beta[1] = D(q13_12 * (x * x) + q025 * Q(y2));
+    }
+    /* beta(3) = 13/12*(a-2-2a-1+a0)**2 + 0.25*(a-2-4a-1+3a0)**2 */
+    x = (Q(a[-2]) - Q(2) * Q(a[-1])) + Q(a[0]);
+    y = (Q(a[-2]) - Q(4) * Q(a[-1])) + Q(3) * Q(a[0]);
+    beta[2] = D(q13_12 * (x * x) + q025 * (y * y));
+}
+
+/* alpha = d/(1d-6+beta)**2 with d a REAL(8) literal, evaluated in REAL(16). */
+static inline double weno_alpha(double d, double beta) {
+    quad e = 1e-6Q + Q(beta);
+    return D(Q(d) / (e * e));
+}
+
+/* weno_right (weno5.f90:120-146): value at x_{i+1/2}; `a` points at average(0) */
+static double weno_right(const double *a) {
+    double um[3], beta[3], al[3], ww[3];
+    for (int k = 1; k <= 3; ++k) {
+        int is = k - 3;
+        double s = 0.0;
+        for (int i = 0; i <= 2; ++i) s = s + CW(-is, i) * a[is + i];
+        um[k - 1] = s;
+    }
+    weno_beta(a, beta);
+    al[0] = weno_alpha(0.3, beta[0]);
+    al[1] = weno_alpha(0.6, beta[1]);
+    al[2] = weno_alpha(0.1, beta[2]);
+    double sum = (al[0] + al[1]) + al[2];
+    ww[0] = al[0] / sum;
+    ww[1] = al[1] / sum;
+    ww[2] = al[2] / sum;
+    return (ww[0] * um[2] + ww[1] * um[1]) + ww[2] * um[0];
+}
+
+/* weno_left (weno5.f90:86-113): value at x_{i-1/2} */
+static double weno_left(const double *a) {
+    double up[3], beta[3], al[3], ww[3];
+    for (int k = 1; k <= 3; ++k) {
+        int is = k - 3;
+        double s = 0.0;
+        for (int i = 0; i <= 2; ++i) s = s + CW(-is - 1, i) * a[is + i];
+        up[k - 1] = s;
+    }
+    weno_beta(a, beta);
+    al[0] = weno_alpha(0.1, beta[0]);
+    al[1] = weno_alpha(0.6, beta[1]);
+    al[2] = weno_alpha(0.3, beta[2]);
+    double sum = (al[0] + al[1]) + al[2];
+    ww[0] = al[0] / sum;
+    ww[1] = al[1] / sum;
+    ww[2] = al[2] / sum;
+    return (ww[0] * up[2] + ww[1] * up[1]) + ww[2] * up[0];
+}
+
+void swo_weno_right_n(const double *a, double *out, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) out[k] = weno_right(a + 5 * k + 2);
+}
+void swo_weno_left_n(const double *a, double *out, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) out[k] = weno_left(a + 5 * k + 2);
+}
+
+/* ------------------------------------------------------------------------- */
+/* context                                                                    */
+/* ------------------------------------------------------------------------- */
+
+struct swo_ctx {
+    swo_config cfg;
+    int nx, ny, nxg, nyg; /* nxg = nx+6 */
+    double gamma1;
+    double weight;        /* weights(i,j) = 1./(nSteps_x*nSteps_y) */
+    double *u, *u0, *rhs, *fpf, *fnf, *gpf, *gnf;
+    double *fp, *fn;      /* one line of split fluxes, 4*(max(nx,ny)+6) */
+    double *pw, *pvx, *pvy, *ph;
+    int32_t *ordI_i, *ordI_j, *ordII_i, *ordII_j;
+    double *rho_t;
+    /* statistics */
+    int64_t sweep_count, den_count, pre_count, stage_sweeps;
+    int64_t *trk_q, *trk_neg, *trk_pos;
+    int64_t trk_cap;
+    int64_t t_pos_count, t_neg_count;
+};
+
+#define U4(arr, c, i, j) (arr)[(c) + 4 * (((i) + 2) + C->nxg * ((j) + 2))]
+
+static void sweeping_order(swo_ctx *C) {
+    /* ex4_sedov_2d/main.f90:1836-1895 */
+    int nx = C->nx, ny = C->ny;
+    int64_t k = 0;
+    for (int j = 1; j <= ny; ++j) {
+        if (j % 2 == 1)
+            for (int i = 1; i <= nx; ++i) { C->ordI_i[k] = i; C->ordI_j[k] = j; ++k; }
+        else
+            for (int i = nx; i >= 1; --i) { C->ordI_i[k] = i; C->ordI_j[k] = j; ++k; }
+    }
+    k = 0;
+    for (int i = 1; i <= nx; ++i) {
+        if (i % 2 == 1)
+            for (int j = 1; j <= ny; ++j) { C->ordII_i[k] = i; C->ordII_j[k] = j; ++k; }
+        else
+            for (int j = ny; j >= 1; --j) { C->ordII_i[k] = i; C->ordII_j[k] = j; ++k; }
+    }
+}
+
+swo_ctx *swo_create(const swo_config *cfg) {
+    swo_ctx *C = (swo_ctx *)calloc(1, sizeof(swo_ctx));
+    C->cfg = *cfg;
+    C->nx = cfg->nx;
+    C->ny = cfg->ny;
+    C->nxg = cfg->nx + 6;
+    C->nyg = cfg->ny + 6;
+    C->gamma1 = cfg->gamma - 1.;
+    C->weight = 1. / ((double)cfg->nx * (double)cfg->ny);
+    size_t n = (size_t)4 * C->nxg * C->nyg;
+    C->u = (double *)calloc(n, sizeof(double));
+    C->u0 = (double *)calloc(n, sizeof(double));
+    C->rhs = (double *)calloc(n, sizeof(double));
+    C->fpf = (double *)calloc(n, sizeof(double));
+    C->fnf = (double *)calloc(n, sizeof(double));
+    C->gpf = (double *)calloc(n, sizeof(double));
+    C->gnf = (double *)calloc(n, sizeof(double));
+    int m = (cfg->nx > cfg->ny ? cfg->nx : cfg->ny) + 6;
+    C->fp = (double *)calloc((size_t)4 * m, sizeof(double));
+    C->fn = (double *)calloc((size_t)4 * m, sizeof(double));
+    C->pw = (double *)calloc(m, sizeof(double));
+    C->pvx = (double *)calloc(m, sizeof(double));
+    C->pvy = (double *)calloc(m, sizeof(double));
+    C->ph = (double *)calloc(m, sizeof(double));
+    size_t N = (size_t)cfg->nx * cfg->ny;
+    C->ordI_i = (int32_t *)malloc(N * sizeof(int32_t));
+    C->ordI_j = (int32_t *)malloc(N * sizeof(int32_t));
+    C->ordII_i = (int32_t *)malloc(N * sizeof(int32_t));
+    C->ordII_j = (int32_t *)malloc(N * sizeof(int32_t));
+    C->rho_t = (double *)malloc(N * sizeof(double));
+    C->trk_cap = 1024;
+    C->trk_q = (int64_t *)calloc(C->trk_cap, sizeof(int64_t));
+    C->trk_neg = (int64_t *)calloc(C->trk_cap, sizeof(int64_t));
+    C->trk_pos = (int64_t *)calloc(C->trk_cap, sizeof(int64_t));
+    sweeping_order(C);
+    return C;
+}
+
+void swo_destroy(swo_ctx *C) {
+    if (!C) return;
+    free(C->u); free(C->u0); free(C->rhs);
+    free(C->fpf); free(C->fnf); free(C->gpf); free(C->gnf);
+    free(C->fp); free(C->fn); free(C->pw); free(C->pvx); free(C->pvy); free(C->ph);
+    free(C->ordI_i); free(C->ordI_j); free(C->ordII_i); free(C->ordII_j);
+    free(C->rho_t);
+    free(C->trk_q); free(C->trk_neg); free(C->trk_pos);
+    free(C);
+}
+
+void swo_set_state(swo_ctx *C, const double *u) {
+    memcpy(C->u, u, sizeof(double) * 4 * (size_t)C->nxg * C->nyg);
+}
+void swo_get_state(const swo_ctx *C, double *u) {
+    memcpy(u, C->u, sizeof(double) * 4 * (size_t)C->nxg * C->nyg);
+}
+
+/* ------------------------------------------------------------------------- */
+/* boundary conditions                                                        */
+/* ------------------------------------------------------------------------- */
+
+static double cell_y(const swo_ctx *C, int j) { return C->cfg.ylo + j * C->cfg.dy - C->cfg.dy / 2.; }
+
+void swo_bc(swo_ctx *C) {
+    double *u = C->u;
+    int nx = C->nx, ny = C->ny;
+    switch (C->cfg.bc) {
+    case SWO_BC_SEDOV:
+        /* ex4_sedov_2d/main.f90:916-994 at cut_x = cut_y = 1.  The periodic
+         * self-exchange that precedes these loops (:857-911) is overwritten in
+         * every ghost cell by the four physical passes below, in this order. */
+        for (int j = -2; j <= ny + 3; ++j)
+            for (int i = 1; i <= 3; ++i) {
+                U4(u, 0, 1 - i, j) = U4(u, 0, i, j);
+                U4(u, 1, 1 - i, j) = -U4(u, 1, i, j);
+                U4(u, 2, 1 - i, j) = U4(u, 2, i, j);
+                U4(u, 3, 1 - i, j) = U4(u, 3, i, j);
+            }
+        for (int j = 1; j <= 3; ++j)
+            for (int i = -2; i <= nx + 3; ++i) {
+                U4(u, 0, i, 1 - j) = U4(u, 0, i, j);
+                U4(u, 1, i, 1 - j) = U4(u, 1, i, j);
+                U4(u, 2, i, 1 - j) = -U4(u, 2, i, j);
+                U4(u, 3, i, 1 - j) = U4(u, 3, i, j);
+            }
+        for (int ig = 1; ig <= 3; ++ig)
+            for (int j = -2; j <= ny + 3; ++j)
+                for (int c = 0; c < 4; ++c) U4(u, c, nx + ig, j) = U4(u, c, nx, j);
+        for (int ig = 1; ig <= 3; ++ig)
+            for (int i = -2; i <= nx + 3; ++i)
+                for (int c = 0; c < 4; ++c) U4(u, c, i, ny + ig) = U4(u, c, i, ny);
+        break;
+    case SWO_BC_PERIODIC:
+        /* ex3_vortex_accuracy_2d/main.f90:653-669 */
+        for (int j = -2; j <= ny + 3; ++j)
+            for (int i = 1; i <= 3; ++i)
+                for (int c = 0; c < 4; ++c) {
+                    U4(u, c, 1 - i, j) = U4(u, c, nx + 1 - i, j);
+                    U4(u, c, nx + i, j) = U4(u, c, i, j);
+                }
+        for (int j = 1; j <= 3; ++j)
+            for (int i = -2; i <= nx + 3; ++i)
+                for (int c = 0; c < 4; ++c) {
+                    U4(u, c, i, 1 - j) = U4(u, c, i, ny + 1 - j);
+                    U4(u, c, i, ny + j) = U4(u, c, i, j);
+                }
+        break;
+    case SWO_BC_JET_HALF:
+    case SWO_BC_JET_FULL: {
+        /* ex5_mach2000_jet/main.f90:686-741 (half domain, reflective y = 0);
+         * the full-domain variant replaces the reflection by outflow and the
+         * inflow test y <= 0.05 by |y| < 0.05. */
+        const double g1 = C->gamma1;
+        for (int j = -2; j <= ny + 3; ++j) {
+            double yj = cell_y(C, j);
+            int in = (C->cfg.bc == SWO_BC_JET_HALF) ? (yj <= C->cfg.jet_halfwidth)
+                                                    : (fabs(yj) < C->cfg.jet_halfwidth);
+            for (int i = 1; i <= 3; ++i) {
+                if (in) {
+                    U4(u, 0, 1 - i, j) = C->cfg.jet_rho;
+                    U4(u, 1, 1 - i, j) = C->cfg.jet_rho * C->cfg.jet_u;
+                    U4(u, 2, 1 - i, j) = 0.;
+                    U4(u, 3, 1 - i, j) = C->cfg.amb_p / g1 +
+                                          0.5 * (U4(u, 1, 1 - i, j) * U4(u, 1, 1 - i, j)) / U4(u, 0, 1 - i, j);
+                } else {
+                    U4(u, 0, 1 - i, j) = C->cfg.amb_rho;
+                    U4(u, 1, 1 - i, j) = 0.;
+                    U4(u, 2, 1 - i, j) = 0.;
+                    U4(u, 3, 1 - i, j) = C->cfg.amb_p / g1;
+                }
+            }
+        }
+        if (C->cfg.bc == SWO_BC_JET_HALF) {
+            for (int j = 1; j <= 3; ++j)
+                for (int i = -2; i <= nx + 3; ++i) {
+                    for (int c = 0; c < 4; ++c) U4(u, c, i, 1 - j) = U4(u, c, i, j);
+                    U4(u, 2, i, 1 - j) = -U4(u, 2, i, j);
+                }
+        } else {
+            for (int j = 1; j <= 3; ++j)
+                for (int i = -2; i <= nx + 3; ++i)
+                    for (int c = 0; c < 4; ++c) U4(u, c, i, 1 - j) = U4(u, c, i, 1);
+        }
+        for (int ig = 1; ig <= 3; ++ig)
+            for (int j = -2; j <= ny + 3; ++j)
+                for (int c = 0; c < 4; ++c) U4(u, c, nx + ig, j) = U4(u, c, nx, j);
+        for (int ig = 1; ig <= 3; ++ig)
+            for (int i = -2; i <= nx + 3; ++i)
+                for (int c = 0; c < 4; ++c) U4(u, c, i, ny + ig) = U4(u, c, i, ny);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* findalpha, fluxes, wenofd, scheme                                          */
+/* ------------------------------------------------------------------------- */
+
+double swo_alpha_max(swo_ctx *C) {
+    const double g = C->cfg.gamma, g1 = C->gamma1, eps2 = C->cfg.eps2;
+    const double *u = C->u;
+    double amax = 0.;
+    for (int i = -2; i <= C->nx + 3; ++i)
+        for (int j = -2; j <= C->ny + 3; ++j) {
+            double u1 = U4(u, 0, i, j), u2 = U4(u, 1, i, j), u3 = U4(u, 2, i, j), u4 = U4(u, 3, i, j);
+            double den = u1;
+            if (C->cfg.findalpha_ex3) {
+                /* ex3_vortex_accuracy_2d/main.f90:549-557 */
+                double pres = g1 * ((u4 - ((u2 * u2) / u1) / 2.) - ((u3 * u3) / u1) / 2.);
+                double sound = sqrt((g * pres) / den);
+                amax = fmax2(fmax2(amax, fabs(u2 / u1) + sound), fabs(u3 / u1) + sound);
+            } else {
+                /* ex4_sedov_2d/main.f90:652-683 */
+                double vx, vy;
+                if (den >= eps2) {
+                    vx = u2 / den;
+                    vy = u3 / den;
+                } else {
+                    vx = 0.;
+                    vy = 0.;
+                }
+                double pres = g1 * ((u4 - (0.5 * den) * (vx * vx)) - (0.5 * den) * (vy * vy));
+                double sound = sqrt((g * pres) / den);
+                double a = fmax2(fabs(u2 / u1) + sound, fabs(u3 / u1) + sound);
+                amax = fmax2(amax, a);
+            }
+        }
+    return amax;
+}
+
+/* rho < eps2 test of fflux/gflux: REAL(8) parameter (ex4) or the 1d-8 literal (ex3/ex5) */
+static inline int below_eps2(const swo_ctx *C, double rho) {
+    if (C->cfg.eps2_quad_literal) return Q(rho) < 1e-8Q;
+    return rho < C->cfg.eps2;
+}
+
+/* fflux / gflux (ex4_sedov_2d/main.f90:125-177); dir 0 = x, 1 = y */
+static void euler_flux(const swo_ctx *C, const double *a, int dir, double f[4]) {
+    double rho = a[0], mx = a[1], my = a[2], e = a[3];
+    double vx = mx / rho, vy = my / rho;
+    if (below_eps2(C, rho)) vx = 0.;
+    if (below_eps2(C, rho)) vy = 0.;
+    double pres = C->gamma1 * ((e - (0.5 * rho) * (vx * vx)) - (0.5 * rho) * (vy * vy));
+    if (dir == 0) {
+        f[0] = mx;
+        f[1] = rho * (vx * vx) + pres;
+        f[2] = (rho * vx) * vy;
+        f[3] = (e + pres) * vx;
+    } else {
+        f[0] = my;
+        f[1] = (rho * vx) * vy;
+        f[2] = rho * (vy * vy) + pres;
+        f[3] = (e + pres) * vy;
+    }
+}
+
+/* primitive data for the Roe average; `ydens` selects the y-pass reciprocal */
+static void roe_prims(const swo_ctx *C, const double *a, int ypass, double *w, double *vx, double *vy,
+                      double *h) {
+    double den = a[0], xmt = a[1], ymt = a[2], eng = a[3];
+    double t0;
+    if (ypass && !C->cfg.ydens_plain)
+        t0 = D(Q(1.) / (Q(den) + 1e-15Q)); /* ex4_sedov_2d/main.f90:426 */
+    else
+        t0 = 1. / den; /* :206 */
+    double vex = xmt * t0, vey = ymt * t0;
+    double pre = C->gamma1 * (eng - 0.5 * (xmt * vex + ymt * vey));
+    *w = sqrt(fabs(den));
+    *vx = vex;
+    *vy = vey;
+    *h = (pre + eng) * t0;
+}
+
+static inline double roe_weight(const swo_ctx *C, double wa, double wb) {
+    if (C->cfg.roe_eps_double) return wa / ((wa + wb) + 1.e-15); /* ex3 :153 */
+    return D(Q(wa) / (Q(wa + wb) + 1e-15Q));                      /* ex4 :234 */
+}
+static inline double cm_eps(const swo_ctx *C, double s) {
+    if (C->cfg.roe_eps_double) return s + 1.e-15; /* ex3 :159 */
+    return D(Q(s) + 1e-15Q);                      /* ex4 :240 */
+}
+
+static inline double dot4(const double e[4], const double *f) {
+    return ((e[0] * f[0] + e[1] * f[1]) + e[2] * f[2]) + e[3] * f[3];
+}
+
+/* One interface: Roe matrices at (a|b), projection of the 5-point stencils
+ * fp/fn centred on `a`, 8 reconstructions, back-projection.  `fps`/`fns` point at
+ * the stencil centre; stride between points is 4 doubles.  dir 0 = x, 1 = y. */
+static void interface_flux(const swo_ctx *C, int dir, double wa, double wb, double vxa, double vxb,
+                           double vya, double vyb, double ha, double hb, const double *fps,
+                           const double *fns, double fpo[4], double fno[4]) {
+    const double g1 = C->gamma1;
+    double evr[4][4], evl[4][4];
+    double t0 = roe_weight(C, wa, wb);
+    double t1 = 1. - t0;
+    double vxm = t0 * vxa + t1 * vxb;
+    double vym = t0 * vya + t1 * vyb;
+    double hm = t0 * ha + t1 * hb;
+    double qm = 0.5 * (vxm * vxm + vym * vym);
+    double cm = cm_eps(C, sqrt(fabs(g1 * (hm - qm))));
+    if (dir == 0) {
+        /* ex4_sedov_2d/main.f90:241-280 */
+        t0 = vxm * cm;
+        evr[0][0] = 1.0; evr[0][1] = 0.0; evr[0][2] = 1.0; evr[0][3] = 1.0;
+        evr[1][0] = vxm - cm; evr[1][1] = 0.0; evr[1][2] = vxm; evr[1][3] = vxm + cm;
+        evr[2][0] = vym; evr[2][1] = 1.0; evr[2][2] = vym; evr[2][3] = vym;
+        evr[3][0] = hm - t0; evr[3][1] = vym; evr[3][2] = qm; evr[3][3] = hm + t0;
+        double rcm = 1. / cm;
+        double b1 = (g1 * rcm) * rcm;
+        double b2 = qm * b1;
+        t0 = vxm * rcm;
+        t1 = b1 * vxm;
+        double t2 = 0.5 * b1;
+        double t3 = b1 * vym;
+        evl[0][0] = 0.5 * (b2 + t0); evl[0][1] = -(0.5 * (t1 + rcm)); evl[0][2] = -(0.5 * t3); evl[0][3] = t2;
+        evl[1][0] = -vym; evl[1][1] = 0.0; evl[1][2] = 1.0; evl[1][3] = 0.0;
+        evl[2][0] = 1. - b2; evl[2][1] = t1; evl[2][2] = t3; evl[2][3] = -b1;
+        evl[3][0] = 0.5 * (b2 - t0); evl[3][1] = -(0.5 * (t1 - rcm)); evl[3][2] = -(0.5 * t3); evl[3][3] = t2;
+    } else {
+        /* ex4_sedov_2d/main.f90:462-501 */
+        t0 = vym * cm;
+        evr[0][0] = 1.0; evr[0][1] = 0.0; evr[0][2] = 1.0; evr[0][3] = 1.0;
+        evr[1][0] = vxm; evr[1][1] = 1.0; evr[1][2] = vxm; evr[1][3] = vxm;
+        evr[2][0] = vym - cm; evr[2][1] = 0.0; evr[2][2] = vym; evr[2][3] = vym + cm;
+        evr[3][0] = hm - t0; evr[3][1] = vxm; evr[3][2] = qm; evr[3][3] = hm + t0;
+        double rcm = 1. / cm;
+        double b1 = (g1 * rcm) * rcm;
+        double b2 = qm * b1;
+        t0 = vym * rcm;
+        t1 = b1 * vym;
+        double t2 = 0.5 * b1;
+        double t3 = b1 * vxm;
+        evl[0][0] = 0.5 * (b2 + t0); evl[0][1] = -(0.5 * t3); evl[0][2] = -(0.5 * (t1 + rcm)); evl[0][3] = t2;
+        evl[1][0] = -vxm; evl[1][1] = 1.0; evl[1][2] = 0.0; evl[1][3] = 0.0;
+        evl[2][0] = 1. - b2; evl[2][1] = t3; evl[2][2] = t1; evl[2][3] = -b1;
+        evl[3][0] = 0.5 * (b2 - t0); evl[3][1] = -(0.5 * t3); evl[3][2] = -(0.5 * (t1 - rcm)); evl[3][3] = t2;
+    }
+    /* projection (:285-290), reconstruction (:294-297), back-projection (:301-304) */
+    double cup[4][5], cun[4][5], ur[4], ul[4];
+    for (int ig = -2; ig <= 2; ++ig)
+        for (int k = 0; k < 4; ++k) {
+            cup[k][ig + 2] = dot4(evl[k], fps + 4 * ig);
+            cun[k][ig + 2] = dot4(evl[k], fns + 4 * ig);
+        }
+    for (int k = 0; k < 4; ++k) {
+        ur[k] = weno_right(&cup[k][2]);
+        ul[k] = weno_left(&cun[k][2]);
+    }
+    for (int k = 0; k < 4; ++k) {
+        fpo[k] = dot4(evr[k], ur);
+        fno[k] = dot4(evr[k], ul);
+    }
+}
+
+/* wenofd (ex4_sedov_2d/main.f90:179-637, use_limiter /= 1), for the rows and
+ * columns that `scheme` reads.  Fills fpf/fnf (x) and gpf/gnf (y). */
+static void wenofd(swo_ctx *C, double alphamax) {
+    const double *u = C->u;
+    const int nx = C->nx, ny = C->ny;
+    const int scale = !C->cfg.alpha_in_scheme;
+    double f[4];
+    /* x sweep over rows j = 1..ny */
+    for (int j = 1; j <= ny; ++j) {
+        for (int i = 0; i <= nx + 2; ++i)
+            roe_prims(C, &U4(u, 0, i, j), 0, &C->pw[i], &C->pvx[i], &C->pvy[i], &C->ph[i]);
+        for (int i = -2; i <= nx + 3; ++i) {
+            const double *a = &U4(u, 0, i, j);
+            euler_flux(C, a, 0, f);
+            for (int c = 0; c < 4; ++c) {
+                C->fp[4 * (i + 2) + c] = 0.5 * (a[c] + f[c] / alphamax);
+                C->fn[4 * (i + 2) + c] = 0.5 * (a[c] - f[c] / alphamax);
+            }
+        }
+        for (int i = 0; i <= nx + 1; ++i) {
+            double fpo[4], fno[4];
+            interface_flux(C, 0, C->pw[i], C->pw[i + 1], C->pvx[i], C->pvx[i + 1], C->pvy[i], C->pvy[i + 1],
+                           C->ph[i], C->ph[i + 1], &C->fp[4 * (i + 2)], &C->fn[4 * (i + 2)], fpo, fno);
+            for (int c = 0; c < 4; ++c) {
+                U4(C->fpf, c, i, j) = scale ? alphamax * fpo[c] : fpo[c];
+                U4(C->fnf, c, i, j) = scale ? alphamax * fno[c] : fno[c];
+            }
+        }
+    }
+    /* y sweep over columns i = 1..nx */
+    for (int i = 1; i <= nx; ++i) {
+        for (int j = 0; j <= ny + 2; ++j)
+            roe_prims(C, &U4(u, 0, i, j), 1, &C->pw[j], &C->pvx[j], &C->pvy[j], &C->ph[j]);
+        for (int j = -2; j <= ny + 3; ++j) {
+            const double *a = &U4(u, 0, i, j);
+            euler_flux(C, a, 1, f);
+            for (int c = 0; c < 4; ++c) {
+                C->fp[4 * (j + 2) + c] = 0.5 * (a[c] + f[c] / alphamax);
+                C->fn[4 * (j + 2) + c] = 0.5 * (a[c] - f[c] / alphamax);
+            }
+        }
+        for (int j = 0; j <= ny + 1; ++j) {
+            double fpo[4], fno[4];
+            interface_flux(C, 1, C->pw[j], C->pw[j + 1], C->pvx[j], C->pvx[j + 1], C->pvy[j], C->pvy[j + 1],
+                           C->ph[j], C->ph[j + 1], &C->fp[4 * (j + 2)], &C->fn[4 * (j + 2)], fpo, fno);
+            for (int c = 0; c < 4; ++c) {
+                U4(C->gpf, c, i, j) = scale ? alphamax * fpo[c] : fpo[c];
+                U4(C->gnf, c, i, j) = scale ? alphamax * fno[c] : fno[c];
+            }
+        }
+    }
+}
+
+/* scheme (ex4_sedov_2d/main.f90:639-650; ex3_vortex_accuracy_2d/main.f90:536-546) */
+static void scheme(swo_ctx *C, double alphamax) {
+    const double dx = C->cfg.dx, dy = C->cfg.dy;
+    memset(C->rhs, 0, sizeof(double) * 4 * (size_t)C->nxg * C->nyg);
+    for (int i = 1; i <= C->nx; ++i)
+        for (int j = 1; j <= C->ny; ++j)
+            for (int c = 0; c < 4; ++c) {
+                double X = (U4(C->fpf, c, i, j) - U4(C->fnf, c, i + 1, j)) -
+                           (U4(C->fpf, c, i - 1, j) - U4(C->fnf, c, i, j));
+                double Y = (U4(C->gpf, c, i, j) - U4(C->gnf, c, i, j + 1)) -
+                           (U4(C->gpf, c, i, j - 1) - U4(C->gnf, c, i, j));
+                double r;
+                if (C->cfg.alpha_in_scheme)
+                    r = -((alphamax * X) / dx) - (alphamax * Y) / dy;
+                else
+                    r = -(X / dx) - Y / dy;
+                U4(C->rhs, c, i, j) = r;
+            }
+}
+
+void swo_rhs(swo_ctx *C, double alphamax, double *rhs_out) {
+    wenofd(C, alphamax);
+    scheme(C, alphamax);
+    memcpy(rhs_out, C->rhs, sizeof(double) * 4 * (size_t)C->nxg * C->nyg);
+}
+
+/* checkpositivity (ex4_sedov_2d/main.f90:779-801; ex3 uses <=) */
+int swo_check_positivity(const swo_ctx *C) {
+    const double eps = C->cfg.eps;
+    const double *u = C->u;
+    int postag = 0;
+    for (int i = 1; i <= C->nx; ++i)
+        for (int j = 1; j <= C->ny; ++j) {
+            double u1 = U4(u, 0, i, j), u2 = U4(u, 1, i, j), u3 = U4(u, 2, i, j), u4 = U4(u, 3, i, j);
+            double ke = (u4 - ((u3 * u3) / u1) / 2.) - ((u2 * u2) / u1) / 2.;
+            if (C->cfg.poscheck_le) {
+                if (u1 <= eps) postag = 1;
+                if (ke <= eps) postag = 1;
+            } else {
+                if (u1 < eps) postag = 1;
+                if (ke < eps) postag = 1;
+            }
+        }
+    return postag;
+}
+
+/* ------------------------------------------------------------------------- */
+/* sweeping post-process                                                      */
+/* ------------------------------------------------------------------------- */
+
+static inline double sweep_pressure(double g1, const double *v) {
+    return g1 * (v[3] - (0.5 * (v[1] * v[1] + v[2] * v[2])) / v[0]);
+}
+
+/* nod_lim (ex4_sedov_2d/main.f90:1359-1396) */
+static int nod_lim(double g1, double eps, const double *v1, const double *v2, const double *vb, double *out) {
+    double p1 = g1 * (v1[3] - (0.5 * (v1[1] * v1[1] + v1[2] * v1[2])) / v1[0]);
+    double p2 = g1 * (v2[3] - (0.5 * (v2[1] * v2[1] + v2[2] * v2[2])) / v2[0]);
+    double t;
+    int pos;
+    if (p2 > eps) {
+        t = (p1 - eps) / (p1 - p2);
+        pos = 1;
+    } else {
+        double pb = g1 * (vb[3] - (0.5 * (vb[1] * vb[1] + vb[2] * vb[2])) / vb[0]);
+        double t_1 = (p1 - eps) / (p1 - pb);
+        double d12 = sqrt((((v2[3] - v1[3]) * (v2[3] - v1[3]) + (v2[2] - v1[2]) * (v2[2] - v1[2])) +
+                           (v2[1] - v1[1]) * (v2[1] - v1[1])) +
+                          (v2[0] - v1[0]) * (v2[0] - v1[0]));
+        double d1b = sqrt((((vb[3] - v1[3]) * (vb[3] - v1[3]) + (vb[2] - v1[2]) * (vb[2] - v1[2])) +
+                           (vb[1] - v1[1]) * (vb[1] - v1[1])) +
+                          (vb[0] - v1[0]) * (vb[0] - v1[0]));
+        t = fmin2((t_1 * d1b) / d12, 0.25);
+        pos = 0;
+    }
+    for (int k = 0; k < 4; ++k) out[k] = (1. - t) * v1[k] + t * v2[k];
+    return pos;
+}
+
+int swo_nod_lim(const double *v1, const double *v2, const double *vbar, double gamma1, double eps,
+                double *out) {
+    return nod_lim(gamma1, eps, v1, v2, vbar, out);
+}
+
+static void load4(const swo_ctx *C, const double *u, int i, int j, double v[4]) {
+    for (int c = 0; c < 4; ++c) v[c] = U4(u, c, i, j);
+}
+static void store4(swo_ctx *C, double *u, int i, int j, const double v[4]) {
+    for (int c = 0; c < 4; ++c) U4(u, c, i, j) = v[c];
+}
+
+/* scalar_sweep (ex4_sedov_2d/main.f90:1765-1833) */
+static void scalar_sweep(swo_ctx *C) {
+    double *u = C->u;
+    const int nx = C->nx, ny = C->ny;
+    const int64_t N = (int64_t)nx * ny;
+    double rho_min = 1e20;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) rho_min = fmin2(U4(u, 0, i, j), rho_min);
+    const double delta = D(1e-12Q);
+    if (rho_min < delta) {
+        double *rt = C->rho_t; /* rho_t(i,j) at (i-1) + nx*(j-1) */
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) rt[(i - 1) + (int64_t)nx * (j - 1)] = U4(u, 0, i, j);
+        const double w = C->weight;
+#define RT(k) rt[(C->ordI_i[k] - 1) + (int64_t)nx * (C->ordI_j[k] - 1)]
+        for (int64_t k = 0; k < N - 1; ++k) {
+            double v_i = RT(k), v_ip1 = RT(k + 1);
+            if (v_i < delta) {
+                RT(k + 1) = v_ip1 + (w / w) * (v_i - delta);
+                RT(k) = delta;
+            }
+        }
+        for (int64_t k = N - 1; k >= 1; --k) {
+            double v_i = RT(k), v_im1 = RT(k - 1);
+            if (v_i < delta) {
+                RT(k - 1) = v_im1 + (w / w) * (v_i - delta);
+                RT(k) = delta;
+            }
+        }
+#undef RT
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) U4(u, 0, i, j) = rt[(i - 1) + (int64_t)nx * (j - 1)];
+        C->den_count++;
+    }
+}
+
+static void track_push(swo_ctx *C, int64_t q, int64_t neg, int64_t pos) {
+    if (C->sweep_count > C->trk_cap) {
+        int64_t nc = C->trk_cap * 2;
+        C->trk_q = (int64_t *)realloc(C->trk_q, nc * sizeof(int64_t));
+        C->trk_neg = (int64_t *)realloc(C->trk_neg, nc * sizeof(int64_t));
+        C->trk_pos = (int64_t *)realloc(C->trk_pos, nc * sizeof(int64_t));
+        memset(C->trk_q + C->trk_cap, 0, (nc - C->trk_cap) * sizeof(int64_t));
+        memset(C->trk_neg + C->trk_cap, 0, (nc - C->trk_cap) * sizeof(int64_t));
+        memset(C->trk_pos + C->trk_cap, 0, (nc - C->trk_cap) * sizeof(int64_t));
+        C->trk_cap = nc;
+    }
+    C->trk_q[C->sweep_count - 1] = q;
+    C->trk_neg[C->sweep_count - 1] = neg;
+    C->trk_pos[C->sweep_count - 1] = pos;
+}
+
+/* one directional pass of sweep_lim over an order (:1468-1586) */
+static void sweep_pass(swo_ctx *C, const int32_t *oi, const int32_t *oj, int forward, const double ub[4]) {
+    double *u = C->u;
+    const double g1 = C->gamma1, eps = C->cfg.eps, w = C->weight;
+    const int64_t N = (int64_t)C->nx * C->ny;
+    double vi[4], vp[4], vit[4], vpt[4];
+    for (int64_t s = 0; s < N - 1; ++s) {
+        int64_t k = forward ? s : (N - 1 - s);
+        int64_t p = forward ? k + 1 : k - 1;
+        load4(C, u, oi[k], oj[k], vi);
+        double pres = g1 * (vi[3] - (0.5 * (vi[1] * vi[1] + vi[2] * vi[2])) / vi[0]);
+        if (pres <= eps || vi[0] <= eps) {
+            load4(C, u, oi[p], oj[p], vp);
+            if (nod_lim(g1, eps, vi, vp, ub, vit))
+                C->t_pos_count++;
+            else
+                C->t_neg_count++;
+            for (int c = 0; c < 4; ++c) vpt[c] = vp[c] + (w / w) * (vi[c] - vit[c]);
+            store4(C, u, oi[k], oj[k], vit);
+            store4(C, u, oi[p], oj[p], vpt);
+        }
+    }
+}
+
+/* sweep_lim (ex4_sedov_2d/main.f90:1401-1761) */
+int swo_sweep(swo_ctx *C) {
+    double *u = C->u;
+    const int nx = C->nx, ny = C->ny;
+    const double g1 = C->gamma1, eps = C->cfg.eps, w = C->weight;
+    C->stage_sweeps++;
+    scalar_sweep(C);
+    double ra = 0.0, m1a = 0.0, m2a = 0.0, ea = 0.0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            ra = ra + w * U4(u, 0, i, j);
+            m1a = m1a + w * U4(u, 1, i, j);
+            m2a = m2a + w * U4(u, 2, i, j);
+            ea = ea + w * U4(u, 3, i, j);
+        }
+    double ub[4] = {ra, m1a, m2a, ea};
+    if (C->cfg.pbar_check) {
+        if (g1 * (ub[3] - (0.5 * (ub[1] * ub[1] + ub[2] * ub[2])) / ub[0]) <= 0.) return SWO_PBAR_NONPOSITIVE;
+    }
+    C->sweep_count++;
+    C->t_pos_count = 0;
+    C->t_neg_count = 0;
+    track_push(C, 0, 0, 0);
+    for (int q = 1; q <= C->cfg.swp_num; ++q) {
+        if (q % 2 == 1) {
+            sweep_pass(C, C->ordI_i, C->ordI_j, 1, ub);
+            sweep_pass(C, C->ordI_i, C->ordI_j, 0, ub);
+        } else {
+            sweep_pass(C, C->ordII_i, C->ordII_j, 1, ub);
+            sweep_pass(C, C->ordII_i, C->ordII_j, 0, ub);
+        }
+        double rho_min = 1e20, p_min = 1e20;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                double v[4];
+                load4(C, u, i, j, v);
+                p_min = fmin2(sweep_pressure(g1, v), p_min);
+                rho_min = fmin2(v[0], rho_min);
+            }
+        C->pre_count++;
+        int ok = C->cfg.success_ge ? (p_min >= eps && rho_min >= eps) : (p_min > eps && rho_min > eps);
+        if (ok) {
+            track_push(C, q, C->t_neg_count, C->t_pos_count);
+            return SWO_OK;
+        }
+        if (q == C->cfg.swp_num) {
+            if (C->cfg.fail_stop) return SWO_SWEEP_FAIL;
+        }
+    }
+    return SWO_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* rk (ex4_sedov_2d/main.f90:685-777)                                         */
+/* ------------------------------------------------------------------------- */
+
+static int post_stage(swo_ctx *C) {
+    int rc = SWO_OK;
+    if (swo_check_positivity(C) && C->cfg.use_limiter == 2) rc = swo_sweep(C);
+    if (rc != SWO_OK) return rc;
+    swo_bc(C);
+    return SWO_OK;
+}
+
+int swo_step(swo_ctx *C, double time, double tfinal, double *dt_out) {
+    const size_t n = (size_t)4 * C->nxg * C->nyg;
+    double *u = C->u, *u0 = C->u0, *rhs = C->rhs;
+    double am = swo_alpha_max(C);
+    wenofd(C, am);
+    double dt;
+    if (C->cfg.dt_rule_dx53) {
+        dt = (pow(C->cfg.dx, 5. / 3.) * C->cfg.cfl) / am;
+    } else {
+        double dtx = (C->cfg.dx * C->cfg.cfl) / am;
+        double dty = (C->cfg.dy * C->cfg.cfl) / am;
+        dt = fmin2(fmin2(dtx, dty), tfinal - time);
+    }
+    if (dt_out) *dt_out = dt;
+    memcpy(u0, u, n * sizeof(double));
+    scheme(C, am);
+    for (size_t k = 0; k < n; ++k) u[k] = u[k] + dt * rhs[k];
+    int rc = post_stage(C);
+    if (rc) return rc;
+
+    am = swo_alpha_max(C);
+    wenofd(C, am);
+    scheme(C, am);
+    for (size_t k = 0; k < n; ++k) {
+        double x = u[k] + dt * rhs[k];
+        u[k] = D(Q(0.75) * Q(u0[k]) + Q(0.25) * Q(x));
+    }
+    rc = post_stage(C);
+    if (rc) return rc;
+
+    am = swo_alpha_max(C);
+    wenofd(C, am);
+    scheme(C, am);
+    for (size_t k = 0; k < n; ++k) {
+        double x = u[k] + dt * rhs[k];
+        u[k] = D((Q(u0[k]) + Q(2) * Q(x)) / Q(3));
+    }
+    return post_stage(C);
+}
+
+int swo_time_continue(double time, double tfinal) { return Q(time) <= Q(tfinal) - 1e-10Q; }
+
+void swo_get_stats(const swo_ctx *C, swo_stats *s) {
+    memset(s, 0, sizeof(*s));
+    s->sweep_count = C->sweep_count;
+    s->den_count = C->den_count;
+    s->pre_count = C->pre_count;
+    s->stage_sweeps = C->stage_sweeps;
+    for (int64_t k = 0; k < C->sweep_count; ++k) {
+        s->total_sweep += C->trk_q[k];
+        s->total_neg += C->trk_neg[k];
+        s->total_pos += C->trk_pos[k];
+        if (C->trk_q[k] > s->max_sweep) s->max_sweep = C->trk_q[k];
+        if (C->trk_neg[k] > s->max_neg) s->max_neg = C->trk_neg[k];
+        if (C->trk_pos[k] > s->max_pos) s->max_pos = C->trk_pos[k];
+    }
+}
+
+int64_t swo_track(const swo_ctx *C, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap) {
+    int64_t n = C->sweep_count < cap ? C->sweep_count : cap;
+    for (int64_t k = 0; k < n; ++k) {
+        q[k] = C->trk_q[k];
+        neg[k] = C->trk_neg[k];
+        pos[k] = C->trk_pos[k];
+    }
+    return C->sweep_count;
+}
+
+/* ------------------------------------------------------------------------- */
+/* initial data                                                               */
+/* ------------------------------------------------------------------------- */
+
+#define UI(c, i, j) u[(c) + 4 * (((i) + 2) + (cfg->nx + 6) * ((j) + 2))]
+
+/* ex4_sedov_2d/main.f90:1121-1189 at cut_x = cut_y = 1 (ghosts filled by swo_bc) */
+void swo_init_sedov(const swo_config *cfg, double e_corner_num, double e_amb, double *u) {
+    const double dx = cfg->dx, dy = cfg->dy;
+    for (int j = -2; j <= cfg->ny + 3; ++j)
+        for (int i = -2; i <= cfg->nx + 3; ++i) {
+            UI(0, i, j) = 1.;
+            UI(1, i, j) = 0.;
+            UI(2, i, j) = 0.;
+            UI(3, i, j) = e_amb;
+        }
+    for (int j = 1; j <= cfg->ny; ++j)
+        for (int i = 1; i <= cfg->nx; ++i) {
+            double gx = cfg->xlo + dx * (i - 0.5);
+            double gy = cfg->ylo + dy * (j - 0.5);
+            if (fabs(gx - 0.0) <= dx / 2.0 && fabs(gy - 0.0) <= dy / 2.0) UI(3, i, j) = e_corner_num / (dx * dy);
+        }
+}
+
+/* ex3_vortex_accuracy_2d/main.f90:699-718 */
+static void vortex_exact(double gamma, double mag, double x, double y, double t, double e[4]) {
+    const double g1 = gamma - 1.;
+    const double pi = 3.14159265358979323846264338327950;
+    double r = sqrt(((x - 5.) - t) * ((x - 5.) - t) + ((y - 5.) - t) * ((y - 5.) - t));
+    e[0] = pow(1. - ((((g1 * (mag * mag)) / 8.) / gamma) / (pi * pi)) * exp(1. - r * r), 1. / g1);
+    e[1] = e[0] * (1. - (((mag / 2.) / pi) * exp((1. - r * r) / 2.)) * (y - 5.));
+    e[2] = e[0] * (1. + (((mag / 2.) / pi) * exp((1. - r * r) / 2.)) * (x - 5.));
+    e[3] = (pow(e[0], gamma) / g1 + (0.5 * (e[1] * e[1])) / e[0]) + (0.5 * (e[2] * e[2])) / e[0];
+}
+
+void swo_vortex_exact(const swo_config *cfg, double mag, double t, double *u) {
+    for (int j = -2; j <= cfg->ny + 3; ++j)
+        for (int i = -2; i <= cfg->nx + 3; ++i) {
+            double x = cfg->xlo + i * cfg->dx - cfg->dx / 2.;
+            double y = cfg->ylo + j * cfg->dy - cfg->dy / 2.;
+            double e[4];
+            vortex_exact(cfg->gamma, mag, x, y, t, e);
+            for (int c = 0; c < 4; ++c) UI(c, i, j) = e[c];
+        }
+}
+
+void swo_init_vortex(const swo_config *cfg, double mag, double t, double *u) {
+    swo_vortex_exact(cfg, mag, t, u);
+}
+
+/* ex5_mach2000_jet/main.f90:743-768 (ghosts filled by swo_bc) */
+void swo_init_jet(const swo_config *cfg, double *u) {
+    const double g1 = cfg->gamma - 1.;
+    for (int j = -2; j <= cfg->ny + 3; ++j)
+        for (int i = -2; i <= cfg->nx + 3; ++i) {
+            UI(0, i, j) = cfg->amb_rho;
+            UI(1, i, j) = 0.;
+            UI(2, i, j) = 0.;
+            UI(3, i, j) = cfg->amb_p / g1;
+        }
+}

@@ -1,0 +1,152 @@
+"""Writes the golden fixtures under tests/golden/ from the reference programs.
+
+Runs in the build container only (it needs /root/reference and flang): the
+reference is compiled unmodified by ``oracle/Makefile`` (target ``ref``) into
+oracle/_ref/, run there, and only its OUTPUTS are stored here:
+
+  ref_weno.npz            weno_right / weno_left (ex4_sedov_2d/weno5.f90:86-146) on
+                          seeded adversarial 5-point stencils, via oracle/ref_harness.
+  ref_ex3_vortex180.json  ex3_vortex_accuracy_2d as shipped (180^2, T = 0.01, sweep on):
+                          SHA-256 of the final rho/m1/m2/p fields, a strided sample,
+                          the printed L1/Linf errors and the sweep statistics.
+  ref_ex5_jet.json        ex5_mach2000_jet as shipped (800x200, T = 0.001), when its
+                          (hour-long) run has finished: log-field hashes + statistics.
+
+Usage:  python tests/golden/make_golden.py [weno] [ex3] [ex5]
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+GOLD = os.path.join(ROOT, "tests", "golden")
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+
+def weno_stencils(seed: int = 20261203, n: int = 1000) -> np.ndarray:
+    """Seeded adversarial stencils a(-2..2)."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.standard_normal((n, 5))]
+    b = rng.standard_normal((n, 1)) * 10.0 ** rng.integers(-20, 20, (n, 1))
+    parts.append(b + b * rng.standard_normal((n, 5)) * 10.0 ** rng.integers(-17, -8, (n, 1)))
+    parts.append(rng.standard_normal((n, 5)) * 10.0 ** rng.integers(-30, 30, (n, 5)))
+    x = np.arange(-2, 3)
+    c = rng.standard_normal((n, 3))
+    parts.append(c[:, :1] + c[:, 1:2] * x + c[:, 2:3] * x ** 2)
+    jump = rng.standard_normal((n, 5))
+    jump[:, 3:] += 1e6
+    parts.append(jump)
+    parts.append(np.round(rng.standard_normal((n, 5)) * 8) / 8)
+    parts.append(np.zeros((8, 5)))
+    return np.ascontiguousarray(np.concatenate(parts))
+
+
+def make_weno() -> None:
+    a = weno_stencils()
+    fin, fout = "/tmp/swf_wp_in.bin", "/tmp/swf_wp_out.bin"
+    a.tofile(fin)
+    subprocess.run([os.path.join(REF, "weno_probe"), fin, fout], check=True)
+    r = np.fromfile(fout).reshape(-1, 2)
+    np.savez_compressed(os.path.join(GOLD, "ref_weno.npz"), a=a, right=r[:, 0], left=r[:, 1])
+
+
+def _tokens(fn: str, ncol: int) -> np.ndarray:
+    return np.array(open(fn).read().split(), dtype=np.float64).reshape(-1, ncol)
+
+
+def field_digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def _parse_stats(text: str) -> dict:
+    out = {}
+    for line in text.splitlines():
+        s = line.strip()
+        if "=" in s and not s.startswith("L1"):
+            k, v = s.split("=", 1)
+            k = k.strip()
+            if k in ("total sweep", "max sweep", "total negative pairs", "max negative pairs",
+                     "total positive neighbors", "max positive neighbors"):
+                out[k] = int(v)
+        if s.startswith("# of limiter"):
+            d, p = s.split()[-2:]
+            out["den_count"], out["pre_count"] = int(d), int(p)
+    return out
+
+
+def _run(prog: str, rundir: str) -> str:
+    os.makedirs(rundir, exist_ok=True)
+    with open(os.path.join(rundir, "fort.7"), "w") as f:
+        f.write("%13.5e%13.5e\n" % (1.0, 1.0))
+    open(os.path.join(rundir, "err_table.dat"), "w").close()
+    res = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + prog], cwd=rundir, check=True,
+                         capture_output=True, text=True)
+    with open(os.path.join(rundir, "stdout.txt"), "w") as f:
+        f.write(res.stdout)
+    return res.stdout
+
+
+def make_ex3() -> None:
+    rundir = os.path.join(REF, "run_ex3")
+    text = _run(os.path.join(REF, "ex3", "a"), rundir)
+    den = _tokens(os.path.join(rundir, "den.dat"), 5)
+    m1 = _tokens(os.path.join(rundir, "m1.dat"), 3)
+    m2 = _tokens(os.path.join(rundir, "m2.dat"), 3)
+    pre = _tokens(os.path.join(rundir, "pre.dat"), 3)
+    lines = text.splitlines()
+    l1, linf = (float(v) for v in lines[1].split())
+    fields = {"rho": den[:, 2], "m1": m1[:, 2], "m2": m2[:, 2], "p": pre[:, 2]}
+    out = {
+        "program": "ex3_vortex_accuracy_2d (unmodified), nx=ny=180, T=0.01, cfl=0.5, mag=10.0828",
+        "layout": "fields in (j outer, i inner) order over the 180x180 interior",
+        "sha256": {k: field_digest(v) for k, v in fields.items()},
+        "sample_stride": 37,
+        "sample": {k: v[::37].tolist() for k, v in fields.items()},
+        "exact_rho_sample": den[::37, 3].tolist(),
+        "L1": l1, "Linf": linf,
+        "stats": _parse_stats(text),
+    }
+    with open(os.path.join(GOLD, "ref_ex3_vortex180.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+def make_ex5() -> None:
+    rundir = os.path.join(REF, "run_ex5")
+    if not os.path.exists(os.path.join(rundir, "DONE")):
+        print("ex5 run not finished; start it with: cd oracle/_ref/run_ex5 && ../ex5/a")
+        return
+    text = open(os.path.join(rundir, "stdout.txt")).read()
+    nx, ny = 800, 200
+    fields = {}
+    for name, fn in (("log_rho", "den.dat"), ("log_p", "pre.dat"), ("log_p_over_rho", "m1.dat")):
+        lines = open(os.path.join(rundir, fn)).read().split("\n", 3)
+        body = lines[3] if fn == "den.dat" else open(os.path.join(rundir, fn)).read().split("\n", 1)[1]
+        t = np.array(body.split(), dtype=np.float64).reshape(-1, 3)
+        fields[name] = t[ny * nx:, 2]  # second (unmirrored) half: j = 1..ny, i = 1..nx
+    out = {
+        "program": "ex5_mach2000_jet (unmodified), 800x200 half domain, T=0.001, cfl=0.25",
+        "layout": "log fields in (j outer, i inner) order over the 800x200 interior",
+        "sha256": {k: field_digest(v) for k, v in fields.items()},
+        "sample_stride": 101,
+        "sample": {k: v[::101].tolist() for k, v in fields.items()},
+        "stats": _parse_stats(text),
+    }
+    with open(os.path.join(GOLD, "ref_ex5_jet.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["weno", "ex3", "ex5"]
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    if "weno" in what:
+        make_weno()
+    if "ex3" in what:
+        make_ex3()
+    if "ex5" in what:
+        make_ex5()

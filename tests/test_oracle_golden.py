@@ -1,0 +1,66 @@
+"""The oracle against the reference's own outputs (fixtures from tests/golden/make_golden.py).
+
+- ref_weno.npz: ex4_sedov_2d/weno5.f90 weno_right/weno_left, compiled unmodified with
+  flang -O3 -fdefault-real-8, on seeded adversarial stencils -> bitwise.
+- ref_ex3_vortex180.json: ex3_vortex_accuracy_2d as shipped (180^2 periodic vortex,
+  T = 0.01, sweeping limiter on) -> final rho/m1/m2/p bitwise (SHA-256) and the
+  printed sweep statistics and error norms exactly.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_weno_matches_reference_bitwise():
+    g = np.load(os.path.join(GOLD, "ref_weno.npz"))
+    r = O.weno_right(g["a"])
+    l = O.weno_left(g["a"])
+    assert np.array_equal(r.view(np.int64), g["right"].view(np.int64))
+    assert np.array_equal(l.view(np.int64), g["left"].view(np.int64))
+
+
+def _digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.slow
+def test_ex3_vortex_run_matches_reference_bitwise():
+    g = json.load(open(os.path.join(GOLD, "ref_ex3_vortex180.json")))
+    cfg = O.vortex_config(180)
+    o = O.Oracle(cfg)
+    o.set_state(O.init_vortex(cfg))
+    o.bc()
+    rc, t, steps = o.run(0.0, 0.01)
+    assert rc == 0 and steps == 9
+    U = o.state()[3:-3, 3:-3, :]
+    rho, m1, m2, e = (U[:, :, c].reshape(-1) for c in range(4))
+    g1 = 1.4 - 1.0
+    p = g1 * (e - 0.5 * m1 ** 2 / rho - 0.5 * m2 ** 2 / rho)
+    got = {"rho": rho, "m1": m1, "m2": m2, "p": p}
+    for k in got:
+        assert _digest(got[k]) == g["sha256"][k], k
+    s = o.stats()
+    st = g["stats"]
+    assert s["total_sweep"] == st["total sweep"] and s["max_sweep"] == st["max sweep"]
+    assert s["total_neg"] == st["total negative pairs"] and s["max_neg"] == st["max negative pairs"]
+    assert s["total_pos"] == st["total positive neighbors"] and s["max_pos"] == st["max positive neighbors"]
+    assert s["den_count"] == st["den_count"] and s["pre_count"] == st["pre_count"]
+    # error norms exactly as ex3 `output` forms them (:725-741)
+    ex = O.vortex_exact(cfg, t)[3:-3, 3:-3, 0]
+    er1 = 0.0
+    eri = 0.0
+    R = U[:, :, 0]
+    for i in range(180):
+        for j in range(180):
+            d = abs(ex[j, i] - R[j, i])
+            er1 = er1 + d
+            eri = max(eri, d)
+    er1 = er1 / 180 / 180
+    assert er1 == g["L1"] and eri == g["Linf"]
