@@ -1,0 +1,138 @@
+/*
+ * swf.h - C-ABI of the MI355X sweeping FD-WENO library (libswf.so).
+ *
+ * Drop-in boundary for the reference's time-stepping hot path.  The reference
+ * (dcgriffi/Sweeping_FD_WENO) has no library API: its "operators" are argument-less
+ * internal procedures of each example PROGRAM that share host-associated arrays.
+ * Each entry point below replaces one of them (ex4_sedov_2d/main.f90 unless noted):
+ *
+ *   swf_create / swf_destroy   the PARAMETER block and array declarations (:15-67)
+ *   swf_upload / swf_download  the global state array u(4,-2:nx+3,-2:ny+3) (:21)
+ *   swf_step                   one call of `rk` (:685-777) inside the time loop (:112-117)
+ *   swf_advance                the time loop `DO WHILE (time <= time_terminal - 1d-10)` (:112)
+ *   swf_alpha_max              `findalpha` (:652-683)
+ *   swf_rhs                    `wenofd` + `scheme` (:179-650)
+ *   swf_check_positivity       `checkpositivity` (:779-801)
+ *   swf_sweep                  `sweep_lim` incl. `scalar_sweep` (:1401-1833)
+ *   swf_bc                     `bc` (:803-996, physical part)
+ *   swf_stats / swf_track      the statistics printed by `output` (:1326-1351)
+ *   swf_weno_points            weno_right / weno_left (ex4_sedov_2d/weno5.f90:86-146)
+ *   swf_time_continue          the binary128 loop test of :112
+ *
+ * Arrays crossing the boundary are host memory in the reference's Fortran layout
+ * u(4, -2:nx+3, -2:ny+3): column-major, component fastest.  The caller owns them;
+ * the context owns its device buffers.  All calls on one context are ordered on
+ * the context's private HIP stream; swf_download, swf_stats and every call that
+ * returns a host value synchronize it.  One context per host thread.
+ *
+ * Return codes: 0 OK; > 0 numerical outcomes of the reference's `stop` paths
+ * (SWF_E_PBAR, SWF_E_SWEEP_FAIL); < 0 usage or HIP errors.  swf_strerror()
+ * returns a static description.
+ */
+#ifndef SWF_H
+#define SWF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision modes */
+#define SWF_FP64_EXACT 0 /* bit-identical to the reference (binary128 sites emulated) */
+#define SWF_FP64 1       /* plain binary64 */
+#define SWF_FP32_BULK 2  /* FP32 smoothness indicators / weights, FP64 state and sweep */
+
+/* boundary conditions */
+#define SWF_BC_SEDOV 0    /* ex4: reflective x=0, y=0; zero-gradient x=xm, y=ym */
+#define SWF_BC_PERIODIC 1 /* ex3 */
+#define SWF_BC_JET_HALF 2 /* ex5: inflow at x=0 for y<=w, reflective y=0, zero-gradient elsewhere */
+#define SWF_BC_JET_FULL 3 /* full-domain jet: inflow at x=0 for |y|<w, zero-gradient elsewhere */
+
+/* return codes */
+#define SWF_OK 0
+#define SWF_E_PBAR 1        /* "ERROR: P(u_bar) <= 0." (ex4_sedov_2d/main.f90:1447-1454) */
+#define SWF_E_SWEEP_FAIL 2  /* "SWEEP FAIL: swp_num =" (ex4_sedov_2d/main.f90:1747-1752) */
+#define SWF_E_INVALID (-1)
+#define SWF_E_HIP (-2)
+#define SWF_E_NOMEM (-3)
+#define SWF_E_NODEVICE (-4)
+
+typedef struct swf_config {
+  int32_t nx, ny;           /* interior cells (total_x, total_y) */
+  double xlo, ylo, dx, dy;  /* grid origin and spacing as the driver computed them */
+  double gamma;             /* gamma1 = gamma - 1 is formed in binary64 */
+  double cfl;
+  double eps;               /* positivity threshold, 1d-13 */
+  double eps2;              /* velocity cut-off, 1d-8 */
+  double delta;             /* density sweep threshold, 1d-12 */
+  int32_t bc;               /* SWF_BC_* */
+  int32_t use_limiter;      /* 2 = sweeping post-process, 0 = none */
+  int32_t swp_num;          /* sweep iteration cap */
+  /* variant switches: 0 reproduces ex4_sedov_2d; the ex3/ex5 programs set: */
+  int32_t alpha_in_scheme;   /* ex3/ex5: scheme multiplies by alphamax */
+  int32_t roe_eps_double;    /* ex3/ex5: Roe weight / sound speed use 1.e-15 (binary64) */
+  int32_t ydens_plain;       /* ex3/ex5: y-pass 1./den instead of 1./(den+1d-15) */
+  int32_t eps2_quad_literal; /* ex3/ex5: fflux tests rho < 1d-8 (binary128 literal) */
+  int32_t findalpha_ex3;     /* ex3 findalpha expression */
+  int32_t dt_rule_dx53;      /* ex3: dt = dx**(5/3)*cfl/alpha, no clamp to time_terminal */
+  int32_t poscheck_le;       /* ex3: checkpositivity with <= */
+  int32_t success_ge;        /* ex5: sweep success with >= */
+  int32_t pbar_check;        /* ex4: stop on P(u_bar) <= 0 */
+  int32_t fail_stop;         /* ex4: SWEEP FAIL stops (else print and continue) */
+  double jet_rho, jet_u, amb_rho, amb_p, jet_halfwidth; /* jet boundary data */
+  int32_t precision;         /* SWF_FP64_EXACT / SWF_FP64 / SWF_FP32_BULK */
+  int32_t device;            /* HIP device ordinal */
+} swf_config;
+
+typedef struct swf_sweep_stats {
+  int64_t sweep_count, den_count, pre_count;
+  int64_t total_sweep, max_sweep, total_neg, max_neg, total_pos, max_pos;
+  int64_t stage_sweeps; /* RK stages in which the sweep ran */
+  int64_t steps;        /* steps advanced on this context */
+} swf_sweep_stats;
+
+typedef struct swf_ctx swf_ctx;
+
+int swf_create(const swf_config *cfg, swf_ctx **out);
+void swf_destroy(swf_ctx *ctx);
+
+int swf_upload(swf_ctx *ctx, const double *u);
+int swf_download(swf_ctx *ctx, double *u);
+
+/* One SSP-RK3 step at `time` (dt decided on the device); dt_out may be NULL. */
+int swf_step(swf_ctx *ctx, double time, double tfinal, double *dt_out);
+/* Up to max_steps steps while time <= tfinal - 1d-10; *time_inout advances,
+ * *steps_out receives the number of steps taken.  Steps are enqueued without
+ * host synchronisation (device-side dt and sweep gates). */
+int swf_advance(swf_ctx *ctx, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out);
+
+int swf_bc(swf_ctx *ctx);
+int swf_alpha_max(swf_ctx *ctx, double *alpha_out);
+int swf_rhs(swf_ctx *ctx, double alphamax, double *rhs_out);
+int swf_check_positivity(swf_ctx *ctx, int32_t *postag_out);
+int swf_sweep(swf_ctx *ctx);
+
+int swf_stats(swf_ctx *ctx, swf_sweep_stats *out);
+/* per-trigger (q, negative pairs, positive neighbours); returns sweep_count */
+int64_t swf_track(swf_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap);
+
+/* weno_right / weno_left of n host stencils a(-2..2) on the device */
+int swf_weno_points(const double *a, double *right, double *left, int64_t n, int32_t precision,
+                    int32_t device);
+
+/* Average duration (ms) of one launch of a hot kernel on the context's stream,
+ * measured with HIP events over `nreps` back-to-back launches on the current state
+ * (which: 0 = x-interface flux, 1 = y-interface flux).  No counterpart in the
+ * reference: measurement support for the x/y halves of `wenofd`. */
+int swf_time_kernel(swf_ctx *ctx, int32_t which, int32_t nreps, double *avg_ms);
+
+int swf_time_continue(double time, double tfinal);
+int swf_sync(swf_ctx *ctx);
+const char *swf_strerror(int code);
+/* last HIP error text of this thread ("" if none) */
+const char *swf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

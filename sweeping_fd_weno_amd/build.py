@@ -1,0 +1,88 @@
+"""Builds libswf.so (HIP, gfx950) and the Fortran driver in-tree.
+
+    python -m sweeping_fd_weno_amd.build          # library + driver
+Every HIP translation unit is compiled for gfx950 with packed-FP32 instructions
+disabled.  The FP64-exact and mode-independent kernels (swf_k_exact.hip) and the
+host API are compiled without floating-point contraction, the FP64/FP32-bulk
+kernels (swf_k_fast.hip) with contraction.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+ROOT = os.path.dirname(PKG)
+LIB = os.path.join(PKG, "libswf.so")
+OBJ = os.path.join(PKG, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+ARCH = os.environ.get("SWF_OFFLOAD_ARCH", "gfx950")
+
+COMMON = [
+    "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fno-fast-math", "-I", os.path.join(ROOT, "include"), "-Wno-unused-result",
+]
+UNITS = [
+    ("swf_k_exact.hip", ["-ffp-contract=off"]),
+    ("swf_k_fast.hip", ["-ffp-contract=fast"]),
+    ("swf_api.hip", ["-ffp-contract=off"]),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + os.path.basename(cmd[-1]))
+    return r
+
+
+def build_library(verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    procs = []
+    for src, extra in UNITS:
+        obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        cmd = [HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True), cmd))
+        objs.append(obj)
+    for p, cmd in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(" ".join(cmd) + "\n" + out + err)
+            raise RuntimeError("hipcc failed on " + cmd[-3])
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_driver() -> str | None:
+    """The Fortran driver (reference parameter surface) linked against libswf.so."""
+    drv_dir = os.path.join(PKG, "fortran")
+    srcs = [os.path.join(drv_dir, "swf_mod.f90"), os.path.join(drv_dir, "swf_driver.f90")]
+    if not all(os.path.exists(s) for s in srcs) or not os.path.exists(FLANG):
+        return None
+    out = os.path.join(drv_dir, "swf_driver")
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    _run([FLANG, "-O2", "-fdefault-real-8", "-module-dir", mod, *srcs, "-o", out,
+          "-L" + PKG, "-lswf", "-Wl,-rpath,$ORIGIN/.."])
+    return out
+
+
+def build(verbose: bool = False) -> None:
+    build_library(verbose)
+    build_driver()
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

@@ -1,0 +1,458 @@
+// swf_api.hip - the C-ABI of libswf.so (include/swf.h): solver contexts, state
+// transfer, stage sequencing and statistics.  Each RK stage is a fixed sequence
+// of kernel launches on the context's stream; the time-step, sweep and step
+// gates live on the device, so steps are enqueued without host synchronisation.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "../../include/swf.h"
+#include "swf_launch.h"
+
+using namespace swf;
+
+namespace {
+
+thread_local char g_last_error[256] = "";
+
+int hip_fail(hipError_t e, const char *what) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+  return SWF_E_HIP;
+}
+
+#define HIPC(call)                                   \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+constexpr int kTrackCap = 1 << 18;
+
+}  // namespace
+
+struct swf_ctx {
+  swf_config cfg;
+  KCfg k;
+  Bufs b;
+  int mode;
+  hipStream_t stream;
+  double *d_fortran;   // staging buffer in the Fortran layout
+  size_t n_fortran;    // doubles in u(4,-2:nx+3,-2:ny+3)
+  long long steps_total;
+};
+
+static int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, what);
+  return SWF_OK;
+}
+
+static int set_device(const swf_ctx *c) {
+  HIPC(hipSetDevice(c->cfg.device));
+  return SWF_OK;
+}
+
+extern "C" {
+
+const char *swf_last_error(void) { return g_last_error; }
+
+const char *swf_strerror(int code) {
+  switch (code) {
+    case SWF_OK: return "ok";
+    case SWF_E_PBAR: return "ERROR: P(u_bar) <= 0.";
+    case SWF_E_SWEEP_FAIL: return "SWEEP FAIL: swp_num reached";
+    case SWF_E_INVALID: return "invalid argument";
+    case SWF_E_HIP: return "HIP runtime error";
+    case SWF_E_NOMEM: return "out of device memory";
+    case SWF_E_NODEVICE: return "no HIP device";
+    default: return "unknown error";
+  }
+}
+
+int swf_time_continue(double time, double tfinal) { return time_continue_q(time, tfinal); }
+
+int swf_create(const swf_config *cfg, swf_ctx **out) {
+  if (!cfg || !out) return SWF_E_INVALID;
+  *out = nullptr;
+  if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->dx > 0) || !(cfg->dy > 0) || cfg->precision < 0 || cfg->precision > 2 ||
+      cfg->bc < 0 || cfg->bc > 3 || (cfg->use_limiter == 2 && cfg->swp_num < 1))
+    return SWF_E_INVALID;
+  if ((long long)cfg->nx * cfg->ny >= (1LL << 31) - 64) return SWF_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
+    snprintf(g_last_error, sizeof(g_last_error), "no HIP device %d (found %d)", cfg->device, ndev);
+    return SWF_E_NODEVICE;
+  }
+  swf_ctx *c = new (std::nothrow) swf_ctx();
+  if (!c) return SWF_E_NOMEM;
+  c->cfg = *cfg;
+  c->mode = cfg->precision;
+  KCfg &k = c->k;
+  k.nx = cfg->nx;
+  k.ny = cfg->ny;
+  k.P = ((cfg->nx + 6) + 15) / 16 * 16;
+  k.plane = (long long)(cfg->ny + 6) * k.P;
+  k.nwords = (int)(((long long)cfg->nx * cfg->ny + 31) / 32);
+  k.dx = cfg->dx;
+  k.dy = cfg->dy;
+  k.gamma = cfg->gamma;
+  k.gamma1 = cfg->gamma - 1.;
+  k.cfl = cfg->cfl;
+  k.eps = cfg->eps;
+  k.eps2 = cfg->eps2;
+  k.delta = cfg->delta;
+  k.weight = 1. / ((double)cfg->nx * (double)cfg->ny);
+  // ex3: dt = dx**(5./3.)*cfl/alphamax, the numerator being a run constant
+  k.dt_num = pow(cfg->dx, 5. / 3.) * cfg->cfl;
+  k.xlo = cfg->xlo;
+  k.ylo = cfg->ylo;
+  k.jet_rho = cfg->jet_rho;
+  k.jet_mx = cfg->jet_rho * cfg->jet_u;
+  k.amb_rho = cfg->amb_rho;
+  k.amb_e = cfg->amb_p / k.gamma1;
+  // ex5_mach2000_jet/main.f90:711: 0.4127/gamma1 + 0.5*u(2)**2/u(1)
+  k.jet_e = k.amb_e + (0.5 * (k.jet_mx * k.jet_mx)) / k.jet_rho;
+  k.jet_halfwidth = cfg->jet_halfwidth;
+  k.bc = cfg->bc;
+  k.use_limiter = cfg->use_limiter;
+  k.swp_num = cfg->swp_num;
+  k.track_cap = kTrackCap;
+  k.alpha_in_scheme = cfg->alpha_in_scheme;
+  k.roe_eps_double = cfg->roe_eps_double;
+  k.ydens_plain = cfg->ydens_plain;
+  k.eps2_quad_literal = cfg->eps2_quad_literal;
+  k.findalpha_ex3 = cfg->findalpha_ex3;
+  k.dt_rule_dx53 = cfg->dt_rule_dx53;
+  k.poscheck_le = cfg->poscheck_le;
+  k.success_ge = cfg->success_ge;
+  k.pbar_check = cfg->pbar_check;
+  k.fail_stop = cfg->fail_stop;
+  c->n_fortran = (size_t)4 * (cfg->nx + 6) * (cfg->ny + 6);
+
+  int rc = SWF_OK;
+  hipError_t e = hipSetDevice(cfg->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  const size_t pbytes = sizeof(double) * 4 * (size_t)k.plane;
+  double *pool = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&pool, 5 * pbytes);
+  unsigned *maps = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&maps, sizeof(unsigned) * 4 * (size_t)k.nwords);
+  DevScalars *g = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&g, sizeof(DevScalars));
+  long long *trk = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&trk, sizeof(long long) * 3 * (size_t)kTrackCap);
+  double *stage = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&stage, sizeof(double) * c->n_fortran);
+  if (e == hipSuccess) e = hipMemsetAsync(pool, 0, 5 * pbytes, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(maps, 0, sizeof(unsigned) * 4 * (size_t)k.nwords, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(g, 0, sizeof(DevScalars), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(trk, 0, sizeof(long long) * 3 * (size_t)kTrackCap, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    rc = (e == hipErrorOutOfMemory) ? SWF_E_NOMEM : hip_fail(e, "swf_create");
+    if (pool) (void)hipFree(pool);
+    if (maps) (void)hipFree(maps);
+    if (g) (void)hipFree(g);
+    if (trk) (void)hipFree(trk);
+    if (stage) (void)hipFree(stage);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
+  const size_t pd = 4 * (size_t)k.plane;
+  c->b.U = pool;
+  c->b.U0 = pool + pd;
+  c->b.Fx = pool + 2 * pd;
+  c->b.Fy = pool + 3 * pd;
+  c->b.R = pool + 4 * pd;
+  c->b.bmR = maps;
+  c->b.bmP = maps + k.nwords;
+  c->b.bmA = maps + 2 * (size_t)k.nwords;
+  c->b.bmB = maps + 3 * (size_t)k.nwords;
+  c->b.g = g;
+  c->b.trk = trk;
+  c->d_fortran = stage;
+  *out = c;
+  return SWF_OK;
+}
+
+void swf_destroy(swf_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->cfg.device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->b.U);
+  (void)hipFree(c->b.bmR);
+  (void)hipFree(c->b.g);
+  (void)hipFree(c->b.trk);
+  (void)hipFree(c->d_fortran);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int swf_upload(swf_ctx *c, const double *u) {
+  if (!c || !u) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  HIPC(hipMemcpyAsync(c->d_fortran, u, sizeof(double) * c->n_fortran, hipMemcpyHostToDevice, c->stream));
+  launch_from_fortran(c->k, c->d_fortran, c->b.U, c->stream);
+  launch_upload_reset(c->b, c->stream);
+  launch_alpha_all(c->k, c->b, c->stream, 0);
+  HIPC(hipStreamSynchronize(c->stream));
+  return check_launch("swf_upload");
+}
+
+int swf_download(swf_ctx *c, double *u) {
+  if (!c || !u) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  launch_to_fortran(c->k, c->b.U, c->d_fortran, c->stream);
+  HIPC(hipMemcpyAsync(u, c->d_fortran, sizeof(double) * c->n_fortran, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return check_launch("swf_download");
+}
+
+// one RK stage (s = 1, 2, 3) of one step with parity p
+static void enqueue_stage(swf_ctx *c, int s, int p) {
+  const KCfg &k = c->k;
+  const Bufs &b = c->b;
+  const int slot_in = s - 1, slot_out = s % 3;
+  launch_xflux(c->mode, k, b, c->stream, slot_in, slot_out, p, 1);
+  launch_yflux(c->mode, k, b, c->stream, slot_in, p);
+  launch_update(c->mode, k, b, c->stream, s, slot_in, slot_out, p, 0);
+  launch_sweep(k, b, c->stream, s, slot_out, p, c->mode == MODE_EXACT ? 1 : 0, 0);
+  launch_alpha_fix(k, b, c->stream, slot_out, p);
+  launch_bc(k, b, c->stream, slot_out, p);
+}
+
+static void enqueue_step(swf_ctx *c, int p) {
+  enqueue_stage(c, 1, p);
+  enqueue_stage(c, 2, p);
+  enqueue_stage(c, 3, p);
+}
+
+static int read_scalars(swf_ctx *c, DevScalars *h) {
+  HIPC(hipMemcpyAsync(h, c->b.g, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return SWF_OK;
+}
+
+int swf_step(swf_ctx *c, double time, double tfinal, double *dt_out) {
+  if (!c) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  launch_begin_run(c->b, c->stream, time, tfinal, 1);
+  enqueue_step(c, 0);
+  rc = check_launch("swf_step");
+  if (rc) return rc;
+  DevScalars h;
+  rc = read_scalars(c, &h);
+  if (rc) return rc;
+  c->steps_total += 1;
+  if (dt_out) *dt_out = h.dt;
+  return h.status;
+}
+
+int swf_advance(swf_ctx *c, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out) {
+  if (!c || !time_inout || max_steps < 0) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  launch_begin_run(c->b, c->stream, *time_inout, tfinal, time_continue_q(*time_inout, tfinal));
+  const int64_t batch = 64;
+  int64_t launched = 0;
+  DevScalars h;
+  while (launched < max_steps) {
+    const int64_t n = (max_steps - launched < batch) ? (max_steps - launched) : batch;
+    for (int64_t q = 0; q < n; ++q) enqueue_step(c, (int)((launched + q) & 1));
+    launched += n;
+    rc = check_launch("swf_advance");
+    if (rc) return rc;
+    rc = read_scalars(c, &h);
+    if (rc) return rc;
+    if (h.steps_done < launched || h.status != ST_OK) break;
+  }
+  rc = read_scalars(c, &h);
+  if (rc) return rc;
+  *time_inout = h.time[h.steps_done & 1];
+  if (steps_out) *steps_out = h.steps_done;
+  c->steps_total += h.steps_done;
+  return h.status;
+}
+
+int swf_bc(swf_ctx *c) {
+  if (!c) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  launch_bc(c->k, c->b, c->stream, -1, -1);
+  HIPC(hipStreamSynchronize(c->stream));
+  return check_launch("swf_bc");
+}
+
+int swf_alpha_max(swf_ctx *c, double *alpha_out) {
+  if (!c || !alpha_out) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  launch_set_alpha(c->b, c->stream, 3, 0ull, -1, 0);
+  launch_alpha_all(c->k, c->b, c->stream, 3);
+  rc = check_launch("swf_alpha_max");
+  if (rc) return rc;
+  DevScalars h;
+  rc = read_scalars(c, &h);
+  if (rc) return rc;
+  memcpy(alpha_out, &h.alpha_bits[3], sizeof(double));
+  return SWF_OK;
+}
+
+int swf_rhs(swf_ctx *c, double alphamax, double *rhs_out) {
+  if (!c || !rhs_out || !(alphamax > 0)) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  unsigned long long bits;
+  memcpy(&bits, &alphamax, sizeof(bits));
+  // save the stage-1 alpha slot and the gate, use slot 0 for the query
+  DevScalars h0;
+  rc = read_scalars(c, &h0);
+  if (rc) return rc;
+  launch_set_alpha(c->b, c->stream, 0, bits, 0, 1);
+  launch_xflux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
+  launch_yflux(c->mode, c->k, c->b, c->stream, 0, 0);
+  launch_update(c->mode, c->k, c->b, c->stream, 1, 0, 3, 0, 1);
+  launch_to_fortran(c->k, c->b.R, c->d_fortran, c->stream);
+  HIPC(hipMemcpyAsync(rhs_out, c->d_fortran, sizeof(double) * c->n_fortran, hipMemcpyDeviceToHost, c->stream));
+  launch_set_alpha(c->b, c->stream, 0, h0.alpha_bits[0], 0, h0.active[0]);
+  HIPC(hipStreamSynchronize(c->stream));
+  return check_launch("swf_rhs");
+}
+
+int swf_check_positivity(swf_ctx *c, int32_t *postag_out) {
+  if (!c || !postag_out) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  launch_flags(c->k, c->b, c->stream);
+  rc = check_launch("swf_check_positivity");
+  if (rc) return rc;
+  DevScalars h;
+  rc = read_scalars(c, &h);
+  if (rc) return rc;
+  *postag_out = h.postag;
+  return SWF_OK;
+}
+
+int swf_sweep(swf_ctx *c) {
+  if (!c) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  launch_flags(c->k, c->b, c->stream);
+  launch_sweep(c->k, c->b, c->stream, 0, -1, -1, c->mode == MODE_EXACT ? 1 : 0, 1);
+  rc = check_launch("swf_sweep");
+  if (rc) return rc;
+  DevScalars h;
+  rc = read_scalars(c, &h);
+  if (rc) return rc;
+  return h.status;
+}
+
+int swf_stats(swf_ctx *c, swf_sweep_stats *s) {
+  if (!c || !s) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  DevScalars h;
+  rc = read_scalars(c, &h);
+  if (rc) return rc;
+  s->sweep_count = h.sweep_count;
+  s->den_count = h.den_count;
+  s->pre_count = h.pre_count;
+  s->total_sweep = h.total_sweep;
+  s->max_sweep = h.max_sweep;
+  s->total_neg = h.total_neg;
+  s->max_neg = h.max_neg;
+  s->total_pos = h.total_pos;
+  s->max_pos = h.max_pos;
+  s->stage_sweeps = h.stage_sweeps;
+  s->steps = c->steps_total;
+  return SWF_OK;
+}
+
+int64_t swf_track(swf_ctx *c, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap) {
+  if (!c) return SWF_E_INVALID;
+  if (set_device(c)) return SWF_E_HIP;
+  DevScalars h;
+  if (read_scalars(c, &h)) return SWF_E_HIP;
+  const int64_t n = h.sweep_count < cap ? h.sweep_count : cap;
+  const int64_t m = n < kTrackCap ? n : kTrackCap;
+  if (m > 0 && q && neg && pos) {
+    if (hipMemcpy(q, c->b.trk, sizeof(int64_t) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(neg, c->b.trk + kTrackCap, sizeof(int64_t) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(pos, c->b.trk + 2 * kTrackCap, sizeof(int64_t) * m, hipMemcpyDeviceToHost) != hipSuccess)
+      return SWF_E_HIP;
+  }
+  return h.sweep_count;
+}
+
+int swf_time_kernel(swf_ctx *c, int32_t which, int32_t nreps, double *avg_ms) {
+  if (!c || !avg_ms || nreps < 1 || which < 0 || which > 1) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  // the flux kernels read U and alpha slot 0 and write only Fx / Fy
+  DevScalars h0;
+  rc = read_scalars(c, &h0);
+  if (rc) return rc;
+  launch_set_alpha(c->b, c->stream, 0, h0.alpha_bits[0], 0, 1);
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  // one untimed launch
+  if (which == 0)
+    launch_xflux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
+  else
+    launch_yflux(c->mode, c->k, c->b, c->stream, 0, 0);
+  HIPC(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < nreps; ++r) {
+    if (which == 0)
+      launch_xflux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
+    else
+      launch_yflux(c->mode, c->k, c->b, c->stream, 0, 0);
+  }
+  HIPC(hipEventRecord(e1, c->stream));
+  HIPC(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  launch_set_alpha(c->b, c->stream, 0, h0.alpha_bits[0], 0, h0.active[0]);
+  HIPC(hipStreamSynchronize(c->stream));
+  *avg_ms = (double)ms / nreps;
+  return check_launch("swf_time_kernel");
+}
+
+int swf_sync(swf_ctx *c) {
+  if (!c) return SWF_E_INVALID;
+  int rc = set_device(c);
+  if (rc) return rc;
+  HIPC(hipStreamSynchronize(c->stream));
+  return SWF_OK;
+}
+
+int swf_weno_points(const double *a, double *right, double *left, int64_t n, int32_t precision, int32_t device) {
+  if (!a || !right || !left || n < 0 || precision < 0 || precision > 2) return SWF_E_INVALID;
+  if (n == 0) return SWF_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SWF_E_NODEVICE;
+  HIPC(hipSetDevice(device));
+  double *d = nullptr;
+  HIPC(hipMalloc(&d, sizeof(double) * 7 * (size_t)n));
+  hipError_t e = hipMemcpy(d, a, sizeof(double) * 5 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_weno_points(precision, d, d + 5 * n, d + 6 * n, n, 0);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(right, d + 5 * n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(left, d + 6 * n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "swf_weno_points");
+  return SWF_OK;
+}
+
+}  // extern "C"

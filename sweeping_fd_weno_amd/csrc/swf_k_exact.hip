@@ -1,0 +1,135 @@
+// swf_k_exact.hip - FP64-exact flux/update kernels and the mode-independent
+// kernels (sweep, boundary conditions, alpha, layout transposes).  Compiled with
+// -ffp-contract=off: every product and sum is rounded separately, as in the
+// reference build (x86-64 SSE2 without FMA).
+#define SWF_COMMON_KERNELS 1
+#include "swf_kernels.h"
+#include "swf_launch.h"
+#include "swf_sweep.h"
+
+namespace swf {
+
+void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
+                     int clear_maps) {
+  hipLaunchKernelGGL(k_xflux<MODE_EXACT>, grid_xflux(k), dim3(64, 4), 0, s, k, b, slot_in, slot_reset, parity,
+                     clear_maps);
+}
+void launch_yflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
+  hipLaunchKernelGGL(k_yflux<MODE_EXACT>, grid_yflux(k), dim3(16, 16), 0, s, k, b, slot_in, parity);
+}
+void launch_update_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out, int parity,
+                      int rhs_only) {
+  hipLaunchKernelGGL(k_update<MODE_EXACT>, grid_cells(k), dim3(64, 4), 0, s, k, b, stage, slot_in, slot_out, parity,
+                     rhs_only);
+}
+void launch_weno_points_m0(const double *a, double *r, double *l, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_weno_points<MODE_EXACT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, r, l, n);
+}
+
+void launch_xflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
+                  int clear_maps) {
+  if (mode == MODE_EXACT)
+    launch_xflux_m0(k, b, s, slot_in, slot_reset, parity, clear_maps);
+  else
+    launch_xflux_m12(mode, k, b, s, slot_in, slot_reset, parity, clear_maps);
+}
+void launch_yflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
+  if (mode == MODE_EXACT)
+    launch_yflux_m0(k, b, s, slot_in, parity);
+  else
+    launch_yflux_m12(mode, k, b, s, slot_in, parity);
+}
+void launch_update(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out,
+                   int parity, int rhs_only) {
+  if (mode == MODE_EXACT)
+    launch_update_m0(k, b, s, stage, slot_in, slot_out, parity, rhs_only);
+  else
+    launch_update_m12(mode, k, b, s, stage, slot_in, slot_out, parity, rhs_only);
+}
+void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, hipStream_t s) {
+  if (mode == MODE_EXACT)
+    launch_weno_points_m0(a, r, l, n, s);
+  else
+    launch_weno_points_m12(mode, a, r, l, n, s);
+}
+
+// checkpositivity flag and sweep bitmaps of the current interior (no RK update)
+__global__ __launch_bounds__(256) void k_flags(KCfg k, Bufs b) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y;
+  if (i > k.nx || j > k.ny) return;
+  const long long o = cidx(k, i, j);
+  double v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = b.U[c * k.plane + o];
+  const double ke = (v[3] - ((v[2] * v[2]) / v[0]) / 2.) - ((v[1] * v[1]) / v[0]) / 2.;
+  const bool bad = k.poscheck_le ? (v[0] <= k.eps || ke <= k.eps) : (v[0] < k.eps || ke < k.eps);
+  if (bad) atomicOr(&b.g->postag, 1);
+  const int p = posI(k, i, j);
+  if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) atomicOr(&b.bmP[p >> 5], 1u << (p & 31));
+  if (v[0] < k.delta) {
+    atomicOr(&b.bmR[p >> 5], 1u << (p & 31));
+    atomicOr(&b.g->rho_any, 1);
+  }
+}
+
+__global__ void k_clear_flags(KCfg k, Bufs b) {
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < k.nwords; w += gridDim.x * blockDim.x) {
+    b.bmR[w] = 0u;
+    b.bmP[w] = 0u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    b.g->postag = 0;
+    b.g->rho_any = 0;
+  }
+}
+
+__global__ void k_begin_run(Bufs b, double time, double tfinal, int active) {
+  b.g->time[0] = time;
+  b.g->tfinal = tfinal;
+  b.g->active[0] = active;
+  b.g->steps_done = 0;
+}
+__global__ void k_set_alpha(Bufs b, int slot, unsigned long long bits, int p, int active) {
+  b.g->alpha_bits[slot] = bits;
+  if (p >= 0) b.g->active[p] = active;
+}
+__global__ void k_upload_reset(Bufs b) {
+  b.g->alpha_bits[0] = 0ull;
+  b.g->swept = 0;
+}
+void launch_begin_run(const Bufs &b, hipStream_t s, double time, double tfinal, int active) {
+  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, s, b, time, tfinal, active);
+}
+void launch_set_alpha(const Bufs &b, hipStream_t s, int slot, unsigned long long bits, int p, int active) {
+  hipLaunchKernelGGL(k_set_alpha, dim3(1), dim3(1), 0, s, b, slot, bits, p, active);
+}
+void launch_upload_reset(const Bufs &b, hipStream_t s) {
+  hipLaunchKernelGGL(k_upload_reset, dim3(1), dim3(1), 0, s, b);
+}
+
+void launch_flags(const KCfg &k, const Bufs &b, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_flags, dim3(64), dim3(256), 0, s, k, b);
+  hipLaunchKernelGGL(k_flags, grid_cells(k), dim3(64, 4), 0, s, k, b);
+}
+void launch_sweep(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_out, int parity, int serial_ubar,
+                  int force) {
+  hipLaunchKernelGGL(k_sweep, dim3(1), dim3(SW_T), 0, s, k, b, stage, slot_out, parity, serial_ubar, force);
+}
+void launch_alpha_fix(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out, int parity) {
+  hipLaunchKernelGGL(k_alpha_fix, grid_cells(k), dim3(64, 4), 0, s, k, b, slot_out, parity);
+}
+void launch_alpha_all(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out) {
+  hipLaunchKernelGGL(k_alpha_all, dim3((k.nx + 6 + 63) / 64, (k.ny + 6 + 3) / 4), dim3(64, 4), 0, s, k, b, slot_out);
+}
+void launch_bc(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out, int parity) {
+  const long long total = 2LL * 3 * (k.nx + 6) + 2LL * 3 * k.ny;
+  hipLaunchKernelGGL(k_bc, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, k, b, slot_out, parity);
+}
+void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_from_fortran, dim3(1024), dim3(256), 0, s, k, src, dst);
+}
+void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_fortran, dim3(1024), dim3(256), 0, s, k, src, dst);
+}
+
+}  // namespace swf

@@ -1,0 +1,485 @@
+// swf_sweep.h - the use_limiter = 2 sweeping post-process on the GPU, with the
+// reference's sequential semantics (ex4_sedov_2d/main.f90:1359-1396 nod_lim,
+// :1401-1761 sweep_lim, :1765-1833 scalar_sweep, :1836-1895 sweeping_order).
+//
+// The reference scans all nx*ny cells in snake order, forward then backward, and
+// adjusts a flagged cell against its successor (forward) or predecessor
+// (backward).  Only flagged cells and the cascades they start do work, so the
+// GPU version works on bitmaps of flagged snake positions:
+//   - the RK update kernel sets a bit for every cell flagged after the stage;
+//   - a pass walks the set bits in order.  A walk starting at a flagged position
+//     adjusts it, hands the difference to the neighbour, and continues while the
+//     neighbour is flagged; it stops at the first unflagged neighbour, which is
+//     exactly where the reference's loop resumes.  Walks are computed
+//     speculatively in parallel (reading unmodified cells only), then one thread
+//     picks, in order, the walks that the sequential loop would really start
+//     (those beginning after the previous real walk ended), and only those write.
+//   - after a forward pass only cells adjusted in it (and the last position) can
+//     still be flagged, so the walk itself records the next pass's bitmap; the
+//     convergence test (global min P and rho) reduces to "no cell left flagged".
+// Statistics (sweep_count, den_count, pre_count, per-trigger q / negative /
+// positive adjustment counts) are accumulated exactly as the reference does.
+// One workgroup of 1024 threads runs the whole sweep (all q iterations).
+#pragma once
+
+#include "swf_device.h"
+
+namespace swf {
+
+constexpr int SW_T = 1024;    // threads of the sweep workgroup
+constexpr int SW_WCH = 256;   // bitmap words per scan chunk (8192 positions)
+
+SWF_D void cell_of(const KCfg &k, int order, int p, int &i, int &j) {
+  if (order == 0)
+    cellI(k, p, i, j);
+  else
+    cellII(k, p, i, j);
+}
+SWF_D int pos_of(const KCfg &k, int order, int i, int j) { return order == 0 ? posI(k, i, j) : posII(k, i, j); }
+
+SWF_D void load_cell(const KCfg &k, const Bufs &b, int order, int p, double v[4]) {
+  int i, j;
+  cell_of(k, order, p, i, j);
+  const long long o = cidx(k, i, j);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = b.U[c * k.plane + o];
+}
+SWF_D void store_cell(const KCfg &k, const Bufs &b, int order, int p, const double v[4]) {
+  int i, j;
+  cell_of(k, order, p, i, j);
+  const long long o = cidx(k, i, j);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) b.U[c * k.plane + o] = v[c];
+}
+SWF_D void set_bit(unsigned *m, int p) { atomicOr(&m[p >> 5], 1u << (p & 31)); }
+SWF_D void clear_bit(unsigned *m, int p) { atomicAnd(&m[p >> 5], ~(1u << (p & 31))); }
+
+// nod_lim (ex4_sedov_2d/main.f90:1359-1396); returns true for the positive branch
+SWF_D bool nod_lim(double g1, double eps, const double v1[4], const double v2[4], const double vb[4],
+                   double out[4]) {
+  double p1 = g1 * (v1[3] - (0.5 * (v1[1] * v1[1] + v1[2] * v1[2])) / v1[0]);
+  double p2 = g1 * (v2[3] - (0.5 * (v2[1] * v2[1] + v2[2] * v2[2])) / v2[0]);
+  double t;
+  bool pos;
+  if (p2 > eps) {
+    t = (p1 - eps) / (p1 - p2);
+    pos = true;
+  } else {
+    double pb = g1 * (vb[3] - (0.5 * (vb[1] * vb[1] + vb[2] * vb[2])) / vb[0]);
+    double t_1 = (p1 - eps) / (p1 - pb);
+    double d12 = sqrt((((v2[3] - v1[3]) * (v2[3] - v1[3]) + (v2[2] - v1[2]) * (v2[2] - v1[2])) +
+                       (v2[1] - v1[1]) * (v2[1] - v1[1])) +
+                      (v2[0] - v1[0]) * (v2[0] - v1[0]));
+    double d1b = sqrt((((vb[3] - v1[3]) * (vb[3] - v1[3]) + (vb[2] - v1[2]) * (vb[2] - v1[2])) +
+                       (vb[1] - v1[1]) * (vb[1] - v1[1])) +
+                      (vb[0] - v1[0]) * (vb[0] - v1[0]));
+    t = fmin2((t_1 * d1b) / d12, 0.25);
+    pos = false;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c] = (1. - t) * v1[c] + t * v2[c];
+  return pos;
+}
+
+struct SweepShared {
+  union {
+    int list[SW_WCH * 32];        // flagged positions of one scan chunk
+    double prod[4][SW_T];         // u_bar partial products
+  };
+  int ends[SW_T];
+  int wscan[SW_WCH];
+  unsigned char genuine[SW_T];
+  int nlist, last_end, flag;
+  long long tpos, tneg;
+  double ubar[4];
+  double red[4][SW_T / 64];
+};
+
+// Pressure-sweep walk from position f in `order`.  FWD: neighbour f+1, else f-1.
+// WRITE == false: speculative, returns the stop position only.
+template <bool FWD, bool WRITE>
+SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], unsigned *next_map,
+                long long *npos, long long *nneg) {
+  const int N = k.nx * k.ny;
+  const double g1 = k.gamma1, eps = k.eps, wr = k.weight / k.weight;
+  int p = f;
+  double v[4];
+  load_cell(k, b, order, p, v);
+  while (true) {
+    const int nb = FWD ? p + 1 : p - 1;
+    double vn[4], vt[4];
+    load_cell(k, b, order, nb, vn);
+    const bool pos = nod_lim(g1, eps, v, vn, ub, vt);
+    double vn2[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) vn2[c] = vn[c] + wr * (v[c] - vt[c]);
+    if (WRITE) {
+      store_cell(k, b, order, p, vt);
+      if (sweep_flag(g1, eps, vt[0], vt[1], vt[2], vt[3])) set_bit(next_map, p);
+      if (pos)
+        ++*npos;
+      else
+        ++*nneg;
+    }
+    p = nb;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = vn2[c];
+    const bool edge = FWD ? (p == N - 1) : (p == 0);
+    if (edge || !sweep_flag(g1, eps, v[0], v[1], v[2], v[3])) {
+      if (WRITE) store_cell(k, b, order, p, v);
+      return p;
+    }
+  }
+}
+
+// Density-sweep walk (scalar_sweep), forward in order I.
+template <bool WRITE>
+SWF_D int swalk(const KCfg &k, const Bufs &b, int f, int dir) {
+  const int N = k.nx * k.ny;
+  const double delta = k.delta, wr = k.weight / k.weight;
+  int p = f, i, j;
+  cellI(k, p, i, j);
+  double v = b.U[cidx(k, i, j)];
+  while (true) {
+    const int nb = p + dir;
+    int ni, nj;
+    cellI(k, nb, ni, nj);
+    const double vn = b.U[cidx(k, ni, nj)];
+    const double vn2 = vn + wr * (v - delta);
+    if (WRITE) {
+      b.U[cidx(k, i, j)] = delta;
+      const long long o = cidx(k, i, j);
+      if (sweep_flag(k.gamma1, k.eps, delta, b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]))
+        set_bit(b.bmP, p);
+      else
+        clear_bit(b.bmP, p);
+    }
+    p = nb;
+    i = ni;
+    j = nj;
+    v = vn2;
+    const bool edge = (dir > 0) ? (p == N - 1) : (p == 0);
+    if (edge || !(v < delta)) {
+      if (WRITE) {
+        const long long o = cidx(k, i, j);
+        b.U[o] = v;
+        if (sweep_flag(k.gamma1, k.eps, v, b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]))
+          set_bit(b.bmP, p);
+        else
+          clear_bit(b.bmP, p);
+      }
+      return p;
+    }
+  }
+}
+
+// Collect the set bits of words [w0, w0+SW_WCH) into sh.list (ascending).
+SWF_D void collect_chunk(const KCfg &k, const unsigned *map, int w0, SweepShared &sh) {
+  const int tid = threadIdx.x;
+  unsigned word = 0;
+  if (tid < SW_WCH) {
+    word = (w0 + tid < k.nwords) ? map[w0 + tid] : 0u;
+    sh.wscan[tid] = __popc(word);
+  }
+  __syncthreads();
+  // inclusive scan over SW_WCH counts (Hillis-Steele)
+  for (int off = 1; off < SW_WCH; off <<= 1) {
+    int v = 0;
+    if (tid < SW_WCH && tid >= off) v = sh.wscan[tid - off];
+    __syncthreads();
+    if (tid < SW_WCH) sh.wscan[tid] += v;
+    __syncthreads();
+  }
+  if (tid < SW_WCH) {
+    int o = sh.wscan[tid] - __popc(word);
+    const int base = (w0 + tid) * 32;
+    while (word) {
+      const int bit = __ffs(word) - 1;
+      word &= word - 1;
+      sh.list[o++] = base + bit;
+    }
+  }
+  if (tid == SW_WCH - 1) sh.nlist = sh.wscan[SW_WCH - 1];
+  __syncthreads();
+}
+
+// One directional pass of the pressure sweep over the flagged positions in
+// `map` (order `order`).  Adjusted-but-still-flagged cells are recorded in `next`.
+template <bool FWD>
+SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned *map, unsigned *next,
+                         const double ub[4], SweepShared &sh) {
+  const int tid = threadIdx.x;
+  const int N = k.nx * k.ny;
+  if (tid == 0) sh.last_end = FWD ? -1 : N;
+  __syncthreads();
+  const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
+  for (int ci = 0; ci < nchunks; ++ci) {
+    const int c = FWD ? ci : (nchunks - 1 - ci);
+    collect_chunk(k, map, c * SW_WCH, sh);
+    const int n = sh.nlist;
+    for (int s0 = 0; s0 < n; s0 += SW_T) {
+      const int idx = s0 + tid;
+      int f = -1;
+      if (idx < n) f = FWD ? sh.list[idx] : sh.list[n - 1 - idx];
+      const int le = sh.last_end;
+      const bool cand = (f >= 0) && (FWD ? (f > le && f <= N - 2) : (f < le && f >= 1));
+      sh.ends[tid] = cand ? pwalk<FWD, false>(k, b, order, f, ub, nullptr, nullptr, nullptr) : -1;
+      sh.genuine[tid] = 0;
+      __syncthreads();
+      if (tid == 0) {
+        int last = sh.last_end;
+        const int m = min(SW_T, n - s0);
+        for (int q = 0; q < m; ++q) {
+          const int fq = FWD ? sh.list[s0 + q] : sh.list[n - 1 - (s0 + q)];
+          const bool ok = FWD ? (fq > last && fq <= N - 2) : (fq < last && fq >= 1);
+          if (ok) {
+            sh.genuine[q] = 1;
+            last = sh.ends[q];
+          }
+        }
+        sh.last_end = last;
+      }
+      __syncthreads();
+      if (sh.genuine[tid]) {
+        long long np = 0, nn = 0;
+        pwalk<FWD, true>(k, b, order, f, ub, next, &np, &nn);
+        atomicAdd((unsigned long long *)&sh.tpos, (unsigned long long)np);
+        atomicAdd((unsigned long long *)&sh.tneg, (unsigned long long)nn);
+      }
+      __syncthreads();
+    }
+  }
+  // the position the pass never examines (N-1 forward, 0 backward)
+  if (tid == 0) {
+    const int pe = FWD ? N - 1 : 0;
+    double v[4];
+    load_cell(k, b, order, pe, v);
+    if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) set_bit(next, pe);
+  }
+  __syncthreads();
+}
+
+SWF_D void clear_map(const KCfg &k, unsigned *m) {
+  for (int w = threadIdx.x; w < k.nwords; w += SW_T) m[w] = 0u;
+  __syncthreads();
+}
+
+// u_bar = sum of weights*u over the interior.  Exact mode: the reference's serial
+// order (j outer, i inner, ex4_sedov_2d/main.f90:1431-1438), one lane per
+// component.  Otherwise a fixed-order parallel sum.
+SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared &sh) {
+  const int tid = threadIdx.x;
+  const long long N = (long long)k.nx * k.ny;
+  const double w = k.weight;
+  if (serial) {
+    double acc = 0.0;  // lane c of wave 0 accumulates component c
+    for (long long c0 = 0; c0 < N; c0 += SW_T) {
+      const long long t = c0 + tid;
+      if (t < N) {
+        const int i = 1 + (int)(t % k.nx), j = 1 + (int)(t / k.nx);
+        const long long o = cidx(k, i, j);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh.prod[c][tid] = w * b.U[c * k.plane + o];
+      }
+      __syncthreads();
+      if (tid < 4) {
+        const int m = (int)min((long long)SW_T, N - c0);
+        for (int q = 0; q < m; ++q) acc = acc + sh.prod[tid][q];
+      }
+      __syncthreads();
+    }
+    if (tid < 4) sh.ubar[tid] = acc;
+  } else {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long long t = tid; t < N; t += SW_T) {
+      const int i = 1 + (int)(t % k.nx), j = 1 + (int)(t / k.nx);
+      const long long o = cidx(k, i, j);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] = acc[c] + w * b.U[c * k.plane + o];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double v = acc[c];
+      for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+      if ((tid & 63) == 0) sh.red[c][tid >> 6] = v;
+    }
+    __syncthreads();
+    if (tid < 4) {
+      double s = 0.0;
+      for (int q = 0; q < SW_T / 64; ++q) s = s + sh.red[tid][q];
+      sh.ubar[tid] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// The whole sweep of one stage, plus the end-of-step time update at stage 3.
+__global__ __launch_bounds__(SW_T) void k_sweep(KCfg k, Bufs b, int stage, int slot_out, int parity,
+                                                int serial_ubar, int force) {
+  __shared__ SweepShared sh;
+  DevScalars *g = b.g;
+  const int tid = threadIdx.x;
+  if (parity >= 0 && !g->active[parity]) {
+    // an inactive step leaves the next one inactive too
+    if (stage == 3 && tid == 0) g->active[1 - parity] = 0;
+    return;
+  }
+  const bool run = (k.use_limiter == 2) && (force || g->postag);
+  if (run) {
+    const int N = k.nx * k.ny;
+    if (tid == 0) {
+      g->stage_sweeps++;
+      sh.tpos = 0;
+      sh.tneg = 0;
+    }
+    // ---- scalar_sweep (density), order I
+    if (g->rho_any) {
+      if (tid == 0) sh.last_end = -1;
+      __syncthreads();
+      const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
+      for (int c = 0; c < nchunks; ++c) {
+        collect_chunk(k, b.bmR, c * SW_WCH, sh);
+        const int n = sh.nlist;
+        for (int s0 = 0; s0 < n; s0 += SW_T) {
+          const int idx = s0 + tid;
+          const int f = (idx < n) ? sh.list[idx] : -1;
+          const int le = sh.last_end;
+          const bool cand = (f >= 0) && f > le && f <= N - 2;
+          sh.ends[tid] = cand ? swalk<false>(k, b, f, +1) : -1;
+          sh.genuine[tid] = 0;
+          __syncthreads();
+          if (tid == 0) {
+            int last = sh.last_end;
+            const int m = min(SW_T, n - s0);
+            for (int q = 0; q < m; ++q) {
+              const int fq = sh.list[s0 + q];
+              if (fq > last && fq <= N - 2) {
+                sh.genuine[q] = 1;
+                last = sh.ends[q];
+              }
+            }
+            sh.last_end = last;
+          }
+          __syncthreads();
+          if (sh.genuine[tid]) swalk<true>(k, b, f, +1);
+          __syncthreads();
+        }
+      }
+      // backward pass: after the forward pass only position N-1 can be below delta
+      if (tid == 0) {
+        int i, j;
+        cellI(k, N - 1, i, j);
+        if (b.U[cidx(k, i, j)] < k.delta) swalk<true>(k, b, N - 1, -1);
+        g->den_count++;
+      }
+      __syncthreads();
+    }
+    // ---- u_bar and the P(u_bar) > 0 precondition
+    compute_ubar(k, b, serial_ubar != 0, sh);
+    double ub[4] = {sh.ubar[0], sh.ubar[1], sh.ubar[2], sh.ubar[3]};
+    bool abort = false;
+    if (k.pbar_check) {
+      const double pb = k.gamma1 * (ub[3] - (0.5 * (ub[1] * ub[1] + ub[2] * ub[2])) / ub[0]);
+      if (pb <= 0.) {
+        if (tid == 0) {
+          g->status = ST_PBAR;
+          g->pbar = pb;
+        }
+        abort = true;
+      }
+    }
+    if (!abort) {
+      long long sc = 0;
+      if (tid == 0) {
+        g->sweep_count++;
+        sc = g->sweep_count;
+        sh.flag = 0;
+      }
+      unsigned *A = b.bmP, *B = b.bmA;
+      int qdone = 0;
+      for (int q = 1; q <= k.swp_num; ++q) {
+        const int order = (q & 1) ? 0 : 1;
+        clear_map(k, B);
+        pressure_pass<true>(k, b, order, A, B, ub, sh);
+        clear_map(k, A);
+        pressure_pass<false>(k, b, order, B, A, ub, sh);
+        // convergence test (:1714-1744): no cell left with P <= eps or rho <= eps
+        // (ex5: P < eps or rho < eps)
+        if (tid == 0) {
+          sh.flag = 0;
+          g->pre_count++;
+        }
+        __syncthreads();
+        for (int w = tid; w < k.nwords; w += SW_T) {
+          unsigned word = A[w];
+          while (word) {
+            const int bit = __ffs(word) - 1;
+            word &= word - 1;
+            double v[4];
+            load_cell(k, b, order, w * 32 + bit, v);
+            bool fail;
+            if (k.success_ge) {
+              const double p = sweep_pres(k.gamma1, v[0], v[1], v[2], v[3]);
+              fail = (p < k.eps) || (v[0] < k.eps);
+            } else {
+              fail = true;
+            }
+            if (fail) atomicOr(&sh.flag, 1);
+          }
+        }
+        __syncthreads();
+        if (!sh.flag) {
+          qdone = q;
+          break;
+        }
+        if (q == k.swp_num) break;
+        // flagged set of the next pass, in the other snake order
+        clear_map(k, B);
+        for (int w = tid; w < k.nwords; w += SW_T) {
+          unsigned word = A[w];
+          while (word) {
+            const int bit = __ffs(word) - 1;
+            word &= word - 1;
+            int i, j;
+            cell_of(k, order, w * 32 + bit, i, j);
+            set_bit(B, pos_of(k, 1 - order, i, j));
+          }
+        }
+        __syncthreads();
+        unsigned *t = A;
+        A = B;
+        B = t;
+      }
+      if (tid == 0) {
+        long long qv = qdone, nn = qdone ? sh.tneg : 0, np = qdone ? sh.tpos : 0;
+        if (!qdone) {
+          g->fail_q = k.swp_num;
+          if (k.fail_stop) g->status = ST_SWEEP_FAIL;
+        }
+        if (sc - 1 < k.track_cap) {
+          b.trk[sc - 1] = qv;
+          b.trk[k.track_cap + sc - 1] = nn;
+          b.trk[2 * k.track_cap + sc - 1] = np;
+        }
+        g->total_sweep += qv;
+        g->total_neg += nn;
+        g->total_pos += np;
+        if (qv > g->max_sweep) g->max_sweep = qv;
+        if (nn > g->max_neg) g->max_neg = nn;
+        if (np > g->max_pos) g->max_pos = np;
+      }
+    }
+    if (tid == 0) {
+      g->swept = 1;
+      if (slot_out >= 0) g->alpha_bits[slot_out] = 0ull;
+    }
+  }
+  if (stage == 3 && parity >= 0 && tid == 0) {
+    const double tn = g->time[parity] + g->dt;
+    g->time[1 - parity] = tn;
+    g->steps_done++;
+    g->active[1 - parity] = (g->status == ST_OK) && time_continue_q(tn, g->tfinal);
+  }
+}
+
+}  // namespace swf
