@@ -122,8 +122,8 @@ int swf_weno_points(const double *a, double *right, double *left, int64_t n, int
 
 /* Average duration (ms) of one launch of a hot kernel on the context's stream,
  * measured with HIP events over `nreps` back-to-back launches on the current state
- * (which: 0 = x-interface flux, 1 = y-interface flux).  No counterpart in the
- * reference: measurement support for the x/y halves of `wenofd`. */
+ * (which: 0 = the interface-flux kernel, both directions of `wenofd`).  No
+ * counterpart in the reference: measurement support. */
 int swf_time_kernel(swf_ctx *ctx, int32_t which, int32_t nreps, double *avg_ms);
 
 int swf_time_continue(double time, double tfinal);

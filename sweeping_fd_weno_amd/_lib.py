@@ -244,11 +244,10 @@ class Solver:
     def sweep(self) -> None:
         _check(lib().swf_sweep(self._h), "swf_sweep")
 
-    def time_kernel(self, which: str, nreps: int = 20) -> float:
-        """Average ms of one x- or y-interface flux launch (HIP events on the solver's stream)."""
+    def time_kernel(self, which: str = "flux", nreps: int = 20) -> float:
+        """Average ms of one launch of the interface-flux kernel (HIP events on the solver's stream)."""
         ms = C.c_double(0.0)
-        _check(lib().swf_time_kernel(self._h, {"xflux": 0, "yflux": 1}[which], nreps, C.byref(ms)),
-               "swf_time_kernel")
+        _check(lib().swf_time_kernel(self._h, {"flux": 0}[which], nreps, C.byref(ms)), "swf_time_kernel")
         return ms.value
 
     def sync(self) -> None:

@@ -30,6 +30,8 @@ int hip_fail(hipError_t e, const char *what) {
   } while (0)
 
 constexpr int kTrackCap = 1 << 18;
+constexpr int kGraphSteps = 16;   // steps per captured graph (even: parity restarts at 0)
+constexpr int kCheckEvery = 256;  // steps between host checks of the device time gate
 
 }  // namespace
 
@@ -42,6 +44,7 @@ struct swf_ctx {
   double *d_fortran;   // staging buffer in the Fortran layout
   size_t n_fortran;    // doubles in u(4,-2:nx+3,-2:ny+3)
   long long steps_total;
+  hipGraphExec_t gexec;  // kGraphSteps steps, captured on first use
 };
 
 static int check_launch(const char *what) {
@@ -116,6 +119,17 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   // ex5_mach2000_jet/main.f90:711: 0.4127/gamma1 + 0.5*u(2)**2/u(1)
   k.jet_e = k.amb_e + (0.5 * (k.jet_mx * k.jet_mx)) / k.jet_rho;
   k.jet_halfwidth = cfg->jet_halfwidth;
+  // findalpha over the prescribed ghost states (the jet's inflow and ambient
+  // columns); every other ghost is an image of an interior cell with equal alpha
+  k.alpha_const = 0.0;
+  if (cfg->bc == SWF_BC_JET_HALF || cfg->bc == SWF_BC_JET_FULL) {
+    k.findalpha_ex3 = cfg->findalpha_ex3;
+    k.eps2 = cfg->eps2;
+    k.gamma = cfg->gamma;
+    k.gamma1 = cfg->gamma - 1.;
+    k.alpha_const = fmax2(cell_alpha(k, k.jet_rho, k.jet_mx, 0.0, k.jet_e),
+                          cell_alpha(k, k.amb_rho, 0.0, 0.0, k.amb_e));
+  }
   k.bc = cfg->bc;
   k.use_limiter = cfg->use_limiter;
   k.swp_num = cfg->swp_num;
@@ -146,6 +160,10 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&trk, sizeof(long long) * 3 * (size_t)kTrackCap);
   double *stage = nullptr;
   if (e == hipSuccess) e = hipMalloc(&stage, sizeof(double) * c->n_fortran);
+  const size_t nblk = (size_t)((cfg->nx + 63) / 64) * ((cfg->ny + 3) / 4);
+  double *apart = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&apart, sizeof(double) * nblk);
+  if (e == hipSuccess) e = hipMemsetAsync(apart, 0, sizeof(double) * nblk, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(pool, 0, 5 * pbytes, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(maps, 0, sizeof(unsigned) * 4 * (size_t)k.nwords, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(g, 0, sizeof(DevScalars), c->stream);
@@ -158,6 +176,7 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
     if (g) (void)hipFree(g);
     if (trk) (void)hipFree(trk);
     if (stage) (void)hipFree(stage);
+    if (apart) (void)hipFree(apart);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return rc;
@@ -174,6 +193,7 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   c->b.bmB = maps + 3 * (size_t)k.nwords;
   c->b.g = g;
   c->b.trk = trk;
+  c->b.apart = apart;
   c->d_fortran = stage;
   *out = c;
   return SWF_OK;
@@ -188,6 +208,8 @@ void swf_destroy(swf_ctx *c) {
   (void)hipFree(c->b.g);
   (void)hipFree(c->b.trk);
   (void)hipFree(c->d_fortran);
+  (void)hipFree(c->b.apart);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -208,23 +230,23 @@ int swf_download(swf_ctx *c, double *u) {
   if (!c || !u) return SWF_E_INVALID;
   int rc = set_device(c);
   if (rc) return rc;
+  // the stepping kernels keep only the edge ghosts the stencils read; the full
+  // ghost frame (corners included) is the reference's `bc` applied to the interior
+  launch_bc(c->k, c->b, c->stream, -1, -1);
   launch_to_fortran(c->k, c->b.U, c->d_fortran, c->stream);
   HIPC(hipMemcpyAsync(u, c->d_fortran, sizeof(double) * c->n_fortran, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   return check_launch("swf_download");
 }
 
-// one RK stage (s = 1, 2, 3) of one step with parity p
+// one RK stage (s = 1, 2, 3) of one step with parity p: wenofd (both directions);
+// scheme + RK combination + checkpositivity + bc images; then one workgroup for
+// sweep_lim (when flagged), findalpha of the new state and, at s = 3, the time.
 static void enqueue_stage(swf_ctx *c, int s, int p) {
-  const KCfg &k = c->k;
-  const Bufs &b = c->b;
   const int slot_in = s - 1, slot_out = s % 3;
-  launch_xflux(c->mode, k, b, c->stream, slot_in, slot_out, p, 1);
-  launch_yflux(c->mode, k, b, c->stream, slot_in, p);
-  launch_update(c->mode, k, b, c->stream, s, slot_in, slot_out, p, 0);
-  launch_sweep(k, b, c->stream, s, slot_out, p, c->mode == MODE_EXACT ? 1 : 0, 0);
-  launch_alpha_fix(k, b, c->stream, slot_out, p);
-  launch_bc(k, b, c->stream, slot_out, p);
+  launch_flux(c->mode, c->k, c->b, c->stream, slot_in, slot_out, p, 1);
+  launch_update(c->mode, c->k, c->b, c->stream, s, slot_in, slot_out, p, 0);
+  launch_stage_end(c->k, c->b, c->stream, s, slot_out, p, c->mode == MODE_EXACT ? 1 : 0);
 }
 
 static void enqueue_step(swf_ctx *c, int p) {
@@ -255,23 +277,47 @@ int swf_step(swf_ctx *c, double time, double tfinal, double *dt_out) {
   return h.status;
 }
 
+static int ensure_graph(swf_ctx *c) {
+  if (c->gexec) return SWF_OK;
+  hipGraph_t graph = nullptr;
+  HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int q = 0; q < kGraphSteps; ++q) enqueue_step(c, q & 1);
+  HIPC(hipStreamEndCapture(c->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  return SWF_OK;
+}
+
 int swf_advance(swf_ctx *c, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out) {
   if (!c || !time_inout || max_steps < 0) return SWF_E_INVALID;
   int rc = set_device(c);
   if (rc) return rc;
+  rc = ensure_graph(c);
+  if (rc) return rc;
   launch_begin_run(c->b, c->stream, *time_inout, tfinal, time_continue_q(*time_inout, tfinal));
-  const int64_t batch = 64;
-  int64_t launched = 0;
+  int64_t launched = 0, since_check = 0;
   DevScalars h;
   while (launched < max_steps) {
-    const int64_t n = (max_steps - launched < batch) ? (max_steps - launched) : batch;
-    for (int64_t q = 0; q < n; ++q) enqueue_step(c, (int)((launched + q) & 1));
-    launched += n;
+    if (max_steps - launched >= kGraphSteps) {
+      HIPC(hipGraphLaunch(c->gexec, c->stream));
+      launched += kGraphSteps;
+      since_check += kGraphSteps;
+    } else {
+      // remainder: plain launches continue the parity sequence
+      for (; launched < max_steps; ++launched, ++since_check) enqueue_step(c, (int)(launched & 1));
+    }
     rc = check_launch("swf_advance");
     if (rc) return rc;
-    rc = read_scalars(c, &h);
-    if (rc) return rc;
-    if (h.steps_done < launched || h.status != ST_OK) break;
+    if (since_check >= kCheckEvery || launched >= max_steps) {
+      since_check = 0;
+      rc = read_scalars(c, &h);
+      if (rc) return rc;
+      if (h.steps_done < launched || h.status != ST_OK) break;
+    }
   }
   rc = read_scalars(c, &h);
   if (rc) return rc;
@@ -316,8 +362,7 @@ int swf_rhs(swf_ctx *c, double alphamax, double *rhs_out) {
   rc = read_scalars(c, &h0);
   if (rc) return rc;
   launch_set_alpha(c->b, c->stream, 0, bits, 0, 1);
-  launch_xflux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
-  launch_yflux(c->mode, c->k, c->b, c->stream, 0, 0);
+  launch_flux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
   launch_update(c->mode, c->k, c->b, c->stream, 1, 0, 3, 0, 1);
   launch_to_fortran(c->k, c->b.R, c->d_fortran, c->stream);
   HIPC(hipMemcpyAsync(rhs_out, c->d_fortran, sizeof(double) * c->n_fortran, hipMemcpyDeviceToHost, c->stream));
@@ -345,7 +390,7 @@ int swf_sweep(swf_ctx *c) {
   int rc = set_device(c);
   if (rc) return rc;
   launch_flags(c->k, c->b, c->stream);
-  launch_sweep(c->k, c->b, c->stream, 0, -1, -1, c->mode == MODE_EXACT ? 1 : 0, 1);
+  launch_sweep(c->k, c->b, c->stream, c->mode == MODE_EXACT ? 1 : 0);
   rc = check_launch("swf_sweep");
   if (rc) return rc;
   DevScalars h;
@@ -392,10 +437,10 @@ int64_t swf_track(swf_ctx *c, int64_t *q, int64_t *neg, int64_t *pos, int64_t ca
 }
 
 int swf_time_kernel(swf_ctx *c, int32_t which, int32_t nreps, double *avg_ms) {
-  if (!c || !avg_ms || nreps < 1 || which < 0 || which > 1) return SWF_E_INVALID;
+  if (!c || !avg_ms || nreps < 1 || which != 0) return SWF_E_INVALID;
   int rc = set_device(c);
   if (rc) return rc;
-  // the flux kernels read U and alpha slot 0 and write only Fx / Fy
+  // the flux kernel reads U and alpha slot 0 and writes only Fx / Fy
   DevScalars h0;
   rc = read_scalars(c, &h0);
   if (rc) return rc;
@@ -403,18 +448,9 @@ int swf_time_kernel(swf_ctx *c, int32_t which, int32_t nreps, double *avg_ms) {
   hipEvent_t e0, e1;
   HIPC(hipEventCreate(&e0));
   HIPC(hipEventCreate(&e1));
-  // one untimed launch
-  if (which == 0)
-    launch_xflux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
-  else
-    launch_yflux(c->mode, c->k, c->b, c->stream, 0, 0);
+  launch_flux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);  // one untimed launch
   HIPC(hipEventRecord(e0, c->stream));
-  for (int r = 0; r < nreps; ++r) {
-    if (which == 0)
-      launch_xflux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
-    else
-      launch_yflux(c->mode, c->k, c->b, c->stream, 0, 0);
-  }
+  for (int r = 0; r < nreps; ++r) launch_flux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
   HIPC(hipEventRecord(e1, c->stream));
   HIPC(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -32,6 +32,7 @@ struct KCfg {
   long long plane;   // doubles per plane: (ny+6)*P
   double dx, dy, gamma, gamma1, cfl, eps, eps2, delta, weight, dt_num;
   double xlo, ylo, jet_rho, jet_mx, jet_e, amb_rho, amb_e, jet_halfwidth;
+  double alpha_const;  // alpha of the prescribed boundary states (jet inflow/ambient), else 0
   int bc, use_limiter, swp_num, track_cap;
   int alpha_in_scheme, roe_eps_double, ydens_plain, eps2_quad_literal, findalpha_ex3;
   int dt_rule_dx53, poscheck_le, success_ge, pbar_check, fail_stop;
@@ -44,6 +45,7 @@ struct DevScalars {
   double tfinal, dt;
   unsigned long long alpha_bits[4];  // alpha max: inputs of stage 1,2,3; [3] queries
   int postag, swept, status, rho_any;
+  unsigned done_blocks;      // update workgroups finished in this stage
   long long steps_done;
   long long sweep_count, den_count, pre_count, stage_sweeps;
   long long total_sweep, max_sweep, total_neg, max_neg, total_pos, max_pos;
@@ -57,6 +59,7 @@ struct Bufs {
   unsigned *bmR, *bmP, *bmA, *bmB;  // sweep bitmaps (order positions)
   DevScalars *g;
   long long *trk;                   // 3 * track_cap: q, neg, pos per sweep trigger
+  double *apart;                    // alpha max of each update workgroup
 };
 
 // index of cell (i, j) (Fortran indices, ghosts from -2) inside a plane
@@ -79,8 +82,8 @@ SWF_D void cellII(const KCfg &k, int p, int &i, int &j) {
 }
 
 // Fortran MAX / MIN of two non-NaN reals
-SWF_D double fmax2(double a, double b) { return (b > a) ? b : a; }
-SWF_D double fmin2(double a, double b) { return (b < a) ? b : a; }
+SWF_HD double fmax2(double a, double b) { return (b > a) ? b : a; }
+SWF_HD double fmin2(double a, double b) { return (b < a) ? b : a; }
 
 // ---------------------------------------------------------------------------
 // Point physics
@@ -350,7 +353,7 @@ SWF_D void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double
 // ---------------------------------------------------------------------------
 // findalpha per cell (ex4_sedov_2d/main.f90:652-683; ex3 :549-557)
 // ---------------------------------------------------------------------------
-SWF_D double cell_alpha(const KCfg &k, double u1, double u2, double u3, double u4) {
+SWF_HD double cell_alpha(const KCfg &k, double u1, double u2, double u3, double u4) {
   double den = u1;
   if (k.findalpha_ex3) {
     double pres = k.gamma1 * ((u4 - ((u2 * u2) / u1) / 2.) - ((u3 * u3) / u1) / 2.);
@@ -372,9 +375,11 @@ SWF_D double cell_alpha(const KCfg &k, double u1, double u2, double u3, double u
 
 // sweep flag: P <= eps or rho <= eps (ex4_sedov_2d/main.f90:1465-1466)
 SWF_D double sweep_pres(double g1, double v0, double v1, double v2, double v3) {
+#pragma clang fp contract(off)
   return g1 * (v3 - (0.5 * (v1 * v1 + v2 * v2)) / v0);
 }
 SWF_D bool sweep_flag(double g1, double eps, double v0, double v1, double v2, double v3) {
+#pragma clang fp contract(off)
   double p = sweep_pres(g1, v0, v1, v2, v3);
   return (p <= eps) || (v0 <= eps);
 }

@@ -5,17 +5,13 @@
 #define SWF_COMMON_KERNELS 1
 #include "swf_kernels.h"
 #include "swf_launch.h"
-#include "swf_sweep.h"
 
 namespace swf {
 
-void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
-                     int clear_maps) {
-  hipLaunchKernelGGL(k_xflux<MODE_EXACT>, grid_xflux(k), dim3(64, 4), 0, s, k, b, slot_in, slot_reset, parity,
+void launch_flux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
+                    int clear_maps) {
+  hipLaunchKernelGGL(k_flux<MODE_EXACT>, dim3(flux_blocks(k)), dim3(256), 0, s, k, b, slot_in, slot_reset, parity,
                      clear_maps);
-}
-void launch_yflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
-  hipLaunchKernelGGL(k_yflux<MODE_EXACT>, grid_yflux(k), dim3(16, 16), 0, s, k, b, slot_in, parity);
 }
 void launch_update_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out, int parity,
                       int rhs_only) {
@@ -26,18 +22,12 @@ void launch_weno_points_m0(const double *a, double *r, double *l, long long n, h
   hipLaunchKernelGGL(k_weno_points<MODE_EXACT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, r, l, n);
 }
 
-void launch_xflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
-                  int clear_maps) {
+void launch_flux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
+                 int clear_maps) {
   if (mode == MODE_EXACT)
-    launch_xflux_m0(k, b, s, slot_in, slot_reset, parity, clear_maps);
+    launch_flux_m0(k, b, s, slot_in, slot_reset, parity, clear_maps);
   else
-    launch_xflux_m12(mode, k, b, s, slot_in, slot_reset, parity, clear_maps);
-}
-void launch_yflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
-  if (mode == MODE_EXACT)
-    launch_yflux_m0(k, b, s, slot_in, parity);
-  else
-    launch_yflux_m12(mode, k, b, s, slot_in, parity);
+    launch_flux_m12(mode, k, b, s, slot_in, slot_reset, parity, clear_maps);
 }
 void launch_update(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out,
                    int parity, int rhs_only) {
@@ -111,12 +101,12 @@ void launch_flags(const KCfg &k, const Bufs &b, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_flags, dim3(64), dim3(256), 0, s, k, b);
   hipLaunchKernelGGL(k_flags, grid_cells(k), dim3(64, 4), 0, s, k, b);
 }
-void launch_sweep(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_out, int parity, int serial_ubar,
-                  int force) {
-  hipLaunchKernelGGL(k_sweep, dim3(1), dim3(SW_T), 0, s, k, b, stage, slot_out, parity, serial_ubar, force);
+void launch_stage_end(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_out, int parity,
+                      int serial_ubar) {
+  hipLaunchKernelGGL(k_stage_end, dim3(1), dim3(SW_T), 0, s, k, b, stage, slot_out, parity, serial_ubar);
 }
-void launch_alpha_fix(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out, int parity) {
-  hipLaunchKernelGGL(k_alpha_fix, grid_cells(k), dim3(64, 4), 0, s, k, b, slot_out, parity);
+void launch_sweep(const KCfg &k, const Bufs &b, hipStream_t s, int serial_ubar) {
+  hipLaunchKernelGGL(k_sweep, dim3(1), dim3(SW_T), 0, s, k, b, serial_ubar);
 }
 void launch_alpha_all(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out) {
   hipLaunchKernelGGL(k_alpha_all, dim3((k.nx + 6 + 63) / 64, (k.ny + 6 + 3) / 4), dim3(64, 4), 0, s, k, b, slot_out);

@@ -5,20 +5,14 @@
 
 namespace swf {
 
-void launch_xflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
-                      int clear_maps) {
+void launch_flux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
+                     int clear_maps) {
   if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_xflux<MODE_FP64>, grid_xflux(k), dim3(64, 4), 0, s, k, b, slot_in, slot_reset, parity,
+    hipLaunchKernelGGL(k_flux<MODE_FP64>, dim3(flux_blocks(k)), dim3(256), 0, s, k, b, slot_in, slot_reset, parity,
                        clear_maps);
   else
-    hipLaunchKernelGGL(k_xflux<MODE_FP32>, grid_xflux(k), dim3(64, 4), 0, s, k, b, slot_in, slot_reset, parity,
+    hipLaunchKernelGGL(k_flux<MODE_FP32>, dim3(flux_blocks(k)), dim3(256), 0, s, k, b, slot_in, slot_reset, parity,
                        clear_maps);
-}
-void launch_yflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
-  if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_yflux<MODE_FP64>, grid_yflux(k), dim3(16, 16), 0, s, k, b, slot_in, parity);
-  else
-    hipLaunchKernelGGL(k_yflux<MODE_FP32>, grid_yflux(k), dim3(16, 16), 0, s, k, b, slot_in, parity);
 }
 void launch_update_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out,
                        int parity, int rhs_only) {

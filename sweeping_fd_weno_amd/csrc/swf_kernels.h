@@ -13,6 +13,7 @@
 #pragma once
 
 #include "swf_device.h"
+#include "swf_sweep.h"
 
 namespace swf {
 
@@ -23,33 +24,28 @@ constexpr int YT_W = 16, YT_H = 16, YT_PTS = YT_H + 5;
 
 SWF_D double alpha_from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
 
+// LDS of one flux tile (x and y tiles share the allocation)
+union FluxSmem {
+  struct {
+    double fp[XT_H][XT_PTS][4];
+    double fn[XT_H][XT_PTS][4];
+    Prim pr[XT_H][XT_PTS];
+  } x;
+  struct {
+    double fp[YT_PTS][YT_W][4];
+    double fn[YT_PTS][YT_W][4];
+    Prim pr[YT_PTS][YT_W];
+  } y;
+};
+
+SWF_D int flux_xblocks(const KCfg &k) { return ((k.nx + 1 + XT_W - 1) / XT_W) * ((k.ny + XT_H - 1) / XT_H); }
+SWF_D int flux_yblocks(const KCfg &k) { return ((k.nx + YT_W - 1) / YT_W) * ((k.ny + 1 + YT_H - 1) / YT_H); }
+
+// x tile: interfaces k0..k0+63 of rows j0..j0+3
 template <int M>
-__global__ __launch_bounds__(256) void k_xflux(KCfg k, Bufs b, int slot_in, int slot_reset, int parity,
-                                               int clear_maps) {
-  const DevScalars *g = b.g;
-  if (!g->active[parity]) return;
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * XT_W;
-  const int k0 = blockIdx.x * XT_W, j0 = 1 + blockIdx.y * XT_H;
-  // housekeeping for the update kernel of this stage
-  if (clear_maps) {
-    const long long gid = (long long)(blockIdx.x + blockIdx.y * gridDim.x) * 256 + tid;
-    if (gid < k.nwords) {
-      b.bmR[gid] = 0u;
-      b.bmP[gid] = 0u;
-    }
-    if (gid == 0) {
-      b.g->alpha_bits[slot_reset] = 0ull;
-      b.g->postag = 0;
-      b.g->rho_any = 0;
-    }
-  }
-  const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
-  const double ia = 1.0 / alpha;
-
-  __shared__ double s_fp[XT_H][XT_PTS][4];
-  __shared__ double s_fn[XT_H][XT_PTS][4];
-  __shared__ Prim s_pr[XT_H][XT_PTS];
-
+SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int bx, int by, double alpha, double ia, FluxSmem &sm,
+                      int tid) {
+  const int k0 = bx * XT_W, j0 = 1 + by * XT_H;
   for (int p = tid; p < XT_H * XT_PTS; p += 256) {
     const int r = p / XT_PTS, q = p - r * XT_PTS;
     const int i = k0 - 2 + q, j = j0 + r;
@@ -58,41 +54,34 @@ __global__ __launch_bounds__(256) void k_xflux(KCfg k, Bufs b, int slot_in, int 
       double u[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) u[c] = b.U[c * k.plane + o];
-      split_point<M, 0>(k, u, alpha, ia, s_fp[r][q], s_fn[r][q]);
+      split_point<M, 0>(k, u, alpha, ia, sm.x.fp[r][q], sm.x.fn[r][q]);
       Prim pr;
       roe_prims<M, 0>(k, u, pr.w, pr.vx, pr.vy, pr.h);
-      s_pr[r][q] = pr;
+      sm.x.pr[r][q] = pr;
     }
   }
   __syncthreads();
+  const int tx = tid & (XT_W - 1), ty = tid / XT_W;
   const int kk = k0 + tx, j = j0 + ty;
   if (kk > k.nx || j > k.ny) return;
   double fp[4], fn[4];
   // f+ at interface kk (Roe(kk, kk+1), stencil kk-2..kk+2)
-  half_flux<M, 0, true>(k, s_pr[ty][tx + 2], s_pr[ty][tx + 3], &s_fp[ty][tx + 2][0], 4, fp);
+  half_flux<M, 0, true>(k, sm.x.pr[ty][tx + 2], sm.x.pr[ty][tx + 3], &sm.x.fp[ty][tx + 2][0], 4, fp);
   // f- at interface kk+1 (Roe(kk+1, kk+2), stencil kk-1..kk+3)
-  half_flux<M, 0, false>(k, s_pr[ty][tx + 3], s_pr[ty][tx + 4], &s_fn[ty][tx + 3][0], 4, fn);
+  half_flux<M, 0, false>(k, sm.x.pr[ty][tx + 3], sm.x.pr[ty][tx + 4], &sm.x.fn[ty][tx + 3][0], 4, fn);
   const long long o = cidx(k, kk, j);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    double f = k.alpha_in_scheme ? (fp[c] - fn[c]) : (alpha * fp[c] - alpha * fn[c]);
+    const double f = k.alpha_in_scheme ? (fp[c] - fn[c]) : (alpha * fp[c] - alpha * fn[c]);
     b.Fx[c * k.plane + o] = f;
   }
 }
 
+// y tile: interfaces jf0..jf0+15 of columns i0..i0+15
 template <int M>
-__global__ __launch_bounds__(256) void k_yflux(KCfg k, Bufs b, int slot_in, int parity) {
-  const DevScalars *g = b.g;
-  if (!g->active[parity]) return;
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + ty * YT_W;
-  const int i0 = 1 + blockIdx.x * YT_W, jf0 = blockIdx.y * YT_H;
-  const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
-  const double ia = 1.0 / alpha;
-
-  __shared__ double s_fp[YT_PTS][YT_W][4];
-  __shared__ double s_fn[YT_PTS][YT_W][4];
-  __shared__ Prim s_pr[YT_PTS][YT_W];
-
+SWF_D void yflux_tile(const KCfg &k, const Bufs &b, int bx, int by, double alpha, double ia, FluxSmem &sm,
+                      int tid) {
+  const int i0 = 1 + bx * YT_W, jf0 = by * YT_H;
   for (int p = tid; p < YT_PTS * YT_W; p += 256) {
     const int r = p / YT_W, q = p - r * YT_W;
     const int i = i0 + q, j = jf0 - 2 + r;
@@ -101,23 +90,58 @@ __global__ __launch_bounds__(256) void k_yflux(KCfg k, Bufs b, int slot_in, int 
       double u[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) u[c] = b.U[c * k.plane + o];
-      split_point<M, 1>(k, u, alpha, ia, s_fp[r][q], s_fn[r][q]);
+      split_point<M, 1>(k, u, alpha, ia, sm.y.fp[r][q], sm.y.fn[r][q]);
       Prim pr;
       roe_prims<M, 1>(k, u, pr.w, pr.vx, pr.vy, pr.h);
-      s_pr[r][q] = pr;
+      sm.y.pr[r][q] = pr;
     }
   }
   __syncthreads();
+  const int tx = tid & (YT_W - 1), ty = tid / YT_W;
   const int i = i0 + tx, jf = jf0 + ty;
   if (i > k.nx || jf > k.ny) return;
   double fp[4], fn[4];
-  half_flux<M, 1, true>(k, s_pr[ty + 2][tx], s_pr[ty + 3][tx], &s_fp[ty + 2][tx][0], YT_W * 4, fp);
-  half_flux<M, 1, false>(k, s_pr[ty + 3][tx], s_pr[ty + 4][tx], &s_fn[ty + 3][tx][0], YT_W * 4, fn);
+  half_flux<M, 1, true>(k, sm.y.pr[ty + 2][tx], sm.y.pr[ty + 3][tx], &sm.y.fp[ty + 2][tx][0], YT_W * 4, fp);
+  half_flux<M, 1, false>(k, sm.y.pr[ty + 3][tx], sm.y.pr[ty + 4][tx], &sm.y.fn[ty + 3][tx][0], YT_W * 4, fn);
   const long long o = cidx(k, i, jf);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    double f = k.alpha_in_scheme ? (fp[c] - fn[c]) : (alpha * fp[c] - alpha * fn[c]);
+    const double f = k.alpha_in_scheme ? (fp[c] - fn[c]) : (alpha * fp[c] - alpha * fn[c]);
     b.Fy[c * k.plane + o] = f;
+  }
+}
+
+// wenofd: both directions in one launch; the first flux_xblocks() workgroups
+// take x tiles, the rest y tiles.  `clear_maps` also prepares this stage's
+// bookkeeping (sweep bitmaps, flags, the alpha slot the update kernel fills).
+template <int M>
+__global__ __launch_bounds__(256) void k_flux(KCfg k, Bufs b, int slot_in, int slot_reset, int parity,
+                                              int clear_maps) {
+  __shared__ FluxSmem sm;
+  DevScalars *g = b.g;
+  if (!g->active[parity]) return;
+  const int tid = threadIdx.x;
+  if (clear_maps) {
+    const long long gid = (long long)blockIdx.x * 256 + tid;
+    if (gid < k.nwords) {
+      b.bmR[gid] = 0u;
+      b.bmP[gid] = 0u;
+    }
+    if (gid == 0) {
+      g->postag = 0;
+      g->rho_any = 0;
+    }
+  }
+  const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
+  const double ia = 1.0 / alpha;
+  const int nbx = flux_xblocks(k);
+  if ((int)blockIdx.x < nbx) {
+    const int gx = (k.nx + 1 + XT_W - 1) / XT_W;
+    xflux_tile<M>(k, b, blockIdx.x % gx, blockIdx.x / gx, alpha, ia, sm, tid);
+  } else {
+    const int gx = (k.nx + YT_W - 1) / YT_W;
+    const int t = blockIdx.x - nbx;
+    yflux_tile<M>(k, b, t % gx, t / gx, alpha, ia, sm, tid);
   }
 }
 
@@ -138,24 +162,173 @@ SWF_D double step_dt(const KCfg &k, double alpha, double time, double tfinal) {
   return fmin2(fmin2(dtx, dty), tfinal - time);
 }
 
-// RK stage combination + checkpositivity + sweep flags + alpha of the new state.
-// `rhs_only` writes RHS into b.R instead (parity tests of the spatial operator).
+// Inflow / ambient state of the jet's x = 0 ghost at row j (ex5_mach2000_jet/main.f90:688-708)
+SWF_D void jet_state(const KCfg &k, int j, double v[4]) {
+  const double ys = k.ylo + j * k.dy - k.dy / 2.;
+  const bool jet = (k.bc == BC_JET_HALF) ? (ys <= k.jet_halfwidth) : (fabs(ys) < k.jet_halfwidth);
+  if (jet) {
+    v[0] = k.jet_rho; v[1] = k.jet_mx; v[2] = 0.; v[3] = k.jet_e;
+  } else {
+    v[0] = k.amb_rho; v[1] = 0.; v[2] = 0.; v[3] = k.amb_e;
+  }
+}
+
+// Write the edge ghost cells whose value is an image of interior cell (i, j) with
+// state v.  These are the ghosts the flux stencils read (edges, not corners);
+// the images equal the reference's `bc` passes (see k_bc).
+SWF_D void write_ghosts(const KCfg &k, const Bufs &b, int i, int j, const double v[4]) {
+  double w[4];
+  switch (k.bc) {
+    case BC_SEDOV:
+      if (i <= 3) {
+        w[0] = v[0]; w[1] = -v[1]; w[2] = v[2]; w[3] = v[3];
+        store_ij(k, b, 1 - i, j, w);
+      }
+      if (i == k.nx)
+        for (int g = 1; g <= 3; ++g) store_ij(k, b, k.nx + g, j, v);
+      if (j <= 3) {
+        w[0] = v[0]; w[1] = v[1]; w[2] = -v[2]; w[3] = v[3];
+        store_ij(k, b, i, 1 - j, w);
+      }
+      if (j == k.ny)
+        for (int g = 1; g <= 3; ++g) store_ij(k, b, i, k.ny + g, v);
+      break;
+    case BC_PERIODIC:
+      if (i >= k.nx - 2) store_ij(k, b, i - k.nx, j, v);
+      if (i <= 3) store_ij(k, b, i + k.nx, j, v);
+      if (j >= k.ny - 2) store_ij(k, b, i, j - k.ny, v);
+      if (j <= 3) store_ij(k, b, i, j + k.ny, v);
+      break;
+    default:  // jets
+      if (i == 1) {
+        jet_state(k, j, w);
+        for (int g = 0; g <= 2; ++g) store_ij(k, b, -g, j, w);
+      }
+      if (i == k.nx)
+        for (int g = 1; g <= 3; ++g) store_ij(k, b, k.nx + g, j, v);
+      if (k.bc == BC_JET_HALF) {
+        if (j <= 3) {
+          w[0] = v[0]; w[1] = v[1]; w[2] = -v[2]; w[3] = v[3];
+          store_ij(k, b, i, 1 - j, w);
+        }
+      } else if (j == 1) {
+        for (int g = 0; g <= 2; ++g) store_ij(k, b, i, -g, v);
+      }
+      if (j == k.ny)
+        for (int g = 1; g <= 3; ++g) store_ij(k, b, i, k.ny + g, v);
+      break;
+  }
+}
+
+// block-wide max of a non-negative value over NT threads; result valid in thread 0
+template <int NT>
+SWF_D double block_max(double a, double *red, int tid) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(a, off, 64);
+    a = (o > a) ? o : a;
+  }
+  if ((tid & 63) == 0) red[tid >> 6] = a;
+  __syncthreads();
+  double m = 0.0;
+  if (tid == 0)
+    for (int q = 0; q < NT / 64; ++q) m = fmax2(m, red[q]);
+  __syncthreads();
+  return m;
+}
+
+// After a sweep: alpha of the update blocks whose cells changed, and the edge
+// ghosts of the boundary band.  One workgroup of NT threads (NT >= 256).
+template <int NT>
+SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double *red, int tid) {
+  const int gx = (k.nx + 63) >> 6, gy = (k.ny + 3) >> 2, nblk = gx * gy;
+  const int nw = sh.all_dirty ? (nblk + 31) / 32 : min(SW_DIRTY_WORDS, (nblk + 31) / 32);
+  constexpr int PER = NT / 256;  // update blocks handled at once
+  for (int w = 0; w < nw; ++w) {
+    unsigned word = sh.all_dirty ? 0xffffffffu : sh.dirty[w];
+    while (word) {
+      // up to PER dirty blocks of this word, one 256-thread group each
+      int blks[PER];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (word) {
+          blks[q] = w * 32 + (__ffs(word) - 1);
+          word &= word - 1;
+        } else {
+          blks[q] = -1;
+        }
+      }
+      const int grp = tid >> 8, lt = tid & 255;
+      const int blk = blks[grp];
+      double a = 0.0;
+      if (blk >= 0 && blk < nblk) {
+        const int i = 1 + (blk % gx) * 64 + (lt & 63), j = 1 + (blk / gx) * 4 + (lt >> 6);
+        if (i <= k.nx && j <= k.ny) {
+          double v[4];
+          load_ij(k, b, i, j, v);
+          a = cell_alpha(k, v[0], v[1], v[2], v[3]);
+        }
+      }
+      // max within each 256-thread group: waves 4*grp .. 4*grp+3
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(a, off, 64);
+        a = (o > a) ? o : a;
+      }
+      if ((tid & 63) == 0) red[tid >> 6] = a;
+      __syncthreads();
+      if (lt == 0 && blk >= 0 && blk < nblk)
+        b.apart[blk] = fmax2(fmax2(red[4 * grp], red[4 * grp + 1]), fmax2(red[4 * grp + 2], red[4 * grp + 3]));
+      __syncthreads();
+    }
+  }
+  // ghosts of the boundary band
+  const int nb = 6 * k.ny + 6 * k.nx;
+  for (int t = tid; t < nb; t += NT) {
+    int i, j;
+    if (t < 6 * k.ny) {
+      const int c = t % 6;
+      j = 1 + t / 6;
+      i = (c < 3) ? 1 + c : k.nx - 5 + c;
+    } else {
+      const int s = t - 6 * k.ny, c = s % 6;
+      i = 1 + s / 6;
+      j = (c < 3) ? 1 + c : k.ny - 5 + c;
+    }
+    if (i >= 1 && i <= k.nx && j >= 1 && j <= k.ny) {
+      double v[4];
+      load_ij(k, b, i, j, v);
+      write_ghosts(k, b, i, j, v);
+    }
+  }
+  __syncthreads();
+}
+
+// RK stage combination + checkpositivity + sweep flags + ghost images + the
+// workgroup's alpha max of the new state (reduced by k_stage_end).  `rhs_only`
+// writes RHS into b.R instead (parity tests of the spatial operator).
 template <int M>
 __global__ __launch_bounds__(256) void k_update(KCfg k, Bufs b, int stage, int slot_in, int slot_out,
                                                 int parity, int rhs_only) {
+  __shared__ double red4[4];
   DevScalars *g = b.g;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
   if (!g->active[parity]) return;
   const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y;
   const bool in = (i <= k.nx && j <= k.ny);
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
+  if (rhs_only) {
+    if (in) {
+      const long long o = cidx(k, i, j);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b.R[c * k.plane + o] = cell_rhs(k, b, c, o, alpha);
+    }
+    return;
+  }
   double anew = 0.0;
   if (in) {
     const long long o = cidx(k, i, j);
-    if (rhs_only) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b.R[c * k.plane + o] = cell_rhs(k, b, c, o, alpha);
-      return;
-    }
     double dt;
     if (stage == 1) {
       dt = step_dt(k, alpha, g->time[parity], g->tfinal);
@@ -184,6 +357,7 @@ __global__ __launch_bounds__(256) void k_update(KCfg k, Bufs b, int stage, int s
       v[c] = un;
       b.U[c * k.plane + o] = un;
     }
+    write_ghosts(k, b, i, j, v);
     // checkpositivity (ex4_sedov_2d/main.f90:779-801)
     const double ke = (v[3] - ((v[2] * v[2]) / v[0]) / 2.) - ((v[1] * v[1]) / v[0]) / 2.;
     const bool bad = k.poscheck_le ? (v[0] <= k.eps || ke <= k.eps) : (v[0] < k.eps || ke < k.eps);
@@ -198,23 +372,45 @@ __global__ __launch_bounds__(256) void k_update(KCfg k, Bufs b, int stage, int s
     }
     anew = cell_alpha(k, v[0], v[1], v[2], v[3]);
   }
-  if (!rhs_only) alpha_atomic_max(&g->alpha_bits[slot_out], anew);
+  const double am = block_max<256>(anew, red4, tid);
+  if (tid == 0) b.apart[bid] = am;
 }
 
 #ifdef SWF_COMMON_KERNELS
 // Mode-independent grid kernels, compiled once (swf_k_exact.hip).
 
-// alpha max over the interior, after the sweep changed the state
-__global__ __launch_bounds__(256) void k_alpha_fix(KCfg k, Bufs b, int slot_out, int parity) {
+// End of an RK stage, one workgroup: sweep_lim when checkpositivity flagged the
+// stage, the alpha max of the new state (findalpha) from the per-workgroup
+// maxima and the prescribed boundary states, and at stage 3 the time update of
+// the reference's loop (time = time + dt, DO WHILE test).
+__global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, int slot_out, int parity,
+                                                    int serial_ubar) {
+  __shared__ SweepShared<SW_T> sh;
+  __shared__ double red[SW_T / 64];
   DevScalars *g = b.g;
-  if (!g->active[parity] || !g->swept) return;
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y;
-  double a = 0.0;
-  if (i <= k.nx && j <= k.ny) {
-    const long long o = cidx(k, i, j);
-    a = cell_alpha(k, b.U[o], b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]);
+  const int tid = threadIdx.x;
+  if (!g->active[parity]) {
+    // an inactive step leaves the next one inactive too
+    if (stage == 3 && tid == 0) g->active[1 - parity] = 0;
+    return;
   }
-  alpha_atomic_max(&g->alpha_bits[slot_out], a);
+  if (k.use_limiter == 2 && g->postag) {
+    sweep_body<SW_T>(k, b, serial_ubar != 0, sh, tid);
+    after_sweep<SW_T>(k, b, sh, red, tid);
+  }
+  const int nblk = ((k.nx + 63) >> 6) * ((k.ny + 3) >> 2);
+  double a = 0.0;
+  for (int q = tid; q < nblk; q += SW_T) a = fmax2(a, b.apart[q]);
+  a = block_max<SW_T>(a, red, tid);
+  if (tid == 0) {
+    g->alpha_bits[slot_out] = (unsigned long long)__double_as_longlong(fmax2(a, k.alpha_const));
+    if (stage == 3) {
+      const double tn = g->time[parity] + g->dt;
+      g->time[1 - parity] = tn;
+      g->steps_done++;
+      g->active[1 - parity] = (g->status == ST_OK) && time_continue_q(tn, g->tfinal);
+    }
+  }
 }
 
 // alpha max over every cell (ghosts included), for a freshly uploaded state

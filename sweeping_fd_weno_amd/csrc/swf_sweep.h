@@ -19,15 +19,16 @@
 //     convergence test (global min P and rho) reduces to "no cell left flagged".
 // Statistics (sweep_count, den_count, pre_count, per-trigger q / negative /
 // positive adjustment counts) are accumulated exactly as the reference does.
-// One workgroup of 1024 threads runs the whole sweep (all q iterations).
+// One workgroup of NT threads runs the whole sweep (all q iterations): either
+// its own single-workgroup kernel, or the last workgroup of the RK update kernel.
 #pragma once
 
 #include "swf_device.h"
 
 namespace swf {
 
-constexpr int SW_T = 1024;    // threads of the sweep workgroup
-constexpr int SW_WCH = 256;   // bitmap words per scan chunk (8192 positions)
+constexpr int SW_WCH = 256;          // bitmap words per scan chunk (8192 positions)
+constexpr int SW_DIRTY_WORDS = 256;  // dirty-block bitmap: up to 8192 update blocks
 
 SWF_D void cell_of(const KCfg &k, int order, int p, int &i, int &j) {
   if (order == 0)
@@ -37,26 +38,52 @@ SWF_D void cell_of(const KCfg &k, int order, int p, int &i, int &j) {
 }
 SWF_D int pos_of(const KCfg &k, int order, int i, int j) { return order == 0 ? posI(k, i, j) : posII(k, i, j); }
 
-SWF_D void load_cell(const KCfg &k, const Bufs &b, int order, int p, double v[4]) {
-  int i, j;
-  cell_of(k, order, p, i, j);
+SWF_D void set_bit(unsigned *m, int p) { atomicOr(&m[p >> 5], 1u << (p & 31)); }
+SWF_D void clear_bit(unsigned *m, int p) { atomicAnd(&m[p >> 5], ~(1u << (p & 31))); }
+
+// update-kernel block holding interior cell (i, j): blocks of 64 x 4 cells
+SWF_D int ublock_of(const KCfg &k, int i, int j) { return ((j - 1) >> 2) * ((k.nx + 63) >> 6) + ((i - 1) >> 6); }
+
+template <int NT>
+struct SweepShared {
+  union {
+    int list[SW_WCH * 32];  // flagged positions of one scan chunk
+    double prod[4][NT];     // u_bar partial products
+  };
+  int ends[NT];
+  int wscan[SW_WCH];
+  unsigned dirty[SW_DIRTY_WORDS];
+  unsigned char genuine[NT];
+  int nlist, last_end, flag, all_dirty;
+  long long tpos, tneg;
+  double ubar[4];
+  double red[4][NT / 64];
+};
+
+template <int NT>
+SWF_D void mark_dirty(const KCfg &k, SweepShared<NT> &sh, int i, int j) {
+  const int blk = ublock_of(k, i, j);
+  if (blk < SW_DIRTY_WORDS * 32)
+    atomicOr(&sh.dirty[blk >> 5], 1u << (blk & 31));
+  else
+    sh.all_dirty = 1;
+}
+
+SWF_D void load_ij(const KCfg &k, const Bufs &b, int i, int j, double v[4]) {
   const long long o = cidx(k, i, j);
 #pragma unroll
   for (int c = 0; c < 4; ++c) v[c] = b.U[c * k.plane + o];
 }
-SWF_D void store_cell(const KCfg &k, const Bufs &b, int order, int p, const double v[4]) {
-  int i, j;
-  cell_of(k, order, p, i, j);
+SWF_D void store_ij(const KCfg &k, const Bufs &b, int i, int j, const double v[4]) {
   const long long o = cidx(k, i, j);
 #pragma unroll
   for (int c = 0; c < 4; ++c) b.U[c * k.plane + o] = v[c];
 }
-SWF_D void set_bit(unsigned *m, int p) { atomicOr(&m[p >> 5], 1u << (p & 31)); }
-SWF_D void clear_bit(unsigned *m, int p) { atomicAnd(&m[p >> 5], ~(1u << (p & 31))); }
 
 // nod_lim (ex4_sedov_2d/main.f90:1359-1396); returns true for the positive branch
 SWF_D bool nod_lim(double g1, double eps, const double v1[4], const double v2[4], const double vb[4],
                    double out[4]) {
+#pragma clang fp contract(off)
   double p1 = g1 * (v1[3] - (0.5 * (v1[1] * v1[1] + v1[2] * v1[2])) / v1[0]);
   double p2 = g1 * (v2[3] - (0.5 * (v2[1] * v2[1] + v2[2] * v2[2])) / v2[0]);
   double t;
@@ -81,40 +108,31 @@ SWF_D bool nod_lim(double g1, double eps, const double v1[4], const double v2[4]
   return pos;
 }
 
-struct SweepShared {
-  union {
-    int list[SW_WCH * 32];        // flagged positions of one scan chunk
-    double prod[4][SW_T];         // u_bar partial products
-  };
-  int ends[SW_T];
-  int wscan[SW_WCH];
-  unsigned char genuine[SW_T];
-  int nlist, last_end, flag;
-  long long tpos, tneg;
-  double ubar[4];
-  double red[4][SW_T / 64];
-};
-
 // Pressure-sweep walk from position f in `order`.  FWD: neighbour f+1, else f-1.
 // WRITE == false: speculative, returns the stop position only.
-template <bool FWD, bool WRITE>
+template <bool FWD, bool WRITE, int NT>
 SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], unsigned *next_map,
-                long long *npos, long long *nneg) {
+                long long *npos, long long *nneg, SweepShared<NT> *sh) {
+#pragma clang fp contract(off)
   const int N = k.nx * k.ny;
   const double g1 = k.gamma1, eps = k.eps, wr = k.weight / k.weight;
-  int p = f;
+  int p = f, i, j;
+  cell_of(k, order, p, i, j);
   double v[4];
-  load_cell(k, b, order, p, v);
+  load_ij(k, b, i, j, v);
   while (true) {
     const int nb = FWD ? p + 1 : p - 1;
+    int ni, nj;
+    cell_of(k, order, nb, ni, nj);
     double vn[4], vt[4];
-    load_cell(k, b, order, nb, vn);
+    load_ij(k, b, ni, nj, vn);
     const bool pos = nod_lim(g1, eps, v, vn, ub, vt);
     double vn2[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) vn2[c] = vn[c] + wr * (v[c] - vt[c]);
     if (WRITE) {
-      store_cell(k, b, order, p, vt);
+      store_ij(k, b, i, j, vt);
+      mark_dirty(k, *sh, i, j);
       if (sweep_flag(g1, eps, vt[0], vt[1], vt[2], vt[3])) set_bit(next_map, p);
       if (pos)
         ++*npos;
@@ -122,19 +140,25 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
         ++*nneg;
     }
     p = nb;
+    i = ni;
+    j = nj;
 #pragma unroll
     for (int c = 0; c < 4; ++c) v[c] = vn2[c];
     const bool edge = FWD ? (p == N - 1) : (p == 0);
     if (edge || !sweep_flag(g1, eps, v[0], v[1], v[2], v[3])) {
-      if (WRITE) store_cell(k, b, order, p, v);
+      if (WRITE) {
+        store_ij(k, b, i, j, v);
+        mark_dirty(k, *sh, i, j);
+      }
       return p;
     }
   }
 }
 
-// Density-sweep walk (scalar_sweep), forward in order I.
-template <bool WRITE>
-SWF_D int swalk(const KCfg &k, const Bufs &b, int f, int dir) {
+// Density-sweep walk (scalar_sweep) in order I; dir +1 forward, -1 backward.
+template <bool WRITE, int NT>
+SWF_D int swalk(const KCfg &k, const Bufs &b, int f, int dir, SweepShared<NT> *sh) {
+#pragma clang fp contract(off)
   const int N = k.nx * k.ny;
   const double delta = k.delta, wr = k.weight / k.weight;
   int p = f, i, j;
@@ -147,8 +171,9 @@ SWF_D int swalk(const KCfg &k, const Bufs &b, int f, int dir) {
     const double vn = b.U[cidx(k, ni, nj)];
     const double vn2 = vn + wr * (v - delta);
     if (WRITE) {
-      b.U[cidx(k, i, j)] = delta;
       const long long o = cidx(k, i, j);
+      b.U[o] = delta;
+      mark_dirty(k, *sh, i, j);
       if (sweep_flag(k.gamma1, k.eps, delta, b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]))
         set_bit(b.bmP, p);
       else
@@ -163,6 +188,7 @@ SWF_D int swalk(const KCfg &k, const Bufs &b, int f, int dir) {
       if (WRITE) {
         const long long o = cidx(k, i, j);
         b.U[o] = v;
+        mark_dirty(k, *sh, i, j);
         if (sweep_flag(k.gamma1, k.eps, v, b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]))
           set_bit(b.bmP, p);
         else
@@ -174,16 +200,16 @@ SWF_D int swalk(const KCfg &k, const Bufs &b, int f, int dir) {
 }
 
 // Collect the set bits of words [w0, w0+SW_WCH) into sh.list (ascending).
-SWF_D void collect_chunk(const KCfg &k, const unsigned *map, int w0, SweepShared &sh) {
-  const int tid = threadIdx.x;
+template <int NT>
+SWF_D void collect_chunk(const KCfg &k, const unsigned *map, int w0, SweepShared<NT> &sh, int tid) {
+  static_assert(NT >= SW_WCH, "the scan needs one thread per bitmap word");
   unsigned word = 0;
   if (tid < SW_WCH) {
     word = (w0 + tid < k.nwords) ? map[w0 + tid] : 0u;
     sh.wscan[tid] = __popc(word);
   }
   __syncthreads();
-  // inclusive scan over SW_WCH counts (Hillis-Steele)
-  for (int off = 1; off < SW_WCH; off <<= 1) {
+  for (int off = 1; off < SW_WCH; off <<= 1) {  // inclusive scan (Hillis-Steele)
     int v = 0;
     if (tid < SW_WCH && tid >= off) v = sh.wscan[tid - off];
     __syncthreads();
@@ -205,30 +231,29 @@ SWF_D void collect_chunk(const KCfg &k, const unsigned *map, int w0, SweepShared
 
 // One directional pass of the pressure sweep over the flagged positions in
 // `map` (order `order`).  Adjusted-but-still-flagged cells are recorded in `next`.
-template <bool FWD>
+template <bool FWD, int NT>
 SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned *map, unsigned *next,
-                         const double ub[4], SweepShared &sh) {
-  const int tid = threadIdx.x;
+                         const double ub[4], SweepShared<NT> &sh, int tid) {
   const int N = k.nx * k.ny;
   if (tid == 0) sh.last_end = FWD ? -1 : N;
   __syncthreads();
   const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
   for (int ci = 0; ci < nchunks; ++ci) {
     const int c = FWD ? ci : (nchunks - 1 - ci);
-    collect_chunk(k, map, c * SW_WCH, sh);
+    collect_chunk<NT>(k, map, c * SW_WCH, sh, tid);
     const int n = sh.nlist;
-    for (int s0 = 0; s0 < n; s0 += SW_T) {
+    for (int s0 = 0; s0 < n; s0 += NT) {
       const int idx = s0 + tid;
       int f = -1;
       if (idx < n) f = FWD ? sh.list[idx] : sh.list[n - 1 - idx];
       const int le = sh.last_end;
       const bool cand = (f >= 0) && (FWD ? (f > le && f <= N - 2) : (f < le && f >= 1));
-      sh.ends[tid] = cand ? pwalk<FWD, false>(k, b, order, f, ub, nullptr, nullptr, nullptr) : -1;
+      sh.ends[tid] = cand ? pwalk<FWD, false, NT>(k, b, order, f, ub, nullptr, nullptr, nullptr, nullptr) : -1;
       sh.genuine[tid] = 0;
       __syncthreads();
       if (tid == 0) {
         int last = sh.last_end;
-        const int m = min(SW_T, n - s0);
+        const int m = min(NT, n - s0);
         for (int q = 0; q < m; ++q) {
           const int fq = FWD ? sh.list[s0 + q] : sh.list[n - 1 - (s0 + q)];
           const bool ok = FWD ? (fq > last && fq <= N - 2) : (fq < last && fq >= 1);
@@ -242,7 +267,7 @@ SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned
       __syncthreads();
       if (sh.genuine[tid]) {
         long long np = 0, nn = 0;
-        pwalk<FWD, true>(k, b, order, f, ub, next, &np, &nn);
+        pwalk<FWD, true, NT>(k, b, order, f, ub, next, &np, &nn, &sh);
         atomicAdd((unsigned long long *)&sh.tpos, (unsigned long long)np);
         atomicAdd((unsigned long long *)&sh.tneg, (unsigned long long)nn);
       }
@@ -252,28 +277,32 @@ SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned
   // the position the pass never examines (N-1 forward, 0 backward)
   if (tid == 0) {
     const int pe = FWD ? N - 1 : 0;
+    int i, j;
+    cell_of(k, order, pe, i, j);
     double v[4];
-    load_cell(k, b, order, pe, v);
+    load_ij(k, b, i, j, v);
     if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) set_bit(next, pe);
   }
   __syncthreads();
 }
 
-SWF_D void clear_map(const KCfg &k, unsigned *m) {
-  for (int w = threadIdx.x; w < k.nwords; w += SW_T) m[w] = 0u;
+template <int NT>
+SWF_D void clear_map(const KCfg &k, unsigned *m, int tid) {
+  for (int w = tid; w < k.nwords; w += NT) m[w] = 0u;
   __syncthreads();
 }
 
 // u_bar = sum of weights*u over the interior.  Exact mode: the reference's serial
 // order (j outer, i inner, ex4_sedov_2d/main.f90:1431-1438), one lane per
 // component.  Otherwise a fixed-order parallel sum.
-SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared &sh) {
-  const int tid = threadIdx.x;
+template <int NT>
+SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared<NT> &sh, int tid) {
+#pragma clang fp contract(off)
   const long long N = (long long)k.nx * k.ny;
   const double w = k.weight;
   if (serial) {
     double acc = 0.0;  // lane c of wave 0 accumulates component c
-    for (long long c0 = 0; c0 < N; c0 += SW_T) {
+    for (long long c0 = 0; c0 < N; c0 += NT) {
       const long long t = c0 + tid;
       if (t < N) {
         const int i = 1 + (int)(t % k.nx), j = 1 + (int)(t / k.nx);
@@ -283,7 +312,7 @@ SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared &
       }
       __syncthreads();
       if (tid < 4) {
-        const int m = (int)min((long long)SW_T, N - c0);
+        const int m = (int)min((long long)NT, N - c0);
         for (int q = 0; q < m; ++q) acc = acc + sh.prod[tid][q];
       }
       __syncthreads();
@@ -291,7 +320,7 @@ SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared &
     if (tid < 4) sh.ubar[tid] = acc;
   } else {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (long long t = tid; t < N; t += SW_T) {
+    for (long long t = tid; t < N; t += NT) {
       const int i = 1 + (int)(t % k.nx), j = 1 + (int)(t / k.nx);
       const long long o = cidx(k, i, j);
 #pragma unroll
@@ -306,180 +335,172 @@ SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared &
     __syncthreads();
     if (tid < 4) {
       double s = 0.0;
-      for (int q = 0; q < SW_T / 64; ++q) s = s + sh.red[tid][q];
+      for (int q = 0; q < NT / 64; ++q) s = s + sh.red[tid][q];
       sh.ubar[tid] = s;
     }
   }
   __syncthreads();
 }
 
-// The whole sweep of one stage, plus the end-of-step time update at stage 3.
-__global__ __launch_bounds__(SW_T) void k_sweep(KCfg k, Bufs b, int stage, int slot_out, int parity,
-                                                int serial_ubar, int force) {
-  __shared__ SweepShared sh;
+// The sweep of one stage (sweep_lim incl. scalar_sweep), run by one workgroup of
+// NT threads.  Marks the update blocks whose cells it changed in sh.dirty.
+template <int NT>
+SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, SweepShared<NT> &sh, int tid) {
   DevScalars *g = b.g;
-  const int tid = threadIdx.x;
-  if (parity >= 0 && !g->active[parity]) {
-    // an inactive step leaves the next one inactive too
-    if (stage == 3 && tid == 0) g->active[1 - parity] = 0;
-    return;
+  const int N = k.nx * k.ny;
+  for (int w = tid; w < SW_DIRTY_WORDS; w += NT) sh.dirty[w] = 0u;
+  if (tid == 0) {
+    g->stage_sweeps++;
+    sh.tpos = 0;
+    sh.tneg = 0;
+    sh.all_dirty = 0;
   }
-  const bool run = (k.use_limiter == 2) && (force || g->postag);
-  if (run) {
-    const int N = k.nx * k.ny;
-    if (tid == 0) {
-      g->stage_sweeps++;
-      sh.tpos = 0;
-      sh.tneg = 0;
-    }
-    // ---- scalar_sweep (density), order I
-    if (g->rho_any) {
-      if (tid == 0) sh.last_end = -1;
-      __syncthreads();
-      const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
-      for (int c = 0; c < nchunks; ++c) {
-        collect_chunk(k, b.bmR, c * SW_WCH, sh);
-        const int n = sh.nlist;
-        for (int s0 = 0; s0 < n; s0 += SW_T) {
-          const int idx = s0 + tid;
-          const int f = (idx < n) ? sh.list[idx] : -1;
-          const int le = sh.last_end;
-          const bool cand = (f >= 0) && f > le && f <= N - 2;
-          sh.ends[tid] = cand ? swalk<false>(k, b, f, +1) : -1;
-          sh.genuine[tid] = 0;
-          __syncthreads();
-          if (tid == 0) {
-            int last = sh.last_end;
-            const int m = min(SW_T, n - s0);
-            for (int q = 0; q < m; ++q) {
-              const int fq = sh.list[s0 + q];
-              if (fq > last && fq <= N - 2) {
-                sh.genuine[q] = 1;
-                last = sh.ends[q];
-              }
+  __syncthreads();
+  // ---- scalar_sweep (density), order I
+  if (g->rho_any) {
+    if (tid == 0) sh.last_end = -1;
+    __syncthreads();
+    const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
+    for (int c = 0; c < nchunks; ++c) {
+      collect_chunk<NT>(k, b.bmR, c * SW_WCH, sh, tid);
+      const int n = sh.nlist;
+      for (int s0 = 0; s0 < n; s0 += NT) {
+        const int idx = s0 + tid;
+        const int f = (idx < n) ? sh.list[idx] : -1;
+        const int le = sh.last_end;
+        const bool cand = (f >= 0) && f > le && f <= N - 2;
+        sh.ends[tid] = cand ? swalk<false, NT>(k, b, f, +1, nullptr) : -1;
+        sh.genuine[tid] = 0;
+        __syncthreads();
+        if (tid == 0) {
+          int last = sh.last_end;
+          const int m = min(NT, n - s0);
+          for (int q = 0; q < m; ++q) {
+            const int fq = sh.list[s0 + q];
+            if (fq > last && fq <= N - 2) {
+              sh.genuine[q] = 1;
+              last = sh.ends[q];
             }
-            sh.last_end = last;
           }
-          __syncthreads();
-          if (sh.genuine[tid]) swalk<true>(k, b, f, +1);
-          __syncthreads();
+          sh.last_end = last;
         }
+        __syncthreads();
+        if (sh.genuine[tid]) swalk<true, NT>(k, b, f, +1, &sh);
+        __syncthreads();
       }
-      // backward pass: after the forward pass only position N-1 can be below delta
+    }
+    // backward pass: after the forward pass only position N-1 can be below delta
+    if (tid == 0) {
+      int i, j;
+      cellI(k, N - 1, i, j);
+      if (b.U[cidx(k, i, j)] < k.delta) swalk<true, NT>(k, b, N - 1, -1, &sh);
+      g->den_count++;
+    }
+    __syncthreads();
+  }
+  // ---- u_bar and the P(u_bar) > 0 precondition
+  compute_ubar<NT>(k, b, serial_ubar, sh, tid);
+  double ub[4] = {sh.ubar[0], sh.ubar[1], sh.ubar[2], sh.ubar[3]};
+  if (k.pbar_check) {
+    const double pb = k.gamma1 * (ub[3] - (0.5 * (ub[1] * ub[1] + ub[2] * ub[2])) / ub[0]);
+    if (pb <= 0.) {
       if (tid == 0) {
+        g->status = ST_PBAR;
+        g->pbar = pb;
+      }
+      __syncthreads();
+      return;
+    }
+  }
+  long long sc = 0;
+  if (tid == 0) {
+    g->sweep_count++;
+    sc = g->sweep_count;
+    sh.flag = 0;
+  }
+  unsigned *A = b.bmP, *B = b.bmA;
+  int qdone = 0;
+  for (int q = 1; q <= k.swp_num; ++q) {
+    const int order = (q & 1) ? 0 : 1;
+    clear_map<NT>(k, B, tid);
+    pressure_pass<true, NT>(k, b, order, A, B, ub, sh, tid);
+    clear_map<NT>(k, A, tid);
+    pressure_pass<false, NT>(k, b, order, B, A, ub, sh, tid);
+    // convergence test (:1714-1744): no cell left with P <= eps or rho <= eps
+    // (ex5: P < eps or rho < eps)
+    if (tid == 0) {
+      sh.flag = 0;
+      g->pre_count++;
+    }
+    __syncthreads();
+    for (int w = tid; w < k.nwords; w += NT) {
+      unsigned word = A[w];
+      while (word) {
+        const int bit = __ffs(word) - 1;
+        word &= word - 1;
+        bool fail = true;
+        if (k.success_ge) {
+          int i, j;
+          cell_of(k, order, w * 32 + bit, i, j);
+          double v[4];
+          load_ij(k, b, i, j, v);
+          const double p = sweep_pres(k.gamma1, v[0], v[1], v[2], v[3]);
+          fail = (p < k.eps) || (v[0] < k.eps);
+        }
+        if (fail) atomicOr(&sh.flag, 1);
+      }
+    }
+    __syncthreads();
+    if (!sh.flag) {
+      qdone = q;
+      break;
+    }
+    if (q == k.swp_num) break;
+    // flagged set of the next pass, in the other snake order
+    clear_map<NT>(k, B, tid);
+    for (int w = tid; w < k.nwords; w += NT) {
+      unsigned word = A[w];
+      while (word) {
+        const int bit = __ffs(word) - 1;
+        word &= word - 1;
         int i, j;
-        cellI(k, N - 1, i, j);
-        if (b.U[cidx(k, i, j)] < k.delta) swalk<true>(k, b, N - 1, -1);
-        g->den_count++;
-      }
-      __syncthreads();
-    }
-    // ---- u_bar and the P(u_bar) > 0 precondition
-    compute_ubar(k, b, serial_ubar != 0, sh);
-    double ub[4] = {sh.ubar[0], sh.ubar[1], sh.ubar[2], sh.ubar[3]};
-    bool abort = false;
-    if (k.pbar_check) {
-      const double pb = k.gamma1 * (ub[3] - (0.5 * (ub[1] * ub[1] + ub[2] * ub[2])) / ub[0]);
-      if (pb <= 0.) {
-        if (tid == 0) {
-          g->status = ST_PBAR;
-          g->pbar = pb;
-        }
-        abort = true;
+        cell_of(k, order, w * 32 + bit, i, j);
+        set_bit(B, pos_of(k, 1 - order, i, j));
       }
     }
-    if (!abort) {
-      long long sc = 0;
-      if (tid == 0) {
-        g->sweep_count++;
-        sc = g->sweep_count;
-        sh.flag = 0;
-      }
-      unsigned *A = b.bmP, *B = b.bmA;
-      int qdone = 0;
-      for (int q = 1; q <= k.swp_num; ++q) {
-        const int order = (q & 1) ? 0 : 1;
-        clear_map(k, B);
-        pressure_pass<true>(k, b, order, A, B, ub, sh);
-        clear_map(k, A);
-        pressure_pass<false>(k, b, order, B, A, ub, sh);
-        // convergence test (:1714-1744): no cell left with P <= eps or rho <= eps
-        // (ex5: P < eps or rho < eps)
-        if (tid == 0) {
-          sh.flag = 0;
-          g->pre_count++;
-        }
-        __syncthreads();
-        for (int w = tid; w < k.nwords; w += SW_T) {
-          unsigned word = A[w];
-          while (word) {
-            const int bit = __ffs(word) - 1;
-            word &= word - 1;
-            double v[4];
-            load_cell(k, b, order, w * 32 + bit, v);
-            bool fail;
-            if (k.success_ge) {
-              const double p = sweep_pres(k.gamma1, v[0], v[1], v[2], v[3]);
-              fail = (p < k.eps) || (v[0] < k.eps);
-            } else {
-              fail = true;
-            }
-            if (fail) atomicOr(&sh.flag, 1);
-          }
-        }
-        __syncthreads();
-        if (!sh.flag) {
-          qdone = q;
-          break;
-        }
-        if (q == k.swp_num) break;
-        // flagged set of the next pass, in the other snake order
-        clear_map(k, B);
-        for (int w = tid; w < k.nwords; w += SW_T) {
-          unsigned word = A[w];
-          while (word) {
-            const int bit = __ffs(word) - 1;
-            word &= word - 1;
-            int i, j;
-            cell_of(k, order, w * 32 + bit, i, j);
-            set_bit(B, pos_of(k, 1 - order, i, j));
-          }
-        }
-        __syncthreads();
-        unsigned *t = A;
-        A = B;
-        B = t;
-      }
-      if (tid == 0) {
-        long long qv = qdone, nn = qdone ? sh.tneg : 0, np = qdone ? sh.tpos : 0;
-        if (!qdone) {
-          g->fail_q = k.swp_num;
-          if (k.fail_stop) g->status = ST_SWEEP_FAIL;
-        }
-        if (sc - 1 < k.track_cap) {
-          b.trk[sc - 1] = qv;
-          b.trk[k.track_cap + sc - 1] = nn;
-          b.trk[2 * k.track_cap + sc - 1] = np;
-        }
-        g->total_sweep += qv;
-        g->total_neg += nn;
-        g->total_pos += np;
-        if (qv > g->max_sweep) g->max_sweep = qv;
-        if (nn > g->max_neg) g->max_neg = nn;
-        if (np > g->max_pos) g->max_pos = np;
-      }
-    }
-    if (tid == 0) {
-      g->swept = 1;
-      if (slot_out >= 0) g->alpha_bits[slot_out] = 0ull;
-    }
+    __syncthreads();
+    unsigned *t = A;
+    A = B;
+    B = t;
   }
-  if (stage == 3 && parity >= 0 && tid == 0) {
-    const double tn = g->time[parity] + g->dt;
-    g->time[1 - parity] = tn;
-    g->steps_done++;
-    g->active[1 - parity] = (g->status == ST_OK) && time_continue_q(tn, g->tfinal);
+  if (tid == 0) {
+    long long qv = qdone, nn = qdone ? sh.tneg : 0, np = qdone ? sh.tpos : 0;
+    if (!qdone) {
+      g->fail_q = k.swp_num;
+      if (k.fail_stop) g->status = ST_SWEEP_FAIL;
+    }
+    if (sc - 1 < k.track_cap) {
+      b.trk[sc - 1] = qv;
+      b.trk[k.track_cap + sc - 1] = nn;
+      b.trk[2 * k.track_cap + sc - 1] = np;
+    }
+    g->total_sweep += qv;
+    g->total_neg += nn;
+    g->total_pos += np;
+    if (qv > g->max_sweep) g->max_sweep = qv;
+    if (nn > g->max_neg) g->max_neg = nn;
+    if (np > g->max_pos) g->max_pos = np;
   }
+  __syncthreads();
 }
+
+constexpr int SW_T = 1024;
+#ifdef SWF_COMMON_KERNELS
+// Stand-alone sweep of the current state (the sweep_lim operator on its own).
+__global__ __launch_bounds__(SW_T) void k_sweep(KCfg k, Bufs b, int serial_ubar) {
+  __shared__ SweepShared<SW_T> sh;
+  if (k.use_limiter == 2) sweep_body<SW_T>(k, b, serial_ubar != 0, sh, threadIdx.x);
+}
+#endif
 
 }  // namespace swf
