@@ -1,0 +1,143 @@
+"""Benchmark: cell-stage updates/s on 2D Sedov 640^2 to T = 1, FP64 vs FP32-bulk.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One step = one SSP-RK3 step (three stages: WENO fluxes, RK update with the
+positivity check, sweep when flagged) over the whole 640^2 domain, inputs
+resident in HBM.  By default the timed region is the complete run from t = 0 to
+T = 1 (28650 steps), so it contains every sweep the run triggers.  The state is
+re-initialised after the warmup steps (untimed).  Each precision mode is timed
+separately; `value` is the plain FP64 path, the FP32-bulk and FP64-exact paths are
+reported beside it together with their L1 density error against FP64-exact, which
+is bit-identical to the reference's FP64 run.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-stage updates/s, 2D Sedov 640²: FP64 vs FP32-bulk at stated L1 error"
+N = 640
+T_END = 1.0
+STEPS_TO_T1 = 28650         # SSP-RK3 steps of Sedov 640^2 (cfl 0.5) from t = 0 to T = 1
+FLOPS_PER_CELL_DIR = 906.5  # algorithmic flops of one direction of wenofd per cell
+FP64_VALU_PEAK = 78.6e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=STEPS_TO_T1)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--modes", default="fp64,fp32_bulk,fp64_exact")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    return ap.parse_args()
+
+
+def run_mode(S, prec: str, steps: int, warmup: int):
+    cfg = S.sedov_config(N, prec)
+    u0 = S.init_sedov(cfg)
+    tfinal = T_END if steps <= STEPS_TO_T1 else 1e30
+    s = S.Solver(cfg)
+    s.upload(u0)
+    if warmup:
+        s.advance(0.0, tfinal, max_steps=warmup)
+    s.close()
+    s = S.Solver(cfg)
+    s.upload(u0)
+    s.sync()
+    t0 = time.perf_counter()
+    t, k = s.advance(0.0, tfinal, max_steps=steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    flux_ms = s.time_kernel("flux", 20)
+    rho = s.download()[3:-3, 3:-3, 0].copy()
+    stats = s.stats()
+    s.close()
+    return dict(steps=k, t_end=t, seconds=wall, value=N * N * 3 * k / wall, ms_per_step=1e3 * wall / max(k, 1),
+                flux_kernel_ms=flux_ms, stats=stats), rho
+
+
+def cpu_baseline(steps: int):
+    from oracle import oracle as O  # the checker, timed as the CPU baseline
+    oc = O.sedov_config(N)
+    o = O.Oracle(oc)
+    o.set_state(O.init_sedov(oc))
+    o.bc()
+    t0 = time.perf_counter()
+    rc, t, k = o.run(0.0, T_END, max_steps=steps)
+    wall = time.perf_counter() - t0
+    return {"value": N * N * 3 * k / wall, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
+            "sample": f"{k} SSP-RK3 step(s) of Sedov 640^2 from t=0 by the C oracle (FP64 with binary128 sites "
+                      f"as the reference build), single thread, {wall:.1f} s"}
+
+
+def traffic():
+    p = os.path.join(ROOT, "profiles", "flux_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not provided (see DESIGN.md, "
+                                   "sharding)", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import sweeping_fd_weno_amd as S
+
+    modes = a.modes.split(",")
+    res, rho = {}, {}
+    for m in modes:
+        res[m], rho[m] = run_mode(S, m, a.steps, a.warmup)
+    err = {}
+    if "fp64_exact" in rho:
+        ref = rho["fp64_exact"]
+        for m in modes:
+            if m != "fp64_exact":
+                err[m] = float(np.abs(rho[m] - ref).sum() / np.abs(ref).sum())
+    head = res.get("fp64") or res[modes[0]]
+    flops = FLOPS_PER_CELL_DIR * 2 * N * N
+    ach = flops / (head["flux_kernel_ms"] * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": head["steps"],
+        "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "sedov2d_640x640_to_T1", "grid": [N, N], "cfl": 0.5, "t_end": T_END,
+                   "precision": "fp64", "parallelism": "single GPU, whole domain"},
+        "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": ach / FP64_VALU_PEAK, "traffic": traffic(),
+                     "kernel": "k_flux<fp64> (x and y interface fluxes)",
+                     "kernel_ms": head["flux_kernel_ms"]},
+        "modes": res,
+        "l1_density_rel_vs_fp64_exact": err,
+    }
+    if "fp32_bulk" in res and "fp64" in res:
+        out["fp32_bulk_speedup_vs_fp64"] = res["fp32_bulk"]["value"] / res["fp64"]["value"]
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -56,18 +56,20 @@ SWF_HD dd two_prod(double a, double b) {
   double e = fma(a, b, -p);
   return {p, e};
 }
+// Sums renormalise with two_sum (no magnitude precondition): under cancellation
+// the leading part can be smaller than the trailing one.
 SWF_HD dd dd_add(dd a, dd b) {
   dd s = two_sum(a.hi, b.hi);
   dd t = two_sum(a.lo, b.lo);
   double c = s.lo + t.hi;
-  dd v = quick_two_sum(s.hi, c);
+  dd v = two_sum(s.hi, c);
   double w = t.lo + v.lo;
-  return quick_two_sum(v.hi, w);
+  return two_sum(v.hi, w);
 }
 SWF_HD dd dd_add_d(dd a, double b) {
   dd s = two_sum(a.hi, b);
   double v = a.lo + s.lo;
-  return quick_two_sum(s.hi, v);
+  return two_sum(s.hi, v);
 }
 SWF_HD dd dd_mul(dd a, dd b) {
   dd p = two_prod(a.hi, b.hi);
@@ -90,12 +92,12 @@ SWF_HD dd d_div_dd(double a, dd b) {
   dd p = dd_mul_d(b, q1);
   dd r = two_sum(a, -p.hi);
   r.lo -= p.lo;
-  r = quick_two_sum(r.hi, r.lo);
+  r = two_sum(r.hi, r.lo);
   double q2 = r.hi / b.hi;
   p = dd_mul_d(b, q2);
   dd r2 = dd_add(r, dd{-p.hi, -p.lo});
   double q3 = r2.hi / b.hi;
-  dd q = quick_two_sum(q1, q2);
+  dd q = two_sum(q1, q2);
   return dd_add_d(q, q3);
 }
 SWF_HD dd dd_div_d(dd a, double b) {
@@ -106,7 +108,7 @@ SWF_HD dd dd_div_d(dd a, double b) {
   p = two_prod(q2, b);
   dd r2 = dd_add(r, dd{-p.hi, -p.lo});
   double q3 = r2.hi / b;
-  dd q = quick_two_sum(q1, q2);
+  dd q = two_sum(q1, q2);
   return dd_add_d(q, q3);
 }
 // round-to-nearest binary64 of hi + lo
@@ -123,33 +125,6 @@ struct qconst {
   // x < 1d-8 (binary128)  <=>  x < 0x1.5798ee2308c3ap-27.
   static constexpr double e8_hi = 0x1.5798ee2308c3ap-27;
 };
-
-// beta(1..3) of weno5.f90:100-102 (= :134-136).  `a` points at average(0).
-SWF_HD void beta_q(const double* a, double beta[3]) {
-  const dd c1312 = {qconst::c13_12_hi, qconst::c13_12_lo};
-  // beta(1): 13/12*(a0-2a1+a2)^2 + 0.25*(3a0-4a1+a2)^2
-  {
-    dd x = dd_add_d(two_sum(a[0], -2.0 * a[1]), a[2]);
-    dd y = dd_add_d(dd_add_d(two_prod(3.0, a[0]), -4.0 * a[1]), a[2]);
-    dd t = dd_add(dd_mul(c1312, dd_sqr(x)), dd_mul_d(dd_sqr(y), 0.25));
-    beta[0] = dd_round(t);
-  }
-  // beta(2): 13/12*(a-1-2a0+a1)^2 + 0.25*(binary64 (a-1 - a1)**2)
-  {
-    dd x = dd_add_d(two_sum(a[-1], -2.0 * a[0]), a[1]);
-    double yd = a[-1] - a[1];
-    double y2 = yd * yd;
-    dd t = dd_add_d(dd_mul(c1312, dd_sqr(x)), 0.25 * y2);
-    beta[1] = dd_round(t);
-  }
-  // beta(3): 13/12*(a-2-2a-1+a0)^2 + 0.25*(a-2-4a-1+3a0)^2
-  {
-    dd x = dd_add_d(two_sum(a[-2], -2.0 * a[-1]), a[0]);
-    dd y = dd_add(two_sum(a[-2], -4.0 * a[-1]), two_prod(3.0, a[0]));
-    dd t = dd_add(dd_mul(c1312, dd_sqr(x)), dd_mul_d(dd_sqr(y), 0.25));
-    beta[2] = dd_round(t);
-  }
-}
 
 // alpha = d/(1d-6 + beta)**2 with d the binary64 value of the REAL(8) literal
 SWF_HD double alpha_q(double d, double beta) {
@@ -171,12 +146,6 @@ SWF_HD double cm_q(double s) { return dd_round(dd_add_d(dd{qconst::e15_hi, qcons
 SWF_HD double yrecip_q(double den) {
   dd d = dd_add_d(dd{qconst::e15_hi, qconst::e15_lo}, den);
   return dd_round(d_div_dd(1.0, d));
-}
-
-// 0.75d0*u0 + 0.25d0*x  (both products exact in binary128)
-SWF_HD double rk2_q(double u0, double x) {
-  dd a = two_prod(0.75, u0);
-  return dd_round(dd_add_d(a, 0.25 * x));
 }
 
 // (u0 + 2d0*x)/3d0

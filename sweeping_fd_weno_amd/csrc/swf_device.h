@@ -15,7 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "swf_dd.h"
+#include "swf_q128.h"
 
 #define SWF_D __device__ __forceinline__
 

@@ -246,3 +246,16 @@ def test_advance_stops_at_final_time():
     assert_bitwise(s.download(), o.state(), "sedov to t=2e-5")
     ts2, ns2 = s.advance(ts, 2e-5, max_steps=10)
     assert ns2 == 0 and ts2 == ts
+
+
+def test_weno_exact_near_constant_stencils():
+    """Near-constant stencils make the smoothness indicators short binary numbers that
+    can fall exactly on a binary64 rounding midpoint; the device must still round as
+    binary128 does (exact soft-binary128 path)."""
+    rng = np.random.default_rng(11)
+    n = 300000
+    base = rng.standard_normal((n, 1)) * 10.0 ** rng.integers(-10, 10, (n, 1))
+    a = base * (1.0 + rng.standard_normal((n, 5)) * 2.0 ** -rng.integers(0, 50, (n, 1)).astype(float))
+    r, l = S.weno_points(a, "fp64_exact")
+    assert_bitwise(r, O.weno_right(a), "weno_right near-constant")
+    assert_bitwise(l, O.weno_left(a), "weno_left near-constant")
