@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--modes", default="fp64,fp32_bulk,fp64_exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one plain device copy of this size first (calibrates HBM counters under rocprofv3)")
     return ap.parse_args()
 
 
@@ -97,6 +99,13 @@ def main():
         sys.exit(2)
     import sweeping_fd_weno_amd as S
 
+    if a.copy_probe_mb:
+        import torch
+
+        x = torch.ones(a.copy_probe_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
     modes = a.modes.split(",")
     res, rho = {}, {}
     for m in modes:
