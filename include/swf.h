@@ -122,8 +122,10 @@ int swf_weno_points(const double *a, double *right, double *left, int64_t n, int
 
 /* Average duration (ms) of one launch of a hot kernel on the context's stream,
  * measured with HIP events over `nreps` back-to-back launches on the current state
- * (which: 0 = the interface-flux kernel, both directions of `wenofd`).  No
- * counterpart in the reference: measurement support. */
+ * (which: 0 = the x-flux kernel, the x-pass of `wenofd`; 1 = the stage kernel, the
+ * y-pass of `wenofd` with `scheme`, the RK combination, `checkpositivity` and the
+ * ghost images, run as stage 1 into a scratch state).  No counterpart in the
+ * reference: measurement support. */
 int swf_time_kernel(swf_ctx *ctx, int32_t which, int32_t nreps, double *avg_ms);
 
 int swf_time_continue(double time, double tfinal);

@@ -244,10 +244,12 @@ class Solver:
     def sweep(self) -> None:
         _check(lib().swf_sweep(self._h), "swf_sweep")
 
-    def time_kernel(self, which: str = "flux", nreps: int = 20) -> float:
-        """Average ms of one launch of the interface-flux kernel (HIP events on the solver's stream)."""
+    def time_kernel(self, which: str = "xflux", nreps: int = 20) -> float:
+        """Average ms of one launch of a stage kernel (HIP events on the solver's stream):
+        "xflux" (x-pass of wenofd) or "ystage" (y-pass fused with the RK update)."""
         ms = C.c_double(0.0)
-        _check(lib().swf_time_kernel(self._h, {"flux": 0}[which], nreps, C.byref(ms)), "swf_time_kernel")
+        _check(lib().swf_time_kernel(self._h, {"xflux": 0, "ystage": 1}[which], nreps, C.byref(ms)),
+               "swf_time_kernel")
         return ms.value
 
     def sync(self) -> None:

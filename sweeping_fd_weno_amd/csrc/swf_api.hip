@@ -38,7 +38,8 @@ constexpr int kCheckEvery = 256;  // steps between host checks of the device tim
 struct swf_ctx {
   swf_config cfg;
   KCfg k;
-  Bufs b;
+  Bufs b;              // outside a stage: Uin == U == U0 == S0
+  double *S1, *S2;     // the other two state buffers of the stage cycle
   int mode;
   hipStream_t stream;
   double *d_fortran;   // staging buffer in the Fortran layout
@@ -101,6 +102,9 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   k.nwords = (int)(((long long)cfg->nx * cfg->ny + 31) / 32);
   k.dx = cfg->dx;
   k.dy = cfg->dy;
+  k.idx = 1. / cfg->dx;
+  k.idy = 1. / cfg->dy;
+  k.ig1 = 1. / (cfg->gamma - 1.);
   k.gamma = cfg->gamma;
   k.gamma1 = cfg->gamma - 1.;
   k.cfl = cfg->cfl;
@@ -160,7 +164,7 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&trk, sizeof(long long) * 3 * (size_t)kTrackCap);
   double *stage = nullptr;
   if (e == hipSuccess) e = hipMalloc(&stage, sizeof(double) * c->n_fortran);
-  const size_t nblk = (size_t)((cfg->nx + 63) / 64) * ((cfg->ny + 3) / 4);
+  const size_t nblk = (size_t)ub_gx(k) * ub_gy(k);
   double *apart = nullptr;
   if (e == hipSuccess) e = hipMalloc(&apart, sizeof(double) * nblk);
   if (e == hipSuccess) e = hipMemsetAsync(apart, 0, sizeof(double) * nblk, c->stream);
@@ -183,9 +187,11 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   }
   const size_t pd = 4 * (size_t)k.plane;
   c->b.U = pool;
-  c->b.U0 = pool + pd;
-  c->b.Fx = pool + 2 * pd;
-  c->b.Fy = pool + 3 * pd;
+  c->b.Uin = pool;
+  c->b.U0 = pool;
+  c->S1 = pool + pd;
+  c->S2 = pool + 2 * pd;
+  c->b.Fx = pool + 3 * pd;
   c->b.R = pool + 4 * pd;
   c->b.bmR = maps;
   c->b.bmP = maps + k.nwords;
@@ -239,14 +245,25 @@ int swf_download(swf_ctx *c, double *u) {
   return check_launch("swf_download");
 }
 
-// one RK stage (s = 1, 2, 3) of one step with parity p: wenofd (both directions);
-// scheme + RK combination + checkpositivity + bc images; then one workgroup for
-// sweep_lim (when flagged), findalpha of the new state and, at s = 3, the time.
+// buffers of stage s: stencils from S0, S1, S2, results into S1, S2, S0
+static Bufs stage_bufs(const swf_ctx *c, int s) {
+  Bufs b = c->b;
+  double *S0 = c->b.U0;
+  b.Uin = (s == 1) ? S0 : (s == 2) ? c->S1 : c->S2;
+  b.U = (s == 1) ? c->S1 : (s == 2) ? c->S2 : S0;
+  return b;
+}
+
+// one RK stage (s = 1, 2, 3) of one step with parity p: wenofd x-pass; the
+// y-pass fused with scheme, the RK combination, checkpositivity and bc images;
+// then one workgroup for sweep_lim (when flagged), findalpha of the new state
+// and, at s = 3, the time.
 static void enqueue_stage(swf_ctx *c, int s, int p) {
   const int slot_in = s - 1, slot_out = s % 3;
-  launch_flux(c->mode, c->k, c->b, c->stream, slot_in, slot_out, p, 1);
-  launch_update(c->mode, c->k, c->b, c->stream, s, slot_in, slot_out, p, 0);
-  launch_stage_end(c->k, c->b, c->stream, s, slot_out, p, c->mode == MODE_EXACT ? 1 : 0);
+  const Bufs b = stage_bufs(c, s);
+  launch_xflux(c->mode, c->k, b, c->stream, s, slot_in, p);
+  launch_ystage(c->mode, c->k, b, c->stream, s, slot_in, p, 0);
+  launch_stage_end(c->k, b, c->stream, s, slot_out, p, c->mode == MODE_EXACT ? 1 : 0);
 }
 
 static void enqueue_step(swf_ctx *c, int p) {
@@ -362,8 +379,8 @@ int swf_rhs(swf_ctx *c, double alphamax, double *rhs_out) {
   rc = read_scalars(c, &h0);
   if (rc) return rc;
   launch_set_alpha(c->b, c->stream, 0, bits, 0, 1);
-  launch_flux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
-  launch_update(c->mode, c->k, c->b, c->stream, 1, 0, 3, 0, 1);
+  launch_xflux(c->mode, c->k, c->b, c->stream, 0, 0, 0);
+  launch_ystage(c->mode, c->k, c->b, c->stream, 1, 0, 0, YS_RHS_ONLY);
   launch_to_fortran(c->k, c->b.R, c->d_fortran, c->stream);
   HIPC(hipMemcpyAsync(rhs_out, c->d_fortran, sizeof(double) * c->n_fortran, hipMemcpyDeviceToHost, c->stream));
   launch_set_alpha(c->b, c->stream, 0, h0.alpha_bits[0], 0, h0.active[0]);
@@ -437,10 +454,12 @@ int64_t swf_track(swf_ctx *c, int64_t *q, int64_t *neg, int64_t *pos, int64_t ca
 }
 
 int swf_time_kernel(swf_ctx *c, int32_t which, int32_t nreps, double *avg_ms) {
-  if (!c || !avg_ms || nreps < 1 || which != 0) return SWF_E_INVALID;
+  if (!c || !avg_ms || nreps < 1 || which < 0 || which > 1) return SWF_E_INVALID;
   int rc = set_device(c);
   if (rc) return rc;
-  // the flux kernel reads U and alpha slot 0 and writes only Fx / Fy
+  // which = 0: the x-flux kernel (reads S0 and alpha slot 0, writes Fx);
+  // which = 1: the stage-1 kernel (reads S0 and Fx, writes S1, the stage flags
+  // and alpha partials, which the next stage recomputes before any use)
   DevScalars h0;
   rc = read_scalars(c, &h0);
   if (rc) return rc;
@@ -448,9 +467,17 @@ int swf_time_kernel(swf_ctx *c, int32_t which, int32_t nreps, double *avg_ms) {
   hipEvent_t e0, e1;
   HIPC(hipEventCreate(&e0));
   HIPC(hipEventCreate(&e1));
-  launch_flux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);  // one untimed launch
+  const Bufs b1 = stage_bufs(c, 1);
+  auto launch = [&]() {
+    if (which == 0)
+      launch_xflux(c->mode, c->k, b1, c->stream, 0, 0, 0);
+    else
+      launch_ystage(c->mode, c->k, b1, c->stream, 1, 0, 0, YS_NO_SCALARS);
+  };
+  launch_xflux(c->mode, c->k, b1, c->stream, 0, 0, 0);  // Fx of the current state, untimed
+  launch();                                              // one untimed launch
   HIPC(hipEventRecord(e0, c->stream));
-  for (int r = 0; r < nreps; ++r) launch_flux(c->mode, c->k, c->b, c->stream, 0, 3, 0, 0);
+  for (int r = 0; r < nreps; ++r) launch();
   HIPC(hipEventRecord(e1, c->stream));
   HIPC(hipEventSynchronize(e1));
   float ms = 0.f;

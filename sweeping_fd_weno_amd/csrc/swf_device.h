@@ -5,17 +5,18 @@
 //   M == 0  FP64 exact: the reference's operation order, no contraction (the
 //           translation unit is compiled with -ffp-contract=off), binary128 sites
 //           emulated in double-double (swf_dd.h).  Bit-identical to the reference.
-//   M == 1  FP64: the same formulas in binary64, binary128 sites evaluated in
-//           binary64, contraction allowed.
-//   M == 2  FP32-bulk: smoothness indicators and nonlinear weights in FP32 with
-//           hardware reciprocals; state, fluxes, projections, RK, pressure,
-//           positivity tests and the sweep in FP64.
+//   M == 1  FP64: the same scheme in binary64 with contraction; reconstructions in
+//           first-difference form with one reciprocal each (swf_weno.h).
+//   M == 2  FP32-bulk: as M == 1, with the smoothness indicators, nonlinear
+//           weights and candidate combinations in FP32; state, split fluxes,
+//           eigen-projections, RK, pressure, positivity tests and the sweep in FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "swf_q128.h"
+#include "swf_weno.h"
 
 #define SWF_D __device__ __forceinline__
 
@@ -24,6 +25,7 @@ namespace swf {
 enum { MODE_EXACT = 0, MODE_FP64 = 1, MODE_FP32 = 2 };
 enum { BC_SEDOV = 0, BC_PERIODIC = 1, BC_JET_HALF = 2, BC_JET_FULL = 3 };
 enum { ST_OK = 0, ST_PBAR = 1, ST_SWEEP_FAIL = 2 };
+enum { YS_RHS_ONLY = 1, YS_NO_SCALARS = 2 };  // flags of the stage kernel
 
 // Kernel-side configuration (plain values, passed by value to every kernel).
 struct KCfg {
@@ -31,6 +33,7 @@ struct KCfg {
   int nwords;        // words of one sweep bitmap: ceil(nx*ny/32)
   long long plane;   // doubles per plane: (ny+6)*P
   double dx, dy, gamma, gamma1, cfl, eps, eps2, delta, weight, dt_num;
+  double idx, idy, ig1;   // 1/dx, 1/dy, 1/gamma1 (FP64 and FP32-bulk modes)
   double xlo, ylo, jet_rho, jet_mx, jet_e, amb_rho, amb_e, jet_halfwidth;
   double alpha_const;  // alpha of the prescribed boundary states (jet inflow/ambient), else 0
   int bc, use_limiter, swp_num, track_cap;
@@ -53,9 +56,12 @@ struct DevScalars {
   double pbar;               // P(u_bar) of the last pbar failure
 };
 
-// Device buffers of one solver context.
+// Device buffers of one solver context.  A step cycles three state buffers
+// S0 -> S1 -> S2 -> S0: stage s reads its stencils from Uin and writes U; the
+// RK combinations of stages 2 and 3 read the step's initial state U0 (= S0)
+// pointwise, so stage 3 writes S0 in place.  Outside a stage Uin == U == S0.
 struct Bufs {
-  double *U, *U0, *Fx, *Fy, *R;
+  double *U, *Uin, *U0, *Fx, *R;
   unsigned *bmR, *bmP, *bmA, *bmB;  // sweep bitmaps (order positions)
   DevScalars *g;
   long long *trk;                   // 3 * track_cap: q, neg, pos per sweep trigger
@@ -64,6 +70,13 @@ struct Bufs {
 
 // index of cell (i, j) (Fortran indices, ghosts from -2) inside a plane
 SWF_D long long cidx(const KCfg &k, int i, int j) { return (long long)(j + 2) * k.P + (i + 2); }
+
+// Update tiles: UB_W x UB_H interior cells per workgroup of the stage kernel;
+// each writes one alpha partial, and the sweep marks the tiles it changed.
+constexpr int UB_W = 16, UB_H = 15;
+SWF_HD int ub_gx(const KCfg &k) { return (k.nx + UB_W - 1) / UB_W; }
+SWF_HD int ub_gy(const KCfg &k) { return (k.ny + UB_H - 1) / UB_H; }
+SWF_HD int ub_of(const KCfg &k, int i, int j) { return ((j - 1) / UB_H) * ub_gx(k) + (i - 1) / UB_W; }
 
 // ---------------------------------------------------------------------------
 // Sweep orders (ex4_sedov_2d/main.f90:1836-1895), closed form, 0-based positions
@@ -94,6 +107,11 @@ SWF_D bool below_eps2(const KCfg &k, double rho) {
   return k.eps2_quad_literal ? (rho < qconst::e8_hi) : (rho < k.eps2);
 }
 
+// Roe-average inputs of one stencil point: sqrt(rho), velocities, enthalpy
+struct Prim {
+  double w, vx, vy, h;
+};
+
 // Lax-Friedrichs split of the Euler flux: fp = 0.5*(u + f/a), fn = 0.5*(u - f/a)
 // (ex4_sedov_2d/main.f90:125-177 fflux/gflux, :223-227 split).  DIR 0 = x, 1 = y.
 template <int M, int DIR>
@@ -119,7 +137,7 @@ SWF_D void split_point(const KCfg &k, const double u[4], double alpha, double ia
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    double fa = (M == MODE_FP32) ? f[c] * ia : f[c] / alpha;
+    double fa = f[c] / alpha;
     fp[c] = 0.5 * (u[c] + fa);
     fn[c] = 0.5 * (u[c] - fa);
   }
@@ -146,6 +164,48 @@ SWF_D void roe_prims(const KCfg &k, const double u[4], double &w, double &vx, do
   h = (pre + eng) * t0;
 }
 
+// Split fluxes and Roe primitives of one stencil point in the fast modes: the
+// formulas of split_point and roe_prims with one reciprocal of rho shared by the
+// velocities and a multiplication by 1/alpha.
+template <int DIR>
+SWF_D void stage_point_fast(const KCfg &k, const double u[4], double ia, double fp[4], double fn[4], Prim &pr) {
+  const double rho = u[0], mx = u[1], my = u[2], e = u[3];
+  const double r = 1.0 / rho;
+  const double vx0 = mx * r, vy0 = my * r;
+  const bool vac = below_eps2(k, rho);
+  const double vx = vac ? 0.0 : vx0, vy = vac ? 0.0 : vy0;
+  const double pres = k.gamma1 * ((e - (0.5 * rho) * (vx * vx)) - (0.5 * rho) * (vy * vy));
+  double f[4];
+  if (DIR == 0) {
+    f[0] = mx;
+    f[1] = rho * (vx * vx) + pres;
+    f[2] = (rho * vx) * vy;
+    f[3] = (e + pres) * vx;
+  } else {
+    f[0] = my;
+    f[1] = (rho * vx) * vy;
+    f[2] = rho * (vy * vy) + pres;
+    f[3] = (e + pres) * vy;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double fa = f[c] * ia;
+    fp[c] = 0.5 * (u[c] + fa);
+    fn[c] = 0.5 * (u[c] - fa);
+  }
+  double t0 = r, vex = vx0, vey = vy0;
+  if (DIR == 1 && !k.ydens_plain) {
+    t0 = 1.0 / (rho + 1e-15);
+    vex = mx * t0;
+    vey = my * t0;
+  }
+  const double pre = k.gamma1 * (e - 0.5 * (mx * vex + my * vey));
+  pr.w = sqrt(fabs(rho));
+  pr.vx = vex;
+  pr.vy = vey;
+  pr.h = (pre + e) * t0;
+}
+
 // ---------------------------------------------------------------------------
 // WENO-JS reconstructions (weno5.f90:86-146)
 // ---------------------------------------------------------------------------
@@ -164,142 +224,77 @@ SWF_D void roe_prims(const KCfg &k, const double u[4], double &w, double &vx, do
 #define SWF_C_2_1 (-1.16666666666666666666666666666667)
 #define SWF_C_2_2 1.83333333333333333333333333333333
 
-// Nonlinear weights in FP32 with hardware reciprocals (FP32-bulk mode).  The
-// weights are d_k/(1e-6+beta_k)^2 normalised; scaling every (1e-6+beta_k) by the
-// smallest one keeps all intermediates in [0, 1] so nothing overflows in FP32.
-SWF_D void weights_fp32(const double *a, float d0, float d1, float d2, float w[3]) {
-  float a2 = (float)a[-2], a1m = (float)a[-1], a0 = (float)a[0], a1 = (float)a[1], a2p = (float)a[2];
-  float x1 = a0 - 2.f * a1 + a2p, y1 = 3.f * a0 - 4.f * a1 + a2p;
-  float x2 = a1m - 2.f * a0 + a1, y2 = a1m - a1;
-  float x3 = a2 - 2.f * a1m + a0, y3 = a2 - 4.f * a1m + 3.f * a0;
-  const float c = 13.f / 12.f;
-  float s1 = 1e-6f + (c * x1 * x1 + 0.25f * y1 * y1);
-  float s2 = 1e-6f + (c * x2 * x2 + 0.25f * y2 * y2);
-  float s3 = 1e-6f + (c * x3 * x3 + 0.25f * y3 * y3);
-  float m = fminf(s1, fminf(s2, s3));
-  float r1 = m * __builtin_amdgcn_rcpf(s1), r2 = m * __builtin_amdgcn_rcpf(s2), r3 = m * __builtin_amdgcn_rcpf(s3);
-  float al1 = d0 * r1 * r1, al2 = d1 * r2 * r2, al3 = d2 * r3 * r3;
-  float inv = __builtin_amdgcn_rcpf(al1 + al2 + al3);
-  w[0] = al1 * inv;
-  w[1] = al2 * inv;
-  w[2] = al3 * inv;
-}
-
-// weno_right: value at x_{i+1/2}; `a` points at average(0)
+// weno_right: value at x_{i+1/2}; `a` points at average(0).  The exact mode
+// follows weno5.f90:120-146 operation by operation (beta and alpha at binary128);
+// the fast modes use the first-difference forms of swf_weno.h.
 template <int M>
 SWF_D double weno_right(const double *a) {
-  double um1 = ((0.0 + SWF_C_2_0 * a[-2]) + SWF_C_2_1 * a[-1]) + SWF_C_2_2 * a[0];
-  double um2 = ((0.0 + SWF_C_1_0 * a[-1]) + SWF_C_1_1 * a[0]) + SWF_C_1_2 * a[1];
-  double um3 = ((0.0 + SWF_C_0_0 * a[0]) + SWF_C_0_1 * a[1]) + SWF_C_0_2 * a[2];
-  if (M == MODE_FP32) {
-    float w[3];
-    weights_fp32(a, 0.3f, 0.6f, 0.1f, w);
-    return ((double)w[0] * um3 + (double)w[1] * um2) + (double)w[2] * um1;
+  if (M != MODE_EXACT) {
+    double e[4];
+    first_differences(a, e);
+    return a[0] + (M == MODE_FP64 ? weno_d64<true>(e) : weno_d32_or_64<true>(e));
   }
+  const double um1 = ((0.0 + SWF_C_2_0 * a[-2]) + SWF_C_2_1 * a[-1]) + SWF_C_2_2 * a[0];
+  const double um2 = ((0.0 + SWF_C_1_0 * a[-1]) + SWF_C_1_1 * a[0]) + SWF_C_1_2 * a[1];
+  const double um3 = ((0.0 + SWF_C_0_0 * a[0]) + SWF_C_0_1 * a[1]) + SWF_C_0_2 * a[2];
   double b[3];
-  if (M == MODE_EXACT) {
-    beta_q(a, b);
-  } else {
-    b[0] = (13. / 12.) * ((a[0] - 2. * a[1] + a[2]) * (a[0] - 2. * a[1] + a[2])) +
-           0.25 * ((3. * a[0] - 4. * a[1] + a[2]) * (3. * a[0] - 4. * a[1] + a[2]));
-    b[1] = (13. / 12.) * ((a[-1] - 2. * a[0] + a[1]) * (a[-1] - 2. * a[0] + a[1])) +
-           0.25 * ((a[-1] - a[1]) * (a[-1] - a[1]));
-    b[2] = (13. / 12.) * ((a[-2] - 2. * a[-1] + a[0]) * (a[-2] - 2. * a[-1] + a[0])) +
-           0.25 * ((a[-2] - 4. * a[-1] + 3. * a[0]) * (a[-2] - 4. * a[-1] + 3. * a[0]));
-  }
-  double al1, al2, al3;
-  if (M == MODE_EXACT) {
-    al1 = alpha_q(0.3, b[0]);
-    al2 = alpha_q(0.6, b[1]);
-    al3 = alpha_q(0.1, b[2]);
-  } else {
-    al1 = 0.3 / ((1e-6 + b[0]) * (1e-6 + b[0]));
-    al2 = 0.6 / ((1e-6 + b[1]) * (1e-6 + b[1]));
-    al3 = 0.1 / ((1e-6 + b[2]) * (1e-6 + b[2]));
-  }
-  double s = (al1 + al2) + al3;
-  double w1 = al1 / s, w2 = al2 / s, w3 = al3 / s;
+  beta_q(a, b);
+  const double al1 = alpha_q(0.3, b[0]), al2 = alpha_q(0.6, b[1]), al3 = alpha_q(0.1, b[2]);
+  const double s = (al1 + al2) + al3;
+  const double w1 = al1 / s, w2 = al2 / s, w3 = al3 / s;
   return (w1 * um3 + w2 * um2) + w3 * um1;
 }
 
-// weno_left: value at x_{i-1/2}; `a` points at average(0)
+// weno_left: value at x_{i-1/2}; `a` points at average(0) (weno5.f90:86-113)
 template <int M>
 SWF_D double weno_left(const double *a) {
-  double up1 = ((0.0 + SWF_C_1_0 * a[-2]) + SWF_C_1_1 * a[-1]) + SWF_C_1_2 * a[0];
-  double up2 = ((0.0 + SWF_C_0_0 * a[-1]) + SWF_C_0_1 * a[0]) + SWF_C_0_2 * a[1];
-  double up3 = ((0.0 + SWF_C_M1_0 * a[0]) + SWF_C_M1_1 * a[1]) + SWF_C_M1_2 * a[2];
-  if (M == MODE_FP32) {
-    float w[3];
-    weights_fp32(a, 0.1f, 0.6f, 0.3f, w);
-    return ((double)w[0] * up3 + (double)w[1] * up2) + (double)w[2] * up1;
+  if (M != MODE_EXACT) {
+    double e[4];
+    first_differences(a, e);
+    return a[0] + (M == MODE_FP64 ? weno_d64<false>(e) : weno_d32_or_64<false>(e));
   }
+  const double up1 = ((0.0 + SWF_C_1_0 * a[-2]) + SWF_C_1_1 * a[-1]) + SWF_C_1_2 * a[0];
+  const double up2 = ((0.0 + SWF_C_0_0 * a[-1]) + SWF_C_0_1 * a[0]) + SWF_C_0_2 * a[1];
+  const double up3 = ((0.0 + SWF_C_M1_0 * a[0]) + SWF_C_M1_1 * a[1]) + SWF_C_M1_2 * a[2];
   double b[3];
-  if (M == MODE_EXACT) {
-    beta_q(a, b);
-  } else {
-    b[0] = (13. / 12.) * ((a[0] - 2. * a[1] + a[2]) * (a[0] - 2. * a[1] + a[2])) +
-           0.25 * ((3. * a[0] - 4. * a[1] + a[2]) * (3. * a[0] - 4. * a[1] + a[2]));
-    b[1] = (13. / 12.) * ((a[-1] - 2. * a[0] + a[1]) * (a[-1] - 2. * a[0] + a[1])) +
-           0.25 * ((a[-1] - a[1]) * (a[-1] - a[1]));
-    b[2] = (13. / 12.) * ((a[-2] - 2. * a[-1] + a[0]) * (a[-2] - 2. * a[-1] + a[0])) +
-           0.25 * ((a[-2] - 4. * a[-1] + 3. * a[0]) * (a[-2] - 4. * a[-1] + 3. * a[0]));
-  }
-  double al1, al2, al3;
-  if (M == MODE_EXACT) {
-    al1 = alpha_q(0.1, b[0]);
-    al2 = alpha_q(0.6, b[1]);
-    al3 = alpha_q(0.3, b[2]);
-  } else {
-    al1 = 0.1 / ((1e-6 + b[0]) * (1e-6 + b[0]));
-    al2 = 0.6 / ((1e-6 + b[1]) * (1e-6 + b[1]));
-    al3 = 0.3 / ((1e-6 + b[2]) * (1e-6 + b[2]));
-  }
-  double s = (al1 + al2) + al3;
-  double w1 = al1 / s, w2 = al2 / s, w3 = al3 / s;
+  beta_q(a, b);
+  const double al1 = alpha_q(0.1, b[0]), al2 = alpha_q(0.6, b[1]), al3 = alpha_q(0.3, b[2]);
+  const double s = (al1 + al2) + al3;
+  const double w1 = al1 / s, w2 = al2 / s, w3 = al3 / s;
   return (w1 * up3 + w2 * up2) + w3 * up1;
 }
 
 // ---------------------------------------------------------------------------
 // One half of an interface flux: Roe matrices at (a|b), projection of the
-// 5-point stencil `st` (stride 4 doubles, pointer at the centre point),
+// 5-point stencil `st` (pointer at component 0 of the centre point; points SS
+// doubles apart, components CS doubles apart),
 // 4 reconstructions and back-projection (ex4_sedov_2d/main.f90:228-304, :452-526).
 // RIGHT selects weno_right (the f+ part) or weno_left (the f- part).
 // ---------------------------------------------------------------------------
-struct Prim {
-  double w, vx, vy, h;
-};
-
-template <int M, int DIR, bool RIGHT>
-SWF_D void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, int sstride,
-                     double out[4]) {
+// FP64-exact: the reference's operations in its order, structural zeros kept
+// (they decide signs of zero), binary128 sites emulated.
+template <int DIR, bool RIGHT, int SS, int CS>
+SWF_D void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   const double g1 = k.gamma1;
-  double t0;
-  if (M == MODE_EXACT && !k.roe_eps_double)
-    t0 = roe_weight_q(pa.w, pb.w);
-  else
-    t0 = pa.w / ((pa.w + pb.w) + 1e-15);
-  double t1 = 1. - t0;
-  double vxm = t0 * pa.vx + t1 * pb.vx;
-  double vym = t0 * pa.vy + t1 * pb.vy;
-  double hm = t0 * pa.h + t1 * pb.h;
-  double qm = 0.5 * (vxm * vxm + vym * vym);
-  double sq = sqrt(fabs(g1 * (hm - qm)));
-  double cm;
-  if (M == MODE_EXACT && !k.roe_eps_double)
-    cm = cm_q(sq);
-  else
-    cm = sq + 1e-15;
+  const double t0 = k.roe_eps_double ? pa.w / ((pa.w + pb.w) + 1e-15) : roe_weight_q(pa.w, pb.w);
+  const double t1 = 1. - t0;
+  const double vxm = t0 * pa.vx + t1 * pb.vx;
+  const double vym = t0 * pa.vy + t1 * pb.vy;
+  const double hm = t0 * pa.h + t1 * pb.h;
+  const double qm = 0.5 * (vxm * vxm + vym * vym);
+  const double sq = sqrt(fabs(g1 * (hm - qm)));
+  const double cm = k.roe_eps_double ? sq + 1e-15 : cm_q(sq);
   // normal (vn) and tangential (vt) velocity for this direction
   const double vn = (DIR == 0) ? vxm : vym;
   const double vt = (DIR == 0) ? vym : vxm;
   const double tc = vn * cm;
-  double rcm = 1. / cm;
-  double b1 = (g1 * rcm) * rcm;
-  double b2 = qm * b1;
-  double tr = vn * rcm;
-  double tn = b1 * vn;
-  double t2 = 0.5 * b1;
-  double tt = b1 * vt;
+  const double rcm = 1. / cm;
+  const double b1 = (g1 * rcm) * rcm;
+  const double b2 = qm * b1;
+  const double tr = vn * rcm;
+  const double tn = b1 * vn;
+  const double t2 = 0.5 * b1;
+  const double tt = b1 * vt;
   // Left eigenvectors, rows k = 1..4, written in (normal, tangential) form;
   // component order (rho, m_n, m_t, E) is mapped to (rho, mx, my, E) below.
   double L[4][4];
@@ -318,7 +313,6 @@ SWF_D void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double
   double r[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    // evl(m, :) in physical component order
     double e[4];
     e[0] = L[m][0];
     e[cn] = L[m][1];
@@ -327,27 +321,80 @@ SWF_D void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double
     double cu[5];
 #pragma unroll
     for (int ig = -2; ig <= 2; ++ig) {
-      const double *f = st + ig * sstride;
-      if (M == MODE_EXACT)
-        cu[ig + 2] = ((e[0] * f[0] + e[1] * f[1]) + e[2] * f[2]) + e[3] * f[3];
-      else if (m == 1)  // structural zeros/ones of row 2 dropped outside the exact mode
-        cu[ig + 2] = e[0] * f[0] + f[ct];
-      else
-        cu[ig + 2] = ((e[0] * f[0] + e[1] * f[1]) + e[2] * f[2]) + e[3] * f[3];
+      const double *f = st + ig * SS;
+      cu[ig + 2] = ((e[0] * f[0] + e[1] * f[CS]) + e[2] * f[2 * CS]) + e[3] * f[3 * CS];
     }
-    r[m] = RIGHT ? weno_right<M>(&cu[2]) : weno_left<M>(&cu[2]);
+    r[m] = RIGHT ? weno_right<MODE_EXACT>(&cu[2]) : weno_left<MODE_EXACT>(&cu[2]);
   }
-  // back-projection, rows in physical component order
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int rr = (c == 0) ? 0 : (c == 3) ? 3 : (c == cn) ? 1 : 2;
-    if (M == MODE_EXACT)
-      out[c] = ((R[rr][0] * r[0] + R[rr][1] * r[1]) + R[rr][2] * r[2]) + R[rr][3] * r[3];
-    else if (rr == 0)
-      out[c] = (r[0] + r[2]) + r[3];
-    else
-      out[c] = ((R[rr][0] * r[0] + R[rr][1] * r[1]) + R[rr][2] * r[2]) + R[rr][3] * r[3];
+    out[c] = ((R[rr][0] * r[0] + R[rr][1] * r[1]) + R[rr][2] * r[2]) + R[rr][3] * r[3];
   }
+}
+
+// FP64 / FP32-bulk: the same Roe eigen-system, with the reconstruction written
+// as centre value plus correction (swf_weno.h):
+//     out = f(0) + R * weno_d(L * (first differences of f))
+// which is R * weno(L * f) in exact arithmetic (R = L^-1, L linear).  Only the
+// correction passes through the eigenvectors and the nonlinear weights.
+template <int M, int DIR, bool RIGHT, int SS, int CS>
+SWF_D void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+  const double g1 = k.gamma1;
+  const double t0 = pa.w * rcp64((pa.w + pb.w) + 1e-15);
+  const double t1 = 1. - t0;
+  const double vxm = t0 * pa.vx + t1 * pb.vx;
+  const double vym = t0 * pa.vy + t1 * pb.vy;
+  const double hm = t0 * pa.h + t1 * pb.h;
+  const double qm = 0.5 * (vxm * vxm + vym * vym);
+  const double cm = sqrt(fabs(g1 * (hm - qm))) + 1e-15;
+  const double vn = (DIR == 0) ? vxm : vym;
+  const double vt = (DIR == 0) ? vym : vxm;
+  const double rcm = rcp64(cm);
+  const double b1 = (g1 * rcm) * rcm, b2 = qm * b1, tr = vn * rcm, tn = b1 * vn, t2 = 0.5 * b1, tt = b1 * vt;
+  // left eigenvector rows 1, 3, 4 in (rho, m_n, m_t, E) order; row 2 is (-vt, 0, 1, 0)
+  const double l00 = 0.5 * (b2 + tr), l01 = -(0.5 * (tn + rcm)), l02 = -(0.5 * tt);
+  const double l20 = 1. - b2;
+  const double l30 = 0.5 * (b2 - tr), l31 = -(0.5 * (tn - rcm));
+  const int cn = (DIR == 0) ? 1 : 2, ct = (DIR == 0) ? 2 : 1;
+  double e[4][4];  // e[field][difference]
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double *f0 = st + (q - 2) * SS, *f1 = st + (q - 1) * SS;
+    const double d0 = f1[0] - f0[0], dn = f1[cn * CS] - f0[cn * CS], dt = f1[ct * CS] - f0[ct * CS],
+                 d3 = f1[3 * CS] - f0[3 * CS];
+    e[0][q] = ((l00 * d0 + l01 * dn) + l02 * dt) + t2 * d3;
+    e[1][q] = dt - vt * d0;
+    e[2][q] = ((l20 * d0 + tn * dn) + tt * dt) - b1 * d3;
+    e[3][q] = ((l30 * d0 + l31 * dn) + l02 * dt) + t2 * d3;
+  }
+  double r[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) r[m] = (M == MODE_FP64) ? weno_d64<RIGHT>(e[m]) : weno_d32_or_64<RIGHT>(e[m]);
+  // R * r with the right eigenvector columns (1, vn-c, vt, H-vn c), (0, 0, 1, vt),
+  // (1, vn, vt, q), (1, vn+c, vt, H+vn c)
+  const double s = (r[0] + r[2]) + r[3], dr = r[3] - r[0];
+  double corr[4];
+  corr[0] = s;
+  corr[cn] = vn * s + cm * dr;
+  corr[ct] = vt * s + r[1];
+  corr[3] = (hm * (r[0] + r[3]) + (vn * cm) * dr) + (vt * r[1] + qm * r[2]);
+  // The reference's R * L is not the identity: cm carries +1e-15, so
+  // H != cm^2/gamma1 + q, and R * L = I + dH * e_E * (L row 1 + L row 4) with
+  // dH = (hm - qm) - cm^2/gamma1.  Its centre value R * (L * f(0)) is therefore
+  // f(0) plus this energy term (relative size ~1e-15/cm: 1e-12 where cm ~ 1e-3).
+  const double dH = (hm - qm) - (cm * cm) * k.ig1;
+  corr[3] += dH * (((b2 * st[0] - tn * st[cn * CS]) - tt * st[ct * CS]) + b1 * st[3 * CS]);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c] = st[c * CS] + corr[c];
+}
+
+template <int M, int DIR, bool RIGHT, int SS, int CS>
+SWF_D void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+  if (M == MODE_EXACT)
+    half_flux_exact<DIR, RIGHT, SS, CS>(k, pa, pb, st, out);
+  else
+    half_flux_fast<M, DIR, RIGHT, SS, CS>(k, pa, pb, st, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -371,6 +418,23 @@ SWF_HD double cell_alpha(const KCfg &k, double u1, double u2, double u3, double 
   double pres = k.gamma1 * ((u4 - (0.5 * den) * (vx * vx)) - (0.5 * den) * (vy * vy));
   double sound = sqrt((k.gamma * pres) / den);
   return fmax2(fabs(u2 / u1) + sound, fabs(u3 / u1) + sound);
+}
+
+// findalpha per cell in the fast modes: the same expression with one
+// reciprocal of rho
+SWF_D double cell_alpha_fast(const KCfg &k, double u1, double u2, double u3, double u4) {
+  const double r = 1.0 / u1;
+  const double ax = fabs(u2 * r), ay = fabs(u3 * r);
+  double pres;
+  if (k.findalpha_ex3) {
+    pres = k.gamma1 * ((u4 - 0.5 * (u2 * u2) * r) - 0.5 * (u3 * u3) * r);
+  } else {
+    const bool vac = !(u1 >= k.eps2);
+    const double vx = vac ? 0.0 : u2 * r, vy = vac ? 0.0 : u3 * r;
+    pres = k.gamma1 * ((u4 - (0.5 * u1) * (vx * vx)) - (0.5 * u1) * (vy * vy));
+  }
+  const double sound = sqrt((k.gamma * pres) * r);
+  return fmax2(ax + sound, ay + sound);
 }
 
 // sweep flag: P <= eps or rho <= eps (ex4_sedov_2d/main.f90:1465-1466)

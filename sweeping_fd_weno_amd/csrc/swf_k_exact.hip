@@ -1,4 +1,4 @@
-// swf_k_exact.hip - FP64-exact flux/update kernels and the mode-independent
+// swf_k_exact.hip - FP64-exact stage kernels and the mode-independent
 // kernels (sweep, boundary conditions, alpha, layout transposes).  Compiled with
 // -ffp-contract=off: every product and sum is rounded separately, as in the
 // reference build (x86-64 SSE2 without FMA).
@@ -8,33 +8,28 @@
 
 namespace swf {
 
-void launch_flux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
-                    int clear_maps) {
-  hipLaunchKernelGGL(k_flux<MODE_EXACT>, dim3(flux_blocks(k)), dim3(256), 0, s, k, b, slot_in, slot_reset, parity,
-                     clear_maps);
+void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
+  hipLaunchKernelGGL(k_xflux<MODE_EXACT>, grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
 }
-void launch_update_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out, int parity,
-                      int rhs_only) {
-  hipLaunchKernelGGL(k_update<MODE_EXACT>, grid_cells(k), dim3(64, 4), 0, s, k, b, stage, slot_in, slot_out, parity,
-                     rhs_only);
+void launch_ystage_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags) {
+  hipLaunchKernelGGL(k_ystage<MODE_EXACT>, grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity, flags);
 }
 void launch_weno_points_m0(const double *a, double *r, double *l, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_weno_points<MODE_EXACT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, r, l, n);
 }
 
-void launch_flux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
-                 int clear_maps) {
+void launch_xflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
   if (mode == MODE_EXACT)
-    launch_flux_m0(k, b, s, slot_in, slot_reset, parity, clear_maps);
+    launch_xflux_m0(k, b, s, stage, slot_in, parity);
   else
-    launch_flux_m12(mode, k, b, s, slot_in, slot_reset, parity, clear_maps);
+    launch_xflux_m12(mode, k, b, s, stage, slot_in, parity);
 }
-void launch_update(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out,
-                   int parity, int rhs_only) {
+void launch_ystage(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
+                   int flags) {
   if (mode == MODE_EXACT)
-    launch_update_m0(k, b, s, stage, slot_in, slot_out, parity, rhs_only);
+    launch_ystage_m0(k, b, s, stage, slot_in, parity, flags);
   else
-    launch_update_m12(mode, k, b, s, stage, slot_in, slot_out, parity, rhs_only);
+    launch_ystage_m12(mode, k, b, s, stage, slot_in, parity, flags);
 }
 void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, hipStream_t s) {
   if (mode == MODE_EXACT)

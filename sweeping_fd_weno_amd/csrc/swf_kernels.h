@@ -1,15 +1,18 @@
 // swf_kernels.h - grid kernels of one SSP-RK3 stage, templated on precision mode.
 //
-// Per stage (ex4_sedov_2d/main.f90:685-777 `rk`):
-//   xflux<M>   net x-interface flux  F(k+1/2) = fp_flux(k) - fn_flux(k+1)   (wenofd x-pass)
-//   yflux<M>   net y-interface flux  G(j+1/2) = gp_flux(j) - gn_flux(j+1)   (wenofd y-pass)
-//   update<M>  RHS (scheme), RK combination, checkpositivity, sweep flags, alpha max
-//   sweep      (swf_sweep.h) single-workgroup order-preserving sweep, when flagged
-//   alpha_fix  alpha max again when the sweep changed the state
-//   bc         ghost cells (bc) and their alpha
-// Each interface kernel stages the stencil points of its tile (split fluxes and
-// Roe primitives) in LDS once; every thread then computes one interface: two
-// Roe eigen-systems, eight WENO reconstructions, two back-projections.
+// One stage (ex4_sedov_2d/main.f90:685-777 `rk`) is three launches:
+//   k_xflux<M>   wenofd x-pass: net x-interface flux F(i+1/2) = fp_flux(i) - fn_flux(i+1)
+//                (scaled by alpha as :414-415) into Fx; also clears the stage's
+//                sweep bitmaps and flags
+//   k_ystage<M>  wenofd y-pass for a tile, then per cell `scheme` (RHS from Fx and
+//                the tile's y fluxes), the RK combination, checkpositivity, sweep
+//                flags, ghost images (`bc`) and the tile's alpha max (findalpha)
+//   k_stage_end  (mode-independent, one workgroup) sweep_lim when flagged, alpha
+//                reduction, and at stage 3 the time update
+// Each flux tile stages its stencil points (split fluxes and Roe primitives) in
+// LDS once; every thread then computes one interface: the f+ half with the Roe
+// system of (i, i+1) and the f- half with that of (i+1, i+2), eight WENO
+// reconstructions in all.
 #pragma once
 
 #include "swf_device.h"
@@ -17,141 +20,165 @@
 
 namespace swf {
 
-// x tile: 64 interfaces x 4 rows; points k0-2 .. k0+66
+// x tile: 64 interfaces x 4 rows; stencil points k0-2 .. k0+66
 constexpr int XT_W = 64, XT_H = 4, XT_PTS = XT_W + 5;
-// y tile: 16 columns x 16 interfaces; points jf0-2 .. jf0+18
-constexpr int YT_W = 16, YT_H = 16, YT_PTS = YT_H + 5;
+// y tile: 16 columns x 16 interfaces (jf0 .. jf0+15) -> 15 cells (jf0+1 .. jf0+15);
+// stencil points jf0-2 .. jf0+18
+constexpr int YT_W = UB_W, YT_H = UB_H + 1, YT_PTS = YT_H + 5;
 
 SWF_D double alpha_from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// LDS of one flux tile (x and y tiles share the allocation)
-union FluxSmem {
-  struct {
-    double fp[XT_H][XT_PTS][4];
-    double fn[XT_H][XT_PTS][4];
-    Prim pr[XT_H][XT_PTS];
-  } x;
-  struct {
-    double fp[YT_PTS][YT_W][4];
-    double fn[YT_PTS][YT_W][4];
-    Prim pr[YT_PTS][YT_W];
-  } y;
+// LDS of a flux tile, component-major so that consecutive lanes read
+// consecutive doubles: split fluxes f+ (fp) and f- (fn) of every stencil point,
+// then the Roe-average inputs w, vx, vy, h.
+struct XSmem {
+  static constexpr int CS = XT_H * XT_PTS;  // component stride
+  double fp[4][XT_H][XT_PTS];
+  double fn[4][XT_H][XT_PTS];
+  double pr[4][XT_H][XT_PTS];
+};
+struct YSmem {
+  static constexpr int CS = YT_PTS * YT_W;
+  double fp[4][YT_PTS][YT_W];
+  double fn[4][YT_PTS][YT_W];
+  double pr[4][YT_PTS][YT_W];
+  double G[4][YT_H][YT_W];  // net y-interface fluxes of the tile
+  double red[4];
 };
 
-SWF_D int flux_xblocks(const KCfg &k) { return ((k.nx + 1 + XT_W - 1) / XT_W) * ((k.ny + XT_H - 1) / XT_H); }
-SWF_D int flux_yblocks(const KCfg &k) { return ((k.nx + YT_W - 1) / YT_W) * ((k.ny + 1 + YT_H - 1) / YT_H); }
+// split fluxes and Roe primitives of stencil point u
+template <int M, int DIR>
+SWF_D void stage_point(const KCfg &k, const double u[4], double alpha, double ia, double fp[4], double fn[4],
+                       double pr[4]) {
+  Prim p;
+  if (M == MODE_EXACT) {
+    split_point<M, DIR>(k, u, alpha, ia, fp, fn);
+    roe_prims<M, DIR>(k, u, p.w, p.vx, p.vy, p.h);
+  } else {
+    stage_point_fast<DIR>(k, u, ia, fp, fn, p);
+  }
+  pr[0] = p.w;
+  pr[1] = p.vx;
+  pr[2] = p.vy;
+  pr[3] = p.h;
+}
 
-// x tile: interfaces k0..k0+63 of rows j0..j0+3
-template <int M>
-SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int bx, int by, double alpha, double ia, FluxSmem &sm,
-                      int tid) {
-  const int k0 = bx * XT_W, j0 = 1 + by * XT_H;
-  for (int p = tid; p < XT_H * XT_PTS; p += 256) {
-    const int r = p / XT_PTS, q = p - r * XT_PTS;
-    const int i = k0 - 2 + q, j = j0 + r;
-    if (j <= k.ny && i <= k.nx + 3) {
-      const long long o = cidx(k, i, j);
-      double u[4];
+// Stage the points p = tid and p = tid + 256 of a tile (at most two per thread):
+// both loads are issued before either point is processed, so the tile waits for
+// one memory round trip.  point(p, i, j) maps a point to its cell and returns
+// false outside the grid; store(p, fp, fn, pr) writes the staged values.
+template <int M, int DIR, int NPTS, class PointF, class StoreF>
+SWF_D void stage_tile(const KCfg &k, const Bufs &b, double alpha, double ia, int tid, PointF point, StoreF store) {
+  static_assert(NPTS <= 512, "at most two points per thread");
+  double u[2][4];
+  bool ok[2];
+  int pp[2];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) u[c] = b.U[c * k.plane + o];
-      split_point<M, 0>(k, u, alpha, ia, sm.x.fp[r][q], sm.x.fn[r][q]);
-      Prim pr;
-      roe_prims<M, 0>(k, u, pr.w, pr.vx, pr.vy, pr.h);
-      sm.x.pr[r][q] = pr;
+  for (int s = 0; s < 2; ++s) {
+    pp[s] = tid + 256 * s;
+    int i = 0, j = 0;
+    ok[s] = pp[s] < NPTS && point(pp[s], i, j);
+    if (ok[s]) {
+      const long long o = cidx(k, i, j);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) u[s][c] = b.Uin[c * k.plane + o];
     }
   }
-  __syncthreads();
-  const int tx = tid & (XT_W - 1), ty = tid / XT_W;
-  const int kk = k0 + tx, j = j0 + ty;
-  if (kk > k.nx || j > k.ny) return;
-  double fp[4], fn[4];
-  // f+ at interface kk (Roe(kk, kk+1), stencil kk-2..kk+2)
-  half_flux<M, 0, true>(k, sm.x.pr[ty][tx + 2], sm.x.pr[ty][tx + 3], &sm.x.fp[ty][tx + 2][0], 4, fp);
-  // f- at interface kk+1 (Roe(kk+1, kk+2), stencil kk-1..kk+3)
-  half_flux<M, 0, false>(k, sm.x.pr[ty][tx + 3], sm.x.pr[ty][tx + 4], &sm.x.fn[ty][tx + 3][0], 4, fn);
-  const long long o = cidx(k, kk, j);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const double f = k.alpha_in_scheme ? (fp[c] - fn[c]) : (alpha * fp[c] - alpha * fn[c]);
-    b.Fx[c * k.plane + o] = f;
+  for (int s = 0; s < 2; ++s) {
+    if (ok[s]) {
+      double fp[4], fn[4], pr[4];
+      stage_point<M, DIR>(k, u[s], alpha, ia, fp, fn, pr);
+      store(pp[s], fp, fn, pr);
+    }
   }
 }
 
-// y tile: interfaces jf0..jf0+15 of columns i0..i0+15
-template <int M>
-SWF_D void yflux_tile(const KCfg &k, const Bufs &b, int bx, int by, double alpha, double ia, FluxSmem &sm,
-                      int tid) {
-  const int i0 = 1 + bx * YT_W, jf0 = by * YT_H;
-  for (int p = tid; p < YT_PTS * YT_W; p += 256) {
-    const int r = p / YT_W, q = p - r * YT_W;
-    const int i = i0 + q, j = jf0 - 2 + r;
-    if (i <= k.nx && j <= k.ny + 3) {
-      const long long o = cidx(k, i, j);
-      double u[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) u[c] = b.U[c * k.plane + o];
-      split_point<M, 1>(k, u, alpha, ia, sm.y.fp[r][q], sm.y.fn[r][q]);
-      Prim pr;
-      roe_prims<M, 1>(k, u, pr.w, pr.vx, pr.vy, pr.h);
-      sm.y.pr[r][q] = pr;
-    }
-  }
-  __syncthreads();
-  const int tx = tid & (YT_W - 1), ty = tid / YT_W;
-  const int i = i0 + tx, jf = jf0 + ty;
-  if (i > k.nx || jf > k.ny) return;
-  double fp[4], fn[4];
-  half_flux<M, 1, true>(k, sm.y.pr[ty + 2][tx], sm.y.pr[ty + 3][tx], &sm.y.fp[ty + 2][tx][0], YT_W * 4, fp);
-  half_flux<M, 1, false>(k, sm.y.pr[ty + 3][tx], sm.y.pr[ty + 4][tx], &sm.y.fn[ty + 3][tx][0], YT_W * 4, fn);
-  const long long o = cidx(k, i, jf);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const double f = k.alpha_in_scheme ? (fp[c] - fn[c]) : (alpha * fp[c] - alpha * fn[c]);
-    b.Fy[c * k.plane + o] = f;
-  }
+// Roe-average inputs of a staged point (component stride CS)
+template <int CS>
+SWF_D Prim load_prim(const double *pr) {
+  Prim p;
+  p.w = pr[0];
+  p.vx = pr[CS];
+  p.vy = pr[2 * CS];
+  p.h = pr[3 * CS];
+  return p;
 }
 
-// wenofd: both directions in one launch; the first flux_xblocks() workgroups
-// take x tiles, the rest y tiles.  `clear_maps` also prepares this stage's
-// bookkeeping (sweep bitmaps, flags, the alpha slot the update kernel fills).
+// net flux at one interface from the staged points: f+ at the interface (centre
+// point p, Roe(p, p+1)) minus f- of the next point (centre p+1, Roe(p+1, p+2)),
+// scaled by alpha inside wenofd (ex4) or later in `scheme` (ex3/ex5).  Pointers
+// are at component 0 of point p; points SS doubles apart, components CS.
+template <int M, int DIR, int SS, int CS>
+SWF_D void iface_flux(const KCfg &k, const double *pr, const double *fp, const double *fn, double alpha,
+                      double F[4]) {
+  const Prim p0 = load_prim<CS>(pr), p1 = load_prim<CS>(pr + SS), p2 = load_prim<CS>(pr + 2 * SS);
+  double a[4], c[4];
+  half_flux<M, DIR, true, SS, CS>(k, p0, p1, fp, a);
+  half_flux<M, DIR, false, SS, CS>(k, p1, p2, fn + SS, c);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) F[q] = k.alpha_in_scheme ? (a[q] - c[q]) : (alpha * a[q] - alpha * c[q]);
+}
+
+// Occupancy of the fast-mode flux kernels: 4 waves per SIMD (at most 128 VGPRs);
+// the exact mode's double-double and binary128 sites need more registers.
+#define SWF_FLUX_BOUNDS(M) __launch_bounds__(256, (M) == MODE_EXACT ? 1 : 4)
+
+// wenofd x-pass for interfaces k0..k0+63 of rows j0..j0+3 -> Fx.  In an RK stage
+// (stage = 1, 2, 3; 0 for a stand-alone evaluation) the launch also resets the
+// stage's sweep bitmaps and flags (the stage kernel sets them), and at stage 1
+// of an inactive step keeps the next step inactive.
 template <int M>
-__global__ __launch_bounds__(256) void k_flux(KCfg k, Bufs b, int slot_in, int slot_reset, int parity,
-                                              int clear_maps) {
-  __shared__ FluxSmem sm;
+__global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_in, int parity) {
+  __shared__ XSmem sm;
+  constexpr int CS = XSmem::CS;
   DevScalars *g = b.g;
-  if (!g->active[parity]) return;
   const int tid = threadIdx.x;
-  if (clear_maps) {
-    const long long gid = (long long)blockIdx.x * 256 + tid;
-    if (gid < k.nwords) {
-      b.bmR[gid] = 0u;
-      b.bmP[gid] = 0u;
+  const bool first = blockIdx.x == 0 && blockIdx.y == 0 && tid == 0;
+  if (!g->active[parity] || g->status != ST_OK) {
+    if (stage == 1 && first) g->active[1 - parity] = 0;
+    return;
+  }
+  if (stage > 0) {
+    const long long nthreads = (long long)gridDim.x * gridDim.y * 256;
+    for (long long w = (long long)(blockIdx.x + blockIdx.y * gridDim.x) * 256 + tid; w < k.nwords; w += nthreads) {
+      b.bmR[w] = 0u;
+      b.bmP[w] = 0u;
     }
-    if (gid == 0) {
+    if (first) {
       g->postag = 0;
       g->rho_any = 0;
     }
   }
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
   const double ia = 1.0 / alpha;
-  const int nbx = flux_xblocks(k);
-  if ((int)blockIdx.x < nbx) {
-    const int gx = (k.nx + 1 + XT_W - 1) / XT_W;
-    xflux_tile<M>(k, b, blockIdx.x % gx, blockIdx.x / gx, alpha, ia, sm, tid);
-  } else {
-    const int gx = (k.nx + YT_W - 1) / YT_W;
-    const int t = blockIdx.x - nbx;
-    yflux_tile<M>(k, b, t % gx, t / gx, alpha, ia, sm, tid);
-  }
-}
-
-// scheme: RHS of one interior cell from the net interface fluxes
-SWF_D double cell_rhs(const KCfg &k, const Bufs &b, int c, long long o, double alpha) {
-  const double *fx = b.Fx + c * k.plane, *fy = b.Fy + c * k.plane;
-  double X = fx[o] - fx[o - 1];
-  double Y = fy[o] - fy[o - k.P];
-  if (k.alpha_in_scheme) return -((alpha * X) / k.dx) - (alpha * Y) / k.dy;
-  return -(X / k.dx) - Y / k.dy;
+  const int k0 = blockIdx.x * XT_W, j0 = 1 + blockIdx.y * XT_H;
+  stage_tile<M, 0, XT_H * XT_PTS>(
+      k, b, alpha, ia, tid,
+      [&](int p, int &i, int &j) {
+        const int r = p / XT_PTS;
+        i = k0 - 2 + (p - r * XT_PTS);
+        j = j0 + r;
+        return j <= k.ny && i <= k.nx + 3;
+      },
+      [&](int p, const double *fp, const double *fn, const double *pr) {
+        const int r = p / XT_PTS, q = p - r * XT_PTS;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          sm.fp[c][r][q] = fp[c];
+          sm.fn[c][r][q] = fn[c];
+          sm.pr[c][r][q] = pr[c];
+        }
+      });
+  __syncthreads();
+  const int tx = tid & (XT_W - 1), ty = tid / XT_W;
+  const int kk = k0 + tx, j = j0 + ty;
+  if (kk > k.nx || j > k.ny) return;
+  double F[4];
+  iface_flux<M, 0, 1, CS>(k, &sm.pr[0][ty][tx + 2], &sm.fp[0][ty][tx + 2], &sm.fn[0][ty][tx + 2], alpha, F);
+  const long long o = cidx(k, kk, j);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) b.Fx[c * k.plane + o] = F[c];
 }
 
 // dt of the step (ex4_sedov_2d/main.f90:696-700; ex3 :570-572)
@@ -237,17 +264,17 @@ SWF_D double block_max(double a, double *red, int tid) {
   return m;
 }
 
-// After a sweep: alpha of the update blocks whose cells changed, and the edge
-// ghosts of the boundary band.  One workgroup of NT threads (NT >= 256).
+// After a sweep: alpha partials of the update tiles whose cells changed, and
+// the edge ghosts of the boundary band.  One workgroup of NT threads (NT >= 256).
 template <int NT>
 SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double *red, int tid) {
-  const int gx = (k.nx + 63) >> 6, gy = (k.ny + 3) >> 2, nblk = gx * gy;
+  const int gx = ub_gx(k), nblk = gx * ub_gy(k);
   const int nw = sh.all_dirty ? (nblk + 31) / 32 : min(SW_DIRTY_WORDS, (nblk + 31) / 32);
-  constexpr int PER = NT / 256;  // update blocks handled at once
+  constexpr int PER = NT / 256;  // tiles handled at once
   for (int w = 0; w < nw; ++w) {
     unsigned word = sh.all_dirty ? 0xffffffffu : sh.dirty[w];
     while (word) {
-      // up to PER dirty blocks of this word, one 256-thread group each
+      // up to PER dirty tiles of this word, one 256-thread group each
       int blks[PER];
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
@@ -261,8 +288,8 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
       const int grp = tid >> 8, lt = tid & 255;
       const int blk = blks[grp];
       double a = 0.0;
-      if (blk >= 0 && blk < nblk) {
-        const int i = 1 + (blk % gx) * 64 + (lt & 63), j = 1 + (blk / gx) * 4 + (lt >> 6);
+      if (blk >= 0 && blk < nblk && lt < UB_W * UB_H) {
+        const int i = 1 + (blk % gx) * UB_W + lt % UB_W, j = 1 + (blk / gx) * UB_H + lt / UB_W;
         if (i <= k.nx && j <= k.ny) {
           double v[4];
           load_ij(k, b, i, j, v);
@@ -304,113 +331,152 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
   __syncthreads();
 }
 
-// RK stage combination + checkpositivity + sweep flags + ghost images + the
-// workgroup's alpha max of the new state (reduced by k_stage_end).  `rhs_only`
-// writes RHS into b.R instead (parity tests of the spatial operator).
+// One stage for a tile of UB_W x UB_H cells: the y-pass of wenofd over the
+// tile's 16 interfaces, then per cell scheme (ex4_sedov_2d/main.f90:639-650),
+// the RK combination (:715, :738, :760), checkpositivity (:779-801), the sweep
+// flags, ghost images and the tile's alpha max (findalpha, :652-683) as one
+// partial of the stage's alpha.  Stage 1 also fixes dt (:696-700) and with it the
+// time after the step and the gate of the next step (the loop test of :112).
+// flags: YS_RHS_ONLY stores RHS into b.R instead (parity tests of the spatial
+// operator); YS_NO_SCALARS leaves the device scalars alone (kernel timing).
 template <int M>
-__global__ __launch_bounds__(256) void k_update(KCfg k, Bufs b, int stage, int slot_in, int slot_out,
-                                                int parity, int rhs_only) {
-  __shared__ double red4[4];
+__global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_in, int parity, int flags) {
+  __shared__ YSmem sm;
   DevScalars *g = b.g;
-  const int tid = threadIdx.x + 64 * threadIdx.y;
-  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
-  if (!g->active[parity]) return;
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x, j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const bool in = (i <= k.nx && j <= k.ny);
+  if (!g->active[parity] || g->status != ST_OK) return;
+  const bool rhs_only = flags & YS_RHS_ONLY, scalars = !(flags & YS_NO_SCALARS);
+  const int tid = threadIdx.x;
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
-  if (rhs_only) {
-    if (in) {
-      const long long o = cidx(k, i, j);
+  const double ia = 1.0 / alpha;
+  const int i0 = 1 + blockIdx.x * UB_W, j0 = 1 + blockIdx.y * UB_H;  // first cell
+  const int jf0 = j0 - 1;                                            // first interface
+  stage_tile<M, 1, YT_PTS * YT_W>(
+      k, b, alpha, ia, tid,
+      [&](int p, int &i, int &j) {
+        const int r = p / YT_W;
+        i = i0 + (p - r * YT_W);
+        j = jf0 - 2 + r;
+        return i <= k.nx && j <= k.ny + 3;
+      },
+      [&](int p, const double *fp, const double *fn, const double *pr) {
+        const int r = p / YT_W, q = p - r * YT_W;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) b.R[c * k.plane + o] = cell_rhs(k, b, c, o, alpha);
-    }
-    return;
-  }
+        for (int c = 0; c < 4; ++c) {
+          sm.fp[c][r][q] = fp[c];
+          sm.fn[c][r][q] = fn[c];
+          sm.pr[c][r][q] = pr[c];
+        }
+      });
+  __syncthreads();
+  const int tx = tid & (YT_W - 1), ty = tid / YT_W;
+  const int i = i0 + tx, jf = jf0 + ty;
+  double G[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i <= k.nx && jf <= k.ny)
+    iface_flux<M, 1, YT_W, YSmem::CS>(k, &sm.pr[0][ty + 2][tx], &sm.fp[0][ty + 2][tx], &sm.fn[0][ty + 2][tx], alpha,
+                                      G);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sm.G[c][ty][tx] = G[c];
+  __syncthreads();
+  // cell j = jf: Y = G(j+1/2) - G(j-1/2), X = F(i+1/2) - F(i-1/2)
+  const int j = jf;
+  const bool in = ty >= 1 && i <= k.nx && j <= k.ny;
   double anew = 0.0;
   if (in) {
     const long long o = cidx(k, i, j);
-    double dt;
-    if (stage == 1) {
-      dt = step_dt(k, alpha, g->time[parity], g->tfinal);
-      if (i == 1 && j == 1) g->dt = dt;
-    } else {
-      dt = g->dt;
-    }
-    double v[4];
+    double rhs[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const double r = cell_rhs(k, b, c, o, alpha);
-      const double u = b.U[c * k.plane + o];
-      double un;
+      const double *fx = b.Fx + c * k.plane;
+      const double X = fx[o] - fx[o - 1];
+      const double Y = G[c] - sm.G[c][ty - 1][tx];
+      if (M == MODE_EXACT)
+        rhs[c] = k.alpha_in_scheme ? -((alpha * X) / k.dx) - (alpha * Y) / k.dy : -(X / k.dx) - Y / k.dy;
+      else
+        rhs[c] = k.alpha_in_scheme ? -((alpha * X) * k.idx) - (alpha * Y) * k.idy : -(X * k.idx) - Y * k.idy;
+    }
+    if (rhs_only) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b.R[c * k.plane + o] = rhs[c];
+    } else {
+      double dt;
       if (stage == 1) {
-        b.U0[c * k.plane + o] = u;
-        un = u + dt * r;
-      } else if (stage == 2) {
-        const double x = u + dt * r;
-        const double u0 = b.U0[c * k.plane + o];
-        un = (M == MODE_EXACT) ? rk2_q(u0, x) : 0.75 * u0 + 0.25 * x;
+        dt = step_dt(k, alpha, g->time[parity], g->tfinal);
+        if (i == 1 && j == 1 && scalars) {
+          // time = time + dt after the step; the DO WHILE test gates the next one
+          const double tn = g->time[parity] + dt;
+          g->dt = dt;
+          g->time[1 - parity] = tn;
+          g->steps_done++;
+          g->active[1 - parity] = time_continue_q(tn, g->tfinal);
+        }
       } else {
-        const double x = u + dt * r;
-        const double u0 = b.U0[c * k.plane + o];
-        un = (M == MODE_EXACT) ? rk3_q(u0, x) : (u0 + 2.0 * x) / 3.0;
+        dt = g->dt;
       }
-      v[c] = un;
-      b.U[c * k.plane + o] = un;
-    }
-    write_ghosts(k, b, i, j, v);
-    // checkpositivity (ex4_sedov_2d/main.f90:779-801)
-    const double ke = (v[3] - ((v[2] * v[2]) / v[0]) / 2.) - ((v[1] * v[1]) / v[0]) / 2.;
-    const bool bad = k.poscheck_le ? (v[0] <= k.eps || ke <= k.eps) : (v[0] < k.eps || ke < k.eps);
-    if (bad) atomicOr(&g->postag, 1);
-    if (k.use_limiter == 2) {
-      const int p = posI(k, i, j);
-      if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) atomicOr(&b.bmP[p >> 5], 1u << (p & 31));
-      if (v[0] < k.delta) {
-        atomicOr(&b.bmR[p >> 5], 1u << (p & 31));
-        atomicOr(&g->rho_any, 1);
+      double v[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double x = b.Uin[c * k.plane + o] + dt * rhs[c];
+        double un;
+        if (stage == 1) {
+          un = x;
+        } else if (stage == 2) {
+          const double u0 = b.U0[c * k.plane + o];
+          un = (M == MODE_EXACT) ? rk2_q(u0, x) : 0.75 * u0 + 0.25 * x;
+        } else {
+          const double u0 = b.U0[c * k.plane + o];
+          un = (M == MODE_EXACT) ? rk3_q(u0, x) : (u0 + 2.0 * x) * (1.0 / 3.0);
+        }
+        v[c] = un;
+        b.U[c * k.plane + o] = un;
       }
+      write_ghosts(k, b, i, j, v);
+      // checkpositivity (ex4_sedov_2d/main.f90:779-801)
+      const double ke = (v[3] - ((v[2] * v[2]) / v[0]) / 2.) - ((v[1] * v[1]) / v[0]) / 2.;
+      const bool bad = k.poscheck_le ? (v[0] <= k.eps || ke <= k.eps) : (v[0] < k.eps || ke < k.eps);
+      if (bad) atomicOr(&g->postag, 1);
+      if (k.use_limiter == 2) {
+        const int p = posI(k, i, j);
+        if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) atomicOr(&b.bmP[p >> 5], 1u << (p & 31));
+        if (v[0] < k.delta) {
+          atomicOr(&b.bmR[p >> 5], 1u << (p & 31));
+          atomicOr(&g->rho_any, 1);
+        }
+      }
+      anew = (M == MODE_EXACT) ? cell_alpha(k, v[0], v[1], v[2], v[3]) : cell_alpha_fast(k, v[0], v[1], v[2], v[3]);
     }
-    anew = cell_alpha(k, v[0], v[1], v[2], v[3]);
   }
-  const double am = block_max<256>(anew, red4, tid);
-  if (tid == 0) b.apart[bid] = am;
+  if (rhs_only) return;
+  const double am = block_max<256>(anew, sm.red, tid);
+  if (tid == 0 && scalars) b.apart[blockIdx.x + blockIdx.y * gridDim.x] = am;
 }
 
 #ifdef SWF_COMMON_KERNELS
 // Mode-independent grid kernels, compiled once (swf_k_exact.hip).
 
 // End of an RK stage, one workgroup: sweep_lim when checkpositivity flagged the
-// stage, the alpha max of the new state (findalpha) from the per-workgroup
-// maxima and the prescribed boundary states, and at stage 3 the time update of
-// the reference's loop (time = time + dt, DO WHILE test).
+// stage, then the alpha max of the new state (findalpha) from the tile partials
+// (recomputed for the tiles a sweep changed) and the prescribed boundary states.
+// The gate, the flag and the partials are loaded together (one memory round
+// trip when no sweep is needed).
 __global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, int slot_out, int parity,
                                                     int serial_ubar) {
   __shared__ SweepShared<SW_T> sh;
   __shared__ double red[SW_T / 64];
   DevScalars *g = b.g;
   const int tid = threadIdx.x;
-  if (!g->active[parity]) {
-    // an inactive step leaves the next one inactive too
-    if (stage == 3 && tid == 0) g->active[1 - parity] = 0;
-    return;
-  }
-  if (k.use_limiter == 2 && g->postag) {
-    sweep_body<SW_T>(k, b, serial_ubar != 0, sh, tid);
-    after_sweep<SW_T>(k, b, sh, red, tid);
-  }
-  const int nblk = ((k.nx + 63) >> 6) * ((k.ny + 3) >> 2);
+  const int nblk = ub_gx(k) * ub_gy(k);
+  const int act = g->active[parity], st = g->status, flagged = g->postag;
   double a = 0.0;
   for (int q = tid; q < nblk; q += SW_T) a = fmax2(a, b.apart[q]);
-  a = block_max<SW_T>(a, red, tid);
-  if (tid == 0) {
-    g->alpha_bits[slot_out] = (unsigned long long)__double_as_longlong(fmax2(a, k.alpha_const));
-    if (stage == 3) {
-      const double tn = g->time[parity] + g->dt;
-      g->time[1 - parity] = tn;
-      g->steps_done++;
-      g->active[1 - parity] = (g->status == ST_OK) && time_continue_q(tn, g->tfinal);
-    }
+  if (!act || st != ST_OK) return;
+  if (k.use_limiter == 2 && flagged) {
+    sweep_body<SW_T>(k, b, serial_ubar != 0, sh, tid);
+    after_sweep<SW_T>(k, b, sh, red, tid);
+    a = 0.0;
+    for (int q = tid; q < nblk; q += SW_T) a = fmax2(a, b.apart[q]);
   }
+  a = block_max<SW_T>(a, red, tid);
+  if (tid == 0) g->alpha_bits[slot_out] = (unsigned long long)__double_as_longlong(fmax2(a, k.alpha_const));
 }
 
 // alpha max over every cell (ghosts included), for a freshly uploaded state

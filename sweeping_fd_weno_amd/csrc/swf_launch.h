@@ -7,23 +7,19 @@
 
 namespace swf {
 
-// flux / update kernels of one precision mode (mode 0 in swf_k_exact.hip,
-// modes 1 and 2 in swf_k_fast.hip)
-void launch_flux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
-                 int clear_maps);
-void launch_update(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out,
-                   int parity, int rhs_only);
+// stage kernels of one precision mode (mode 0 in swf_k_exact.hip, modes 1 and 2
+// in swf_k_fast.hip)
+void launch_xflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity);
+void launch_ystage(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
+                   int flags);
 void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, hipStream_t s);
 
-void launch_flux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
-                    int clear_maps);
-void launch_update_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out, int parity,
-                      int rhs_only);
+void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity);
+void launch_ystage_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags);
 void launch_weno_points_m0(const double *a, double *r, double *l, long long n, hipStream_t s);
-void launch_flux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int slot_reset, int parity,
-                     int clear_maps);
-void launch_update_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int slot_out,
-                       int parity, int rhs_only);
+void launch_xflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity);
+void launch_ystage_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
+                       int flags);
 void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, hipStream_t s);
 
 // mode-independent kernels (swf_k_exact.hip)
@@ -39,9 +35,9 @@ void launch_upload_reset(const Bufs &b, hipStream_t s);
 void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 
-inline unsigned flux_blocks(const KCfg &k) {
-  return (unsigned)(((k.nx + 1 + 63) / 64) * ((k.ny + 3) / 4) + ((k.nx + 15) / 16) * ((k.ny + 1 + 15) / 16));
-}
+// x-flux tiles: 64 interfaces (0..nx) x 4 rows; stage tiles: UB_W x UB_H cells
+inline dim3 grid_xflux(const KCfg &k) { return dim3((k.nx + 1 + 63) / 64, (k.ny + 3) / 4); }
+inline dim3 grid_ystage(const KCfg &k) { return dim3(ub_gx(k), ub_gy(k)); }
 inline dim3 grid_cells(const KCfg &k) { return dim3((k.nx + 63) / 64, (k.ny + 3) / 4); }
 
 }  // namespace swf

@@ -28,7 +28,7 @@
 namespace swf {
 
 constexpr int SW_WCH = 256;          // bitmap words per scan chunk (8192 positions)
-constexpr int SW_DIRTY_WORDS = 256;  // dirty-block bitmap: up to 8192 update blocks
+constexpr int SW_DIRTY_WORDS = 256;  // dirty-tile bitmap: up to 8192 update tiles
 
 SWF_D void cell_of(const KCfg &k, int order, int p, int &i, int &j) {
   if (order == 0)
@@ -41,8 +41,6 @@ SWF_D int pos_of(const KCfg &k, int order, int i, int j) { return order == 0 ? p
 SWF_D void set_bit(unsigned *m, int p) { atomicOr(&m[p >> 5], 1u << (p & 31)); }
 SWF_D void clear_bit(unsigned *m, int p) { atomicAnd(&m[p >> 5], ~(1u << (p & 31))); }
 
-// update-kernel block holding interior cell (i, j): blocks of 64 x 4 cells
-SWF_D int ublock_of(const KCfg &k, int i, int j) { return ((j - 1) >> 2) * ((k.nx + 63) >> 6) + ((i - 1) >> 6); }
 
 template <int NT>
 struct SweepShared {
@@ -62,7 +60,7 @@ struct SweepShared {
 
 template <int NT>
 SWF_D void mark_dirty(const KCfg &k, SweepShared<NT> &sh, int i, int j) {
-  const int blk = ublock_of(k, i, j);
+  const int blk = ub_of(k, i, j);
   if (blk < SW_DIRTY_WORDS * 32)
     atomicOr(&sh.dirty[blk >> 5], 1u << (blk & 31));
   else
