@@ -97,7 +97,7 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   KCfg &k = c->k;
   k.nx = cfg->nx;
   k.ny = cfg->ny;
-  k.P = ((cfg->nx + 6) + 15) / 16 * 16;
+  k.P = row_pitch(cfg->nx);
   k.plane = (long long)(cfg->ny + 6) * k.P;
   k.nwords = (int)(((long long)cfg->nx * cfg->ny + 31) / 32);
   k.dx = cfg->dx;

@@ -68,8 +68,17 @@ struct Bufs {
   double *apart;                    // alpha max of each update workgroup
 };
 
-// index of cell (i, j) (Fortran indices, ghosts from -2) inside a plane
-SWF_D long long cidx(const KCfg &k, int i, int j) { return (long long)(j + 2) * k.P + (i + 2); }
+// Row layout of every plane: cell i (Fortran index, ghosts from -2) at column
+// i + XOFF, so interior cell 1 starts a 128-byte line and each 16-cell tile row of
+// the stage kernel is exactly one cache line; the pitch P is a multiple of 16.
+constexpr int XOFF = 15;
+// x-flux tile origins are 64 bx - XT_SHIFT: the tile's first stencil point
+// (origin - 2) then starts a line
+constexpr int XT_SHIFT = XOFF + 46;
+SWF_HD int row_pitch(int nx) { return (nx + 3 + XOFF + 1 + 15) / 16 * 16; }
+
+// index of cell (i, j) (ghost rows from j = -2) inside a plane
+SWF_D long long cidx(const KCfg &k, int i, int j) { return (long long)(j + 2) * k.P + (i + XOFF); }
 
 // Update tiles: UB_W x UB_H interior cells per workgroup of the stage kernel;
 // each writes one alpha partial, and the sweep marks the tiles it changed.

@@ -20,7 +20,8 @@
 
 namespace swf {
 
-// x tile: 64 interfaces x 4 rows; stencil points k0-2 .. k0+66
+// x tile: 64 interfaces x 4 rows; stencil points k0-2 .. k0+66 with the tile
+// origin k0 = 64 bx - XT_SHIFT (first stencil point at the start of a line)
 constexpr int XT_W = 64, XT_H = 4, XT_PTS = XT_W + 5;
 // y tile: 16 columns x 16 interfaces (jf0 .. jf0+15) -> 15 cells (jf0+1 .. jf0+15);
 // stencil points jf0-2 .. jf0+18
@@ -152,14 +153,14 @@ __global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_i
   }
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
   const double ia = 1.0 / alpha;
-  const int k0 = blockIdx.x * XT_W, j0 = 1 + blockIdx.y * XT_H;
+  const int k0 = blockIdx.x * XT_W - XT_SHIFT, j0 = 1 + blockIdx.y * XT_H;
   stage_tile<M, 0, XT_H * XT_PTS>(
       k, b, alpha, ia, tid,
       [&](int p, int &i, int &j) {
         const int r = p / XT_PTS;
         i = k0 - 2 + (p - r * XT_PTS);
         j = j0 + r;
-        return j <= k.ny && i <= k.nx + 3;
+        return j <= k.ny && i >= -2 && i <= k.nx + 3;
       },
       [&](int p, const double *fp, const double *fn, const double *pr) {
         const int r = p / XT_PTS, q = p - r * XT_PTS;
@@ -173,7 +174,7 @@ __global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_i
   __syncthreads();
   const int tx = tid & (XT_W - 1), ty = tid / XT_W;
   const int kk = k0 + tx, j = j0 + ty;
-  if (kk > k.nx || j > k.ny) return;
+  if (kk < 0 || kk > k.nx || j > k.ny) return;
   double F[4];
   iface_flux<M, 0, 1, CS>(k, &sm.pr[0][ty][tx + 2], &sm.fp[0][ty][tx + 2], &sm.fn[0][ty][tx + 2], alpha, F);
   const long long o = cidx(k, kk, j);
@@ -350,6 +351,10 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
   const double ia = 1.0 / alpha;
   const int i0 = 1 + blockIdx.x * UB_W, j0 = 1 + blockIdx.y * UB_H;  // first cell
   const int jf0 = j0 - 1;                                            // first interface
+  const int tx = tid & (YT_W - 1), ty = tid / YT_W;
+  const int i = i0 + tx, jf = jf0 + ty;
+  const int j = jf;
+  const bool in = ty >= 1 && i <= k.nx && j <= k.ny;
   stage_tile<M, 1, YT_PTS * YT_W>(
       k, b, alpha, ia, tid,
       [&](int p, int &i, int &j) {
@@ -368,8 +373,6 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
         }
       });
   __syncthreads();
-  const int tx = tid & (YT_W - 1), ty = tid / YT_W;
-  const int i = i0 + tx, jf = jf0 + ty;
   double G[4] = {0.0, 0.0, 0.0, 0.0};
   if (i <= k.nx && jf <= k.ny)
     iface_flux<M, 1, YT_W, YSmem::CS>(k, &sm.pr[0][ty + 2][tx], &sm.fp[0][ty + 2][tx], &sm.fn[0][ty + 2][tx], alpha,
@@ -378,8 +381,6 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
   for (int c = 0; c < 4; ++c) sm.G[c][ty][tx] = G[c];
   __syncthreads();
   // cell j = jf: Y = G(j+1/2) - G(j-1/2), X = F(i+1/2) - F(i-1/2)
-  const int j = jf;
-  const bool in = ty >= 1 && i <= k.nx && j <= k.ny;
   double anew = 0.0;
   if (in) {
     const long long o = cidx(k, i, j);

@@ -35,8 +35,9 @@ void launch_upload_reset(const Bufs &b, hipStream_t s);
 void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 
-// x-flux tiles: 64 interfaces (0..nx) x 4 rows; stage tiles: UB_W x UB_H cells
-inline dim3 grid_xflux(const KCfg &k) { return dim3((k.nx + 1 + 63) / 64, (k.ny + 3) / 4); }
+// x-flux tiles: 64 interfaces x 4 rows covering interfaces 0..nx from origin
+// -XT_SHIFT (swf_kernels.h); stage tiles: UB_W x UB_H cells
+inline dim3 grid_xflux(const KCfg &k) { return dim3((k.nx + XT_SHIFT + 64) / 64, (k.ny + 3) / 4); }
 inline dim3 grid_ystage(const KCfg &k) { return dim3(ub_gx(k), ub_gy(k)); }
 inline dim3 grid_cells(const KCfg &k) { return dim3((k.nx + 63) / 64, (k.ny + 3) / 4); }
 
