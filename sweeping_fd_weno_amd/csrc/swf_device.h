@@ -19,6 +19,8 @@
 #include "swf_weno.h"
 
 #define SWF_D __device__ __forceinline__
+// the per-point physics below is also compiled for the host (host checks of the
+// fast modes against the oracle); kernels and wave-level helpers stay device-only
 
 namespace swf {
 
@@ -78,7 +80,7 @@ constexpr int XT_SHIFT = XOFF + 46;
 SWF_HD int row_pitch(int nx) { return (nx + 3 + XOFF + 1 + 15) / 16 * 16; }
 
 // index of cell (i, j) (ghost rows from j = -2) inside a plane
-SWF_D long long cidx(const KCfg &k, int i, int j) { return (long long)(j + 2) * k.P + (i + XOFF); }
+SWF_HD long long cidx(const KCfg &k, int i, int j) { return (long long)(j + 2) * k.P + (i + XOFF); }
 
 // Update tiles: UB_W x UB_H interior cells per workgroup of the stage kernel;
 // each writes one alpha partial, and the sweep marks the tiles it changed.
@@ -90,14 +92,14 @@ SWF_HD int ub_of(const KCfg &k, int i, int j) { return ((j - 1) / UB_H) * ub_gx(
 // ---------------------------------------------------------------------------
 // Sweep orders (ex4_sedov_2d/main.f90:1836-1895), closed form, 0-based positions
 // ---------------------------------------------------------------------------
-SWF_D int posI(const KCfg &k, int i, int j) { return (j - 1) * k.nx + ((j & 1) ? (i - 1) : (k.nx - i)); }
-SWF_D int posII(const KCfg &k, int i, int j) { return (i - 1) * k.ny + ((i & 1) ? (j - 1) : (k.ny - j)); }
-SWF_D void cellI(const KCfg &k, int p, int &i, int &j) {
+SWF_HD int posI(const KCfg &k, int i, int j) { return (j - 1) * k.nx + ((j & 1) ? (i - 1) : (k.nx - i)); }
+SWF_HD int posII(const KCfg &k, int i, int j) { return (i - 1) * k.ny + ((i & 1) ? (j - 1) : (k.ny - j)); }
+SWF_HD void cellI(const KCfg &k, int p, int &i, int &j) {
   j = p / k.nx + 1;
   int r = p - (j - 1) * k.nx;
   i = (j & 1) ? r + 1 : k.nx - r;
 }
-SWF_D void cellII(const KCfg &k, int p, int &i, int &j) {
+SWF_HD void cellII(const KCfg &k, int p, int &i, int &j) {
   i = p / k.ny + 1;
   int r = p - (i - 1) * k.ny;
   j = (i & 1) ? r + 1 : k.ny - r;
@@ -112,7 +114,7 @@ SWF_HD double fmin2(double a, double b) { return (b < a) ? b : a; }
 // ---------------------------------------------------------------------------
 
 // rho < eps2 test of fflux/gflux (ex4: REAL(8) eps2; ex3/ex5: the binary128 1d-8)
-SWF_D bool below_eps2(const KCfg &k, double rho) {
+SWF_HD bool below_eps2(const KCfg &k, double rho) {
   return k.eps2_quad_literal ? (rho < qconst::e8_hi) : (rho < k.eps2);
 }
 
@@ -124,7 +126,7 @@ struct Prim {
 // Lax-Friedrichs split of the Euler flux: fp = 0.5*(u + f/a), fn = 0.5*(u - f/a)
 // (ex4_sedov_2d/main.f90:125-177 fflux/gflux, :223-227 split).  DIR 0 = x, 1 = y.
 template <int M, int DIR>
-SWF_D void split_point(const KCfg &k, const double u[4], double alpha, double ia, double fp[4], double fn[4]) {
+SWF_HD void split_point(const KCfg &k, const double u[4], double alpha, double ia, double fp[4], double fn[4]) {
   double rho = u[0], mx = u[1], my = u[2], e = u[3];
   double vx = mx / rho, vy = my / rho;
   if (below_eps2(k, rho)) {
@@ -154,7 +156,7 @@ SWF_D void split_point(const KCfg &k, const double u[4], double alpha, double ia
 
 // Primitive data for the Roe average (ex4_sedov_2d/main.f90:201-214 x, :421-434 y)
 template <int M, int DIR>
-SWF_D void roe_prims(const KCfg &k, const double u[4], double &w, double &vx, double &vy, double &h) {
+SWF_HD void roe_prims(const KCfg &k, const double u[4], double &w, double &vx, double &vy, double &h) {
   double den = u[0], xmt = u[1], ymt = u[2], eng = u[3];
   double t0;
   if (DIR == 1 && !k.ydens_plain) {
@@ -177,7 +179,7 @@ SWF_D void roe_prims(const KCfg &k, const double u[4], double &w, double &vx, do
 // formulas of split_point and roe_prims with one reciprocal of rho shared by the
 // velocities and a multiplication by 1/alpha.
 template <int DIR>
-SWF_D void stage_point_fast(const KCfg &k, const double u[4], double ia, double fp[4], double fn[4], Prim &pr) {
+SWF_HD void stage_point_fast(const KCfg &k, const double u[4], double ia, double fp[4], double fn[4], Prim &pr) {
   const double rho = u[0], mx = u[1], my = u[2], e = u[3];
   const double r = 1.0 / rho;
   const double vx0 = mx * r, vy0 = my * r;
@@ -237,7 +239,7 @@ SWF_D void stage_point_fast(const KCfg &k, const double u[4], double ia, double 
 // follows weno5.f90:120-146 operation by operation (beta and alpha at binary128);
 // the fast modes use the first-difference forms of swf_weno.h.
 template <int M>
-SWF_D double weno_right(const double *a) {
+SWF_HD double weno_right(const double *a) {
   if (M != MODE_EXACT) {
     double e[4];
     first_differences(a, e);
@@ -256,7 +258,7 @@ SWF_D double weno_right(const double *a) {
 
 // weno_left: value at x_{i-1/2}; `a` points at average(0) (weno5.f90:86-113)
 template <int M>
-SWF_D double weno_left(const double *a) {
+SWF_HD double weno_left(const double *a) {
   if (M != MODE_EXACT) {
     double e[4];
     first_differences(a, e);
@@ -283,7 +285,7 @@ SWF_D double weno_left(const double *a) {
 // FP64-exact: the reference's operations in its order, structural zeros kept
 // (they decide signs of zero), binary128 sites emulated.
 template <int DIR, bool RIGHT, int SS, int CS>
-SWF_D void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   const double g1 = k.gamma1;
   const double t0 = k.roe_eps_double ? pa.w / ((pa.w + pb.w) + 1e-15) : roe_weight_q(pa.w, pb.w);
   const double t1 = 1. - t0;
@@ -348,7 +350,7 @@ SWF_D void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const 
 // which is R * weno(L * f) in exact arithmetic (R = L^-1, L linear).  Only the
 // correction passes through the eigenvectors and the nonlinear weights.
 template <int M, int DIR, bool RIGHT, int SS, int CS>
-SWF_D void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   const double g1 = k.gamma1;
   const double t0 = pa.w * rcp64((pa.w + pb.w) + 1e-15);
   const double t1 = 1. - t0;
@@ -399,7 +401,7 @@ SWF_D void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const d
 }
 
 template <int M, int DIR, bool RIGHT, int SS, int CS>
-SWF_D void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+SWF_HD void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   if (M == MODE_EXACT)
     half_flux_exact<DIR, RIGHT, SS, CS>(k, pa, pb, st, out);
   else
@@ -431,7 +433,7 @@ SWF_HD double cell_alpha(const KCfg &k, double u1, double u2, double u3, double 
 
 // findalpha per cell in the fast modes: the same expression with one
 // reciprocal of rho
-SWF_D double cell_alpha_fast(const KCfg &k, double u1, double u2, double u3, double u4) {
+SWF_HD double cell_alpha_fast(const KCfg &k, double u1, double u2, double u3, double u4) {
   const double r = 1.0 / u1;
   const double ax = fabs(u2 * r), ay = fabs(u3 * r);
   double pres;
@@ -447,11 +449,11 @@ SWF_D double cell_alpha_fast(const KCfg &k, double u1, double u2, double u3, dou
 }
 
 // sweep flag: P <= eps or rho <= eps (ex4_sedov_2d/main.f90:1465-1466)
-SWF_D double sweep_pres(double g1, double v0, double v1, double v2, double v3) {
+SWF_HD double sweep_pres(double g1, double v0, double v1, double v2, double v3) {
 #pragma clang fp contract(off)
   return g1 * (v3 - (0.5 * (v1 * v1 + v2 * v2)) / v0);
 }
-SWF_D bool sweep_flag(double g1, double eps, double v0, double v1, double v2, double v3) {
+SWF_HD bool sweep_flag(double g1, double eps, double v0, double v1, double v2, double v3) {
 #pragma clang fp contract(off)
   double p = sweep_pres(g1, v0, v1, v2, v3);
   return (p <= eps) || (v0 <= eps);

@@ -24,9 +24,9 @@
 // weights unchanged.
 //   FP64: a common denominator, alpha_k ~ d_k * s_j * s_l with s = (4e-6+4beta)^2,
 //         so one reciprocal per reconstruction.
-//   FP32: ratios t_k = q_k / min(q) >= 1 of q = 4e-6+4beta, capped at 1e9 (a
-//         weight below 1e-18 of the largest is negligible in FP32), then
-//         alpha_k ~ d_k (t_j t_l)^2 -- no overflow for any finite beta.
+//   FP32: alpha_k = d_k (min(q)/q_k)^2 with q = 4e-6+4beta (no overflow for any
+//         finite beta; negligible weights underflow to 0); the candidates and
+//         the normalised weighted sum stay FP64.
 #pragma once
 
 #include <float.h>
@@ -83,39 +83,48 @@ SWF_HD double weno_d64(const double e[4]) {
   return ((b1 * c3 + b2 * c2) + b3 * c1) / (6.0 * ((b1 + b2) + b3));
 }
 
-// weno_right(a) - a(0) (RIGHT) or weno_left(a) - a(0), FP32 (inputs rounded
-// from FP64 differences).  Non-finite only when a difference exceeds FP32 range
-// or beta overflows; callers then use weno_d64.
+// Unnormalised nonlinear weights in FP32 from the smoothness indicators' FP64
+// arguments x_k, y_k rounded to FP32: alpha_k = d_k (min(q)/q_k)^2 with
+// q = 4e-6 + 4 beta.  Every alpha_k carries its own relative rounding error only;
+// the weight of a stencil across a jump underflows to 0 as its exact value would
+// be negligible.  The smallest-q stencil has alpha = d_k >= 0.1, so the sum never
+// vanishes.  Requires max |x|, |y| <= 1e18 (then 4 beta < 1e38).
 template <bool RIGHT>
-SWF_HD float weno_d32(const float e[4]) {
-  const float em2 = e[0], em1 = e[1], e0 = e[2], e1 = e[3];
-  const float x1 = e1 - e0, y1 = e1 - 3.0f * e0;
-  const float x2 = e0 - em1, y2 = em1 + e0;
-  const float x3 = em1 - em2, y3 = 3.0f * em1 - em2;
+SWF_HD void alphas_d32(const float x[3], const float y[3], float a[3]) {
   const float c = 13.0f / 3.0f;
-  const float q1 = 4e-6f + ((c * x1) * x1 + y1 * y1);
-  const float q2 = 4e-6f + ((c * x2) * x2 + y2 * y2);
-  const float q3 = 4e-6f + ((c * x3) * x3 + y3 * y3);
-  const float d1 = RIGHT ? 0.3f : 0.1f, d2 = 0.6f, d3 = RIGHT ? 0.1f : 0.3f;
-  const float c3 = RIGHT ? 4.0f * e0 - e1 : 2.0f * e1 - 5.0f * e0;
-  const float c2 = RIGHT ? em1 + 2.0f * e0 : -(2.0f * em1 + e0);
-  const float c1 = RIGHT ? 5.0f * em1 - 2.0f * em2 : em2 - 4.0f * em1;
-  const float sm = rcp32(fminf(q1, fminf(q2, q3)));
-  const float t1 = fminf(q1 * sm, 1e9f), t2 = fminf(q2 * sm, 1e9f), t3 = fminf(q3 * sm, 1e9f);
-  const float p1 = t2 * t3, p2 = t1 * t3, p3 = t1 * t2;
-  const float a1 = d1 * (p1 * p1), a2 = d2 * (p2 * p2), a3 = d3 * (p3 * p3);
-  const float inv = rcp32((a1 + a2) + a3) * (1.0f / 6.0f);
-  return ((a1 * inv) * c3 + (a2 * inv) * c2) + (a3 * inv) * c1;
+  const float q1 = 4e-6f + ((c * x[0]) * x[0] + y[0] * y[0]);
+  const float q2 = 4e-6f + ((c * x[1]) * x[1] + y[1] * y[1]);
+  const float q3 = 4e-6f + ((c * x[2]) * x[2] + y[2] * y[2]);
+  const float m = fminf(q1, fminf(q2, q3));
+  const float r1 = m * rcp32(q1), r2 = m * rcp32(q2), r3 = m * rcp32(q3);
+  a[0] = (RIGHT ? 0.3f : 0.1f) * (r1 * r1);
+  a[1] = 0.6f * (r2 * r2);
+  a[2] = (RIGHT ? 0.1f : 0.3f) * (r3 * r3);
 }
 
-// FP32 form when every |difference| is at most 1e18 (then 4 beta < 1e38 and the
-// weight ratios stay in range), the FP64 form otherwise
+// weno_right(a) - a(0) (RIGHT) or weno_left(a) - a(0), FP32-bulk: smoothness
+// indicators and nonlinear weights in FP32; the candidate values and the
+// normalised weighted sum sum(alpha_k c_k) / sum(alpha_k) in FP64 (the FP32
+// weights convert exactly), so the result is a weighted mean of the FP64
+// candidates with each weight perturbed by ~1e-7 of itself.  Falls back to
+// weno_d64 beyond FP32 range.
 template <bool RIGHT>
 SWF_HD double weno_d32_or_64(const double e[4]) {
-  const float ef[4] = {(float)e[0], (float)e[1], (float)e[2], (float)e[3]};
-  const float big = fmaxf(fmaxf(fabsf(ef[0]), fabsf(ef[1])), fmaxf(fabsf(ef[2]), fabsf(ef[3])));
-  if (big <= 1e18f) return (double)weno_d32<RIGHT>(ef);
-  return weno_d64<RIGHT>(e);
+  const double em2 = e[0], em1 = e[1], e0 = e[2], e1 = e[3];
+  const double x1 = e1 - e0, y1 = e1 - 3.0 * e0;
+  const double x2 = e0 - em1, y2 = em1 + e0;
+  const double x3 = em1 - em2, y3 = 3.0 * em1 - em2;
+  const float x[3] = {(float)x1, (float)x2, (float)x3}, y[3] = {(float)y1, (float)y2, (float)y3};
+  const float big = fmaxf(fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(y[0]))),
+                          fmaxf(fabsf(y[1]), fabsf(y[2])));
+  if (!(big <= 1e18f)) return weno_d64<RIGHT>(e);
+  float al[3];
+  alphas_d32<RIGHT>(x, y, al);
+  const double a1 = al[0], a2 = al[1], a3 = al[2];
+  const double c3 = RIGHT ? 4.0 * e0 - e1 : 2.0 * e1 - 5.0 * e0;
+  const double c2 = RIGHT ? em1 + 2.0 * e0 : -(2.0 * em1 + e0);
+  const double c1 = RIGHT ? 5.0 * em1 - 2.0 * em2 : em2 - 4.0 * em1;
+  return ((a1 * c3 + a2 * c2) + a3 * c1) * rcp64(6.0 * ((a1 + a2) + a3));
 }
 
 // the four first differences of a(-2..2)

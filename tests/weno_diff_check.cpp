@@ -1,4 +1,4 @@
-// Host check of the first-difference WENO forms (FP64, and FP32 with its FP64 guard) (sweeping_fd_weno_amd/csrc/swf_weno.h)
+// Host check of the first-difference WENO forms (FP64, and FP32-bulk with its FP64 guard) (sweeping_fd_weno_amd/csrc/swf_weno.h)
 // against weno_right / weno_left of weno5.f90:86-146 evaluated in binary128
 // throughout.  Prints one line per (form, side, input kind):
 //     <form> <side> <kind> <n> <max err / scale>
