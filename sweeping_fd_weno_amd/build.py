@@ -64,18 +64,25 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
-def build_driver() -> str | None:
-    """The Fortran driver (reference parameter surface) linked against libswf.so."""
+def build_driver() -> list:
+    """The Fortran drivers (reference parameter surfaces) linked against libswf.so:
+    swf_driver (ex4 Sedov) and swf_vortex (ex3 vortex accuracy)."""
     drv_dir = os.path.join(PKG, "fortran")
-    srcs = [os.path.join(drv_dir, "swf_mod.f90"), os.path.join(drv_dir, "swf_driver.f90")]
-    if not all(os.path.exists(s) for s in srcs) or not os.path.exists(FLANG):
-        return None
-    out = os.path.join(drv_dir, "swf_driver")
+    mod_src = os.path.join(drv_dir, "swf_mod.f90")
+    if not os.path.exists(FLANG) or not os.path.exists(mod_src):
+        return []
     mod = os.path.join(OBJ, "fmod")
     os.makedirs(mod, exist_ok=True)
-    _run([FLANG, "-O2", "-fdefault-real-8", "-module-dir", mod, *srcs, "-o", out,
-          "-L" + PKG, "-lswf", "-Wl,-rpath,$ORIGIN/.."])
-    return out
+    built = []
+    for prog in ("swf_driver", "swf_vortex"):
+        src = os.path.join(drv_dir, prog + ".f90")
+        if not os.path.exists(src):
+            continue
+        out = os.path.join(drv_dir, prog)
+        _run([FLANG, "-O2", "-fdefault-real-8", "-module-dir", mod, mod_src, src, "-o", out,
+              "-L" + PKG, "-lswf", "-Wl,-rpath,$ORIGIN/.."])
+        built.append(out)
+    return built
 
 
 def build(verbose: bool = False) -> None:

@@ -111,6 +111,12 @@ def make_ex3() -> None:
         "exact_rho_sample": den[::37, 3].tolist(),
         "L1": l1, "Linf": linf,
         "stats": _parse_stats(text),
+        # the program's own output files (fort.7 / err_table.dat chained from fort.7 = 1.0 1.0)
+        "err_table_dat": open(os.path.join(rundir, "err_table.dat")).read(),
+        "fort7": open(os.path.join(rundir, "fort.7")).read(),
+        "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
+                        for fn in ("den.dat", "pre.dat", "m1.dat", "m2.dat", "den2.dat", "pre2.dat")},
+        "stdout_head": "\n".join(lines[:4]),
     }
     with open(os.path.join(GOLD, "ref_ex3_vortex180.json"), "w") as f:
         json.dump(out, f, indent=1)
