@@ -66,7 +66,8 @@ typedef struct swf_config {
   double eps2;              /* velocity cut-off, 1d-8 */
   double delta;             /* density sweep threshold, 1d-12 */
   int32_t bc;               /* SWF_BC_* */
-  int32_t use_limiter;      /* 2 = sweeping post-process, 0 = none */
+  int32_t use_limiter;      /* 2 = sweeping post-process, 1 = Zhang-Shu flux limiter
+                               (den_count / pre_count in swf_sweep_stats), 0 = none */
   int32_t swp_num;          /* sweep iteration cap */
   /* variant switches: 0 reproduces ex4_sedov_2d; the ex3/ex5 programs set: */
   int32_t alpha_in_scheme;   /* ex3/ex5: scheme multiplies by alphamax */

@@ -215,10 +215,10 @@ def sedov_config(n: int = 640, cfl: float = 0.5, swp_num: int = 20000, xm: float
                        swp_num=swp_num, pbar_check=1, fail_stop=1)
 
 
-def vortex_config(n: int = 180, cfl: float = 0.5, swp_num: int = 20000) -> SwoConfig:
+def vortex_config(n: int = 180, cfl: float = 0.5, swp_num: int = 20000, use_limiter: int = 2) -> SwoConfig:
     """ex3_vortex_accuracy_2d/main.f90:5-7,570-572,671-697."""
     return make_config(nx=n, ny=n, xlo=0.0, ylo=0.0, dx=(10.0 - 0.0) / n, dy=(10.0 - 0.0) / n, gamma=1.4,
-                       cfl=cfl, eps=1e-13, eps2=1e-8, delta=1e-12, bc=BC_PERIODIC, use_limiter=2,
+                       cfl=cfl, eps=1e-13, eps2=1e-8, delta=1e-12, bc=BC_PERIODIC, use_limiter=use_limiter,
                        swp_num=swp_num, alpha_in_scheme=1, roe_eps_double=1, ydens_plain=1,
                        eps2_quad_literal=1, findalpha_ex3=1, dt_rule_dx53=1, poscheck_le=1)
 

@@ -480,15 +480,67 @@ static void interface_flux(const swo_ctx *C, int dir, double wa, double wb, doub
     }
 }
 
-/* wenofd (ex4_sedov_2d/main.f90:179-637, use_limiter /= 1), for the rows and
- * columns that `scheme` reads.  Fills fpf/fnf (x) and gpf/gnf (y). */
+/* Zhang-Shu positivity limiter of one reconstructed split flux F against the
+ * split flux `ave` of its centre cell (use_limiter == 1;
+ * ex4_sedov_2d/main.f90:306-412 x-pass, :527-629 y-pass; ex3 :241-330, :444-530).
+ * Density first: theta from the two points F and (ave - F/12)/(11/12); then
+ * pressure, with the density component of both points updated and the other
+ * components as before the density step.  The ex4 y-pass clamps the pressure
+ * theta of gp at 0 (:573); theta >= 0 for finite data, so the clamp is omitted.
+ * Returns 1/2 in bit 0/1 when the density/pressure step changed F. */
+static int zs_limit(double g1, const double ave[4], double F[4]) {
+    int changed = 0;
+    const double d_ave = ave[0], m_ave = ave[1], n_ave = ave[2], e_ave = ave[3];
+    const double p_ave = g1 * ((e_ave - (0.5 * (m_ave * m_ave)) / d_ave) - (0.5 * (n_ave * n_ave)) / d_ave);
+    double point[2][4];
+    double theta = 1.;
+    for (int c = 0; c < 4; ++c) {
+        point[0][c] = F[c];
+        point[1][c] = (ave[c] - F[c] / 12.) / (11. / 12.);
+    }
+    for (int ig = 0; ig < 2; ++ig) {
+        const double d_l = point[ig][0];
+        if (d_l < 0.) theta = fmin2(theta, fabs(d_ave - 0.) / fabs(d_ave - d_l));
+    }
+    if (theta < 1.) {
+        F[0] = theta * F[0] + (1. - theta) * d_ave;
+        changed |= 1;
+    }
+    theta = 1.;
+    point[0][0] = F[0];
+    point[1][0] = (ave[0] - F[0] / 12.) / (11. / 12.);
+    for (int ig = 0; ig < 2; ++ig) {
+        const double d_l = point[ig][0], m_l = point[ig][1], n_l = point[ig][2], e_l = point[ig][3];
+        const double p_l = g1 * ((e_l - ((m_l * m_l) / d_l) / 2.) - ((n_l * n_l) / d_l) / 2.);
+        if (p_l < 0.) theta = fmin2(theta, fabs(p_ave - 0.) / fabs(p_ave - p_l));
+    }
+    if (theta < 1.) {
+        F[0] = theta * F[0] + (1. - theta) * d_ave;
+        F[1] = theta * F[1] + (1. - theta) * m_ave;
+        F[2] = theta * F[2] + (1. - theta) * n_ave;
+        F[3] = theta * F[3] + (1. - theta) * e_ave;
+        changed |= 2;
+    }
+    return changed;
+}
+
+static void zs_count(swo_ctx *C, int changed) {
+    if (changed & 1) C->den_count++;
+    if (changed & 2) C->pre_count++;
+}
+
+/* wenofd (ex4_sedov_2d/main.f90:179-637).  Without the flux limiter, the rows
+ * and columns that `scheme` reads; with it (use_limiter == 1) the reference's
+ * full ranges (x: rows 0..ny+1, y: columns 0..nx+1, cells 0..n+1), whose every
+ * limited half counts in den_count / pre_count.  Fills fpf/fnf and gpf/gnf. */
 static void wenofd(swo_ctx *C, double alphamax) {
     const double *u = C->u;
     const int nx = C->nx, ny = C->ny;
     const int scale = !C->cfg.alpha_in_scheme;
     double f[4];
-    /* x sweep over rows j = 1..ny */
-    for (int j = 1; j <= ny; ++j) {
+    const int zs = C->cfg.use_limiter == 1;
+    /* x sweep over rows j = 1..ny (0..ny+1 with the limiter) */
+    for (int j = zs ? 0 : 1; j <= (zs ? ny + 1 : ny); ++j) {
         for (int i = 0; i <= nx + 2; ++i)
             roe_prims(C, &U4(u, 0, i, j), 0, &C->pw[i], &C->pvx[i], &C->pvy[i], &C->ph[i]);
         for (int i = -2; i <= nx + 3; ++i) {
@@ -503,14 +555,18 @@ static void wenofd(swo_ctx *C, double alphamax) {
             double fpo[4], fno[4];
             interface_flux(C, 0, C->pw[i], C->pw[i + 1], C->pvx[i], C->pvx[i + 1], C->pvy[i], C->pvy[i + 1],
                            C->ph[i], C->ph[i + 1], &C->fp[4 * (i + 2)], &C->fn[4 * (i + 2)], fpo, fno);
+            if (zs) {
+                zs_count(C, zs_limit(C->gamma1, &C->fp[4 * (i + 2)], fpo));
+                zs_count(C, zs_limit(C->gamma1, &C->fn[4 * (i + 2)], fno));
+            }
             for (int c = 0; c < 4; ++c) {
                 U4(C->fpf, c, i, j) = scale ? alphamax * fpo[c] : fpo[c];
                 U4(C->fnf, c, i, j) = scale ? alphamax * fno[c] : fno[c];
             }
         }
     }
-    /* y sweep over columns i = 1..nx */
-    for (int i = 1; i <= nx; ++i) {
+    /* y sweep over columns i = 1..nx (0..nx+1 with the limiter) */
+    for (int i = zs ? 0 : 1; i <= (zs ? nx + 1 : nx); ++i) {
         for (int j = 0; j <= ny + 2; ++j)
             roe_prims(C, &U4(u, 0, i, j), 1, &C->pw[j], &C->pvx[j], &C->pvy[j], &C->ph[j]);
         for (int j = -2; j <= ny + 3; ++j) {
@@ -525,6 +581,10 @@ static void wenofd(swo_ctx *C, double alphamax) {
             double fpo[4], fno[4];
             interface_flux(C, 1, C->pw[j], C->pw[j + 1], C->pvx[j], C->pvx[j + 1], C->pvy[j], C->pvy[j + 1],
                            C->ph[j], C->ph[j + 1], &C->fp[4 * (j + 2)], &C->fn[4 * (j + 2)], fpo, fno);
+            if (zs) {
+                zs_count(C, zs_limit(C->gamma1, &C->fp[4 * (j + 2)], fpo));
+                zs_count(C, zs_limit(C->gamma1, &C->fn[4 * (j + 2)], fno));
+            }
             for (int c = 0; c < 4; ++c) {
                 U4(C->gpf, c, i, j) = scale ? alphamax * fpo[c] : fpo[c];
                 U4(C->gnf, c, i, j) = scale ? alphamax * fno[c] : fno[c];

@@ -34,7 +34,7 @@ typedef struct swo_config {
     double eps2;     /* velocity cut-off in fflux/findalpha, 1d-8 */
     double delta;    /* density sweep threshold, 1d-12 */
     int32_t bc;      /* enum swo_bc */
-    int32_t use_limiter;  /* 2 = sweeping post-process, 0 = none */
+    int32_t use_limiter;  /* 2 = sweeping post-process, 1 = Zhang-Shu flux limiter, 0 = none */
     int32_t swp_num;
     /* variant switches (0 = ex4 behaviour) */
     int32_t alpha_in_scheme;   /* ex3/ex5: fluxes unscaled in wenofd, scheme multiplies by alphamax */

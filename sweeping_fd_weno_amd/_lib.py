@@ -128,18 +128,19 @@ def _config(**kw) -> SwfConfig:
 
 
 def sedov_config(n: int = 640, precision="fp64_exact", cfl: float = 0.5, swp_num: int = 20000,
-                 xm: float = 1.3, device: int = 0) -> SwfConfig:
-    """2D Sedov blast on [0, xm]^2 (ex4_sedov_2d, cut_x = cut_y = 1: dx = xm/nx)."""
+                 xm: float = 1.3, device: int = 0, use_limiter: int = 2) -> SwfConfig:
+    """2D Sedov blast on [0, xm]^2 (ex4_sedov_2d, cut_x = cut_y = 1: dx = xm/nx).
+    use_limiter: 2 = sweeping, 1 = Zhang-Shu flux limiter, 0 = none."""
     return _config(nx=n, ny=n, xlo=0.0, ylo=0.0, dx=(xm - 0.0) / n, dy=(xm - 0.0) / n, gamma=1.4, cfl=cfl,
-                   eps=1e-13, eps2=1e-8, delta=1e-12, bc=BC_SEDOV, use_limiter=2, swp_num=swp_num,
+                   eps=1e-13, eps2=1e-8, delta=1e-12, bc=BC_SEDOV, use_limiter=use_limiter, swp_num=swp_num,
                    pbar_check=1, fail_stop=1, precision=_prec(precision), device=device)
 
 
 def vortex_config(n: int = 180, precision="fp64_exact", cfl: float = 0.5, swp_num: int = 20000,
-                  device: int = 0) -> SwfConfig:
+                  device: int = 0, use_limiter: int = 2) -> SwfConfig:
     """Low-density isentropic vortex on the periodic [0, 10]^2 (ex3_vortex_accuracy_2d)."""
     return _config(nx=n, ny=n, xlo=0.0, ylo=0.0, dx=(10.0 - 0.0) / n, dy=(10.0 - 0.0) / n, gamma=1.4,
-                   cfl=cfl, eps=1e-13, eps2=1e-8, delta=1e-12, bc=BC_PERIODIC, use_limiter=2,
+                   cfl=cfl, eps=1e-13, eps2=1e-8, delta=1e-12, bc=BC_PERIODIC, use_limiter=use_limiter,
                    swp_num=swp_num, alpha_in_scheme=1, roe_eps_double=1, ydens_plain=1, eps2_quad_literal=1,
                    findalpha_ex3=1, dt_rule_dx53=1, poscheck_le=1, precision=_prec(precision), device=device)
 

@@ -82,7 +82,8 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   if (!cfg || !out) return SWF_E_INVALID;
   *out = nullptr;
   if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->dx > 0) || !(cfg->dy > 0) || cfg->precision < 0 || cfg->precision > 2 ||
-      cfg->bc < 0 || cfg->bc > 3 || (cfg->use_limiter == 2 && cfg->swp_num < 1))
+      cfg->bc < 0 || cfg->bc > 3 || cfg->use_limiter < 0 || cfg->use_limiter > 2 ||
+      (cfg->use_limiter == 2 && cfg->swp_num < 1))
     return SWF_E_INVALID;
   if ((long long)cfg->nx * cfg->ny >= (1LL << 31) - 64) return SWF_E_INVALID;
   int ndev = 0;
@@ -164,10 +165,12 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&trk, sizeof(long long) * 3 * (size_t)kTrackCap);
   double *stage = nullptr;
   if (e == hipSuccess) e = hipMalloc(&stage, sizeof(double) * c->n_fortran);
+  // stage-tile alpha partials, then the flux limiter's (den, pre) count pairs
   const size_t nblk = (size_t)ub_gx(k) * ub_gy(k);
+  const size_t part_bytes = sizeof(double) * nblk + sizeof(int) * 2 * (size_t)zs_pairs(k);
   double *apart = nullptr;
-  if (e == hipSuccess) e = hipMalloc(&apart, sizeof(double) * nblk);
-  if (e == hipSuccess) e = hipMemsetAsync(apart, 0, sizeof(double) * nblk, c->stream);
+  if (e == hipSuccess) e = hipMalloc(&apart, part_bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(apart, 0, part_bytes, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(pool, 0, 5 * pbytes, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(maps, 0, sizeof(unsigned) * 4 * (size_t)k.nwords, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(g, 0, sizeof(DevScalars), c->stream);
@@ -200,6 +203,7 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   c->b.g = g;
   c->b.trk = trk;
   c->b.apart = apart;
+  c->b.zs = reinterpret_cast<int *>(apart + nblk);
   c->d_fortran = stage;
   *out = c;
   return SWF_OK;
@@ -257,12 +261,18 @@ static Bufs stage_bufs(const swf_ctx *c, int s) {
 // one RK stage (s = 1, 2, 3) of one step with parity p: wenofd x-pass; the
 // y-pass fused with scheme, the RK combination, checkpositivity and bc images;
 // then one workgroup for sweep_lim (when flagged), findalpha of the new state
-// and, at s = 3, the time.
+// and, at s = 3, the time.  With the flux limiter (use_limiter = 1) the halves
+// outside the flux tiles are counted from the stage input, and the new state gets
+// its full ghost frame, which those halves read at the next stage.
 static void enqueue_stage(swf_ctx *c, int s, int p) {
   const int slot_in = s - 1, slot_out = s % 3;
   const Bufs b = stage_bufs(c, s);
   launch_xflux(c->mode, c->k, b, c->stream, s, slot_in, p);
   launch_ystage(c->mode, c->k, b, c->stream, s, slot_in, p, 0);
+  if (c->k.use_limiter == 1) {
+    launch_zs_edges(c->mode, c->k, b, c->stream, slot_in, p);
+    launch_bc(c->k, b, c->stream, -1, p);
+  }
   launch_stage_end(c->k, b, c->stream, s, slot_out, p, c->mode == MODE_EXACT ? 1 : 0);
 }
 

@@ -68,6 +68,8 @@ struct Bufs {
   DevScalars *g;
   long long *trk;                   // 3 * track_cap: q, neg, pos per sweep trigger
   double *apart;                    // alpha max of each update workgroup
+  int *zs;                          // flux limiter: (den, pre) counts per workgroup of the
+                                    // x-flux, stage and edge kernels (zs_offsets)
 };
 
 // Row layout of every plane: cell i (Fortran index, ghosts from -2) at column
@@ -88,6 +90,21 @@ constexpr int UB_W = 16, UB_H = 15;
 SWF_HD int ub_gx(const KCfg &k) { return (k.nx + UB_W - 1) / UB_W; }
 SWF_HD int ub_gy(const KCfg &k) { return (k.ny + UB_H - 1) / UB_H; }
 SWF_HD int ub_of(const KCfg &k, int i, int j) { return ((j - 1) / UB_H) * ub_gx(k) + (i - 1) / UB_W; }
+
+// x-flux workgroups: 64 interfaces x 4 rows, tile origins 64 bx - XT_SHIFT
+// covering interfaces 0..nx
+SWF_HD int xf_gx(const KCfg &k) { return (k.nx + XT_SHIFT + 64) / 64; }
+SWF_HD int xf_gy(const KCfg &k) { return (k.ny + 3) / 4; }
+
+// Flux-limiter half fluxes that `scheme` does not read but the reference's
+// wenofd limits and counts (use_limiter == 1): x-pass ghost rows 0 and ny+1
+// (f+ and f- of cells 0..nx+1), f+ of cell nx+1 and f- of cell 0 in rows 1..ny,
+// and the same for the y-pass.
+SWF_HD int zs_edge_halves(const KCfg &k) { return 6 * (k.nx + k.ny) + 16; }
+// (den, pre) count pairs: x-flux workgroups, stage workgroups, edge workgroups
+SWF_HD int zs_pairs(const KCfg &k) {
+  return xf_gx(k) * xf_gy(k) + ub_gx(k) * ub_gy(k) + (zs_edge_halves(k) + 255) / 256;
+}
 
 // ---------------------------------------------------------------------------
 // Sweep orders (ex4_sedov_2d/main.f90:1836-1895), closed form, 0-based positions
@@ -446,6 +463,55 @@ SWF_HD double cell_alpha_fast(const KCfg &k, double u1, double u2, double u3, do
   }
   const double sound = sqrt((k.gamma * pres) * r);
   return fmax2(ax + sound, ay + sound);
+}
+
+// Zhang-Shu positivity limiter of one reconstructed split flux F (before the
+// alpha scaling) against the split flux `ave` of its centre cell, components
+// CS doubles apart (use_limiter == 1; ex4_sedov_2d/main.f90:306-412 x-pass,
+// :527-629 y-pass).  Density first: theta from the points F and
+// (ave - F/12)/(11/12); then pressure, with the density component of both points
+// updated and the others as before.  The ex4 y-pass clamps the pressure theta of
+// gp at 0 (:573), which for finite data (theta >= 0) changes nothing.  Returns
+// bit 0 / bit 1 when the density / pressure step changed F (den_count /
+// pre_count of the reference).
+template <int CS>
+SWF_HD int zs_limit(double g1, const double *ave, double F[4]) {
+#pragma clang fp contract(off)
+  int changed = 0;
+  const double d_ave = ave[0], m_ave = ave[CS], n_ave = ave[2 * CS], e_ave = ave[3 * CS];
+  const double p_ave = g1 * ((e_ave - (0.5 * (m_ave * m_ave)) / d_ave) - (0.5 * (n_ave * n_ave)) / d_ave);
+  const double av[4] = {d_ave, m_ave, n_ave, e_ave};
+  double pt[2][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    pt[0][c] = F[c];
+    pt[1][c] = (av[c] - F[c] / 12.) / (11. / 12.);
+  }
+  double theta = 1.;
+#pragma unroll
+  for (int ig = 0; ig < 2; ++ig) {
+    const double d_l = pt[ig][0];
+    if (d_l < 0.) theta = fmin2(theta, fabs(d_ave - 0.) / fabs(d_ave - d_l));
+  }
+  if (theta < 1.) {
+    F[0] = theta * F[0] + (1. - theta) * d_ave;
+    changed |= 1;
+  }
+  theta = 1.;
+  pt[0][0] = F[0];
+  pt[1][0] = (d_ave - F[0] / 12.) / (11. / 12.);
+#pragma unroll
+  for (int ig = 0; ig < 2; ++ig) {
+    const double d_l = pt[ig][0], m_l = pt[ig][1], n_l = pt[ig][2], e_l = pt[ig][3];
+    const double p_l = g1 * ((e_l - ((m_l * m_l) / d_l) / 2.) - ((n_l * n_l) / d_l) / 2.);
+    if (p_l < 0.) theta = fmin2(theta, fabs(p_ave - 0.) / fabs(p_ave - p_l));
+  }
+  if (theta < 1.) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) F[c] = theta * F[c] + (1. - theta) * av[c];
+    changed |= 2;
+  }
+  return changed;
 }
 
 // sweep flag: P <= eps or rho <= eps (ex4_sedov_2d/main.f90:1465-1466)

@@ -9,10 +9,21 @@
 namespace swf {
 
 void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
-  hipLaunchKernelGGL(k_xflux<MODE_EXACT>, grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
+  if (k.use_limiter == 1)
+    hipLaunchKernelGGL((k_xflux<MODE_EXACT, true>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
+  else
+    hipLaunchKernelGGL((k_xflux<MODE_EXACT, false>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
 }
 void launch_ystage_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags) {
-  hipLaunchKernelGGL(k_ystage<MODE_EXACT>, grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity, flags);
+  if (k.use_limiter == 1)
+    hipLaunchKernelGGL((k_ystage<MODE_EXACT, true>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity,
+                       flags);
+  else
+    hipLaunchKernelGGL((k_ystage<MODE_EXACT, false>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity,
+                       flags);
+}
+void launch_zs_edges_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
+  hipLaunchKernelGGL(k_zs_edges<MODE_EXACT>, dim3(grid_zs_edges(k)), dim3(256), 0, s, k, b, slot_in, parity);
 }
 void launch_weno_points_m0(const double *a, double *r, double *l, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_weno_points<MODE_EXACT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, r, l, n);
@@ -30,6 +41,12 @@ void launch_ystage(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int st
     launch_ystage_m0(k, b, s, stage, slot_in, parity, flags);
   else
     launch_ystage_m12(mode, k, b, s, stage, slot_in, parity, flags);
+}
+void launch_zs_edges(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
+  if (mode == MODE_EXACT)
+    launch_zs_edges_m0(k, b, s, slot_in, parity);
+  else
+    launch_zs_edges_m12(mode, k, b, s, slot_in, parity);
 }
 void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, hipStream_t s) {
   if (mode == MODE_EXACT)

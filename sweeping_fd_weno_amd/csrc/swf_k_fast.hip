@@ -5,18 +5,37 @@
 
 namespace swf {
 
+// the flux limiter (use_limiter = 1) has kernels of its own, so that the others
+// carry none of its registers
+template <int M, bool LIM>
+static void xflux(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
+  hipLaunchKernelGGL((k_xflux<M, LIM>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
+}
+template <int M, bool LIM>
+static void ystage(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags) {
+  hipLaunchKernelGGL((k_ystage<M, LIM>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity, flags);
+}
+
 void launch_xflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
+  const bool lim = k.use_limiter == 1;
   if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_xflux<MODE_FP64>, grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
+    (lim ? xflux<MODE_FP64, true> : xflux<MODE_FP64, false>)(k, b, s, stage, slot_in, parity);
   else
-    hipLaunchKernelGGL(k_xflux<MODE_FP32>, grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
+    (lim ? xflux<MODE_FP32, true> : xflux<MODE_FP32, false>)(k, b, s, stage, slot_in, parity);
 }
 void launch_ystage_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
                        int flags) {
+  const bool lim = k.use_limiter == 1;
   if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_ystage<MODE_FP64>, grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity, flags);
+    (lim ? ystage<MODE_FP64, true> : ystage<MODE_FP64, false>)(k, b, s, stage, slot_in, parity, flags);
   else
-    hipLaunchKernelGGL(k_ystage<MODE_FP32>, grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity, flags);
+    (lim ? ystage<MODE_FP32, true> : ystage<MODE_FP32, false>)(k, b, s, stage, slot_in, parity, flags);
+}
+void launch_zs_edges_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
+  if (mode == MODE_FP64)
+    hipLaunchKernelGGL(k_zs_edges<MODE_FP64>, dim3(grid_zs_edges(k)), dim3(256), 0, s, k, b, slot_in, parity);
+  else
+    hipLaunchKernelGGL(k_zs_edges<MODE_FP32>, dim3(grid_zs_edges(k)), dim3(256), 0, s, k, b, slot_in, parity);
 }
 void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, hipStream_t s) {
   const int blocks = (int)((n + 255) / 256);

@@ -1,14 +1,18 @@
 // swf_kernels.h - grid kernels of one SSP-RK3 stage, templated on precision mode.
 //
 // One stage (ex4_sedov_2d/main.f90:685-777 `rk`) is three launches:
-//   k_xflux<M>   wenofd x-pass: net x-interface flux F(i+1/2) = fp_flux(i) - fn_flux(i+1)
+//   k_xflux<M,LIM> wenofd x-pass: net x-interface flux F(i+1/2) = fp_flux(i) - fn_flux(i+1)
 //                (scaled by alpha as :414-415) into Fx; also clears the stage's
 //                sweep bitmaps and flags
-//   k_ystage<M>  wenofd y-pass for a tile, then per cell `scheme` (RHS from Fx and
+//   k_ystage<M,LIM> wenofd y-pass for a tile, then per cell `scheme` (RHS from Fx and
 //                the tile's y fluxes), the RK combination, checkpositivity, sweep
 //                flags, ghost images (`bc`) and the tile's alpha max (findalpha)
 //   k_stage_end  (mode-independent, one workgroup) sweep_lim when flagged, alpha
 //                reduction, and at stage 3 the time update
+// With the Zhang-Shu flux limiter (use_limiter = 1; LIM = true) the two flux
+// kernels limit every half flux and count the changes per workgroup, and two
+// launches come before k_stage_end: k_zs_edges<M> (the halves of the ghost rows
+// and columns, counted only) and k_bc (the new state's full ghost frame).
 // Each flux tile stages its stencil points (split fluxes and Roe primitives) in
 // LDS once; every thread then computes one interface: the f+ half with the Roe
 // system of (i, i+1) and the f- half with that of (i+1, i+2), eight WENO
@@ -37,6 +41,7 @@ struct XSmem {
   double fp[4][XT_H][XT_PTS];
   double fn[4][XT_H][XT_PTS];
   double pr[4][XT_H][XT_PTS];
+  int zred[8];
 };
 struct YSmem {
   static constexpr int CS = YT_PTS * YT_W;
@@ -45,6 +50,7 @@ struct YSmem {
   double pr[4][YT_PTS][YT_W];
   double G[4][YT_H][YT_W];  // net y-interface fluxes of the tile
   double red[4];
+  int zred[8];
 };
 
 // split fluxes and Roe primitives of stencil point u
@@ -110,15 +116,41 @@ SWF_D Prim load_prim(const double *pr) {
 // point p, Roe(p, p+1)) minus f- of the next point (centre p+1, Roe(p+1, p+2)),
 // scaled by alpha inside wenofd (ex4) or later in `scheme` (ex3/ex5).  Pointers
 // are at component 0 of point p; points SS doubles apart, components CS.
-template <int M, int DIR, int SS, int CS>
-SWF_D void iface_flux(const KCfg &k, const double *pr, const double *fp, const double *fn, double alpha,
-                      double F[4]) {
+template <int M, int DIR, int SS, int CS, bool LIM>
+SWF_D int iface_flux(const KCfg &k, const double *pr, const double *fp, const double *fn, double alpha,
+                     double F[4]) {
   const Prim p0 = load_prim<CS>(pr), p1 = load_prim<CS>(pr + SS), p2 = load_prim<CS>(pr + 2 * SS);
   double a[4], c[4];
   half_flux<M, DIR, true, SS, CS>(k, p0, p1, fp, a);
   half_flux<M, DIR, false, SS, CS>(k, p1, p2, fn + SS, c);
+  int lim = 0;
+  if (LIM) lim = zs_limit<CS>(k.gamma1, fp, a) | (zs_limit<CS>(k.gamma1, fn + SS, c) << 2);
 #pragma unroll
   for (int q = 0; q < 4; ++q) F[q] = k.alpha_in_scheme ? (a[q] - c[q]) : (alpha * a[q] - alpha * c[q]);
+  return lim;
+}
+
+// den_count / pre_count increments of the limiter change bits of iface_flux
+SWF_D int zs_den(int lim) { return (lim & 1) + ((lim >> 2) & 1); }
+SWF_D int zs_pre(int lim) { return ((lim >> 1) & 1) + ((lim >> 3) & 1); }
+
+// sums of two counts over a 256-thread block, stored by thread 0 as one
+// (den, pre) pair of the limiter count partials
+SWF_D void zs_store(int den, int pre, int *pair, int *red, int tid) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    den += __shfl_xor(den, off, 64);
+    pre += __shfl_xor(pre, off, 64);
+  }
+  if ((tid & 63) == 0) {
+    red[2 * (tid >> 6)] = den;
+    red[2 * (tid >> 6) + 1] = pre;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    pair[0] = (red[0] + red[2]) + (red[4] + red[6]);
+    pair[1] = (red[1] + red[3]) + (red[5] + red[7]);
+  }
 }
 
 // Occupancy of the fast-mode flux kernels: 4 waves per SIMD (at most 128 VGPRs);
@@ -129,7 +161,7 @@ SWF_D void iface_flux(const KCfg &k, const double *pr, const double *fp, const d
 // (stage = 1, 2, 3; 0 for a stand-alone evaluation) the launch also resets the
 // stage's sweep bitmaps and flags (the stage kernel sets them), and at stage 1
 // of an inactive step keeps the next step inactive.
-template <int M>
+template <int M, bool LIM>
 __global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_in, int parity) {
   __shared__ XSmem sm;
   constexpr int CS = XSmem::CS;
@@ -174,12 +206,16 @@ __global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_i
   __syncthreads();
   const int tx = tid & (XT_W - 1), ty = tid / XT_W;
   const int kk = k0 + tx, j = j0 + ty;
-  if (kk < 0 || kk > k.nx || j > k.ny) return;
-  double F[4];
-  iface_flux<M, 0, 1, CS>(k, &sm.pr[0][ty][tx + 2], &sm.fp[0][ty][tx + 2], &sm.fn[0][ty][tx + 2], alpha, F);
-  const long long o = cidx(k, kk, j);
+  int lim = 0;
+  if (kk >= 0 && kk <= k.nx && j <= k.ny) {
+    double F[4];
+    lim = iface_flux<M, 0, 1, CS, LIM>(k, &sm.pr[0][ty][tx + 2], &sm.fp[0][ty][tx + 2], &sm.fn[0][ty][tx + 2], alpha, F);
+    const long long o = cidx(k, kk, j);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) b.Fx[c * k.plane + o] = F[c];
+    for (int c = 0; c < 4; ++c) b.Fx[c * k.plane + o] = F[c];
+  }
+  if (LIM && stage > 0)
+    zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (blockIdx.x + blockIdx.y * gridDim.x), sm.zred, tid);
 }
 
 // dt of the step (ex4_sedov_2d/main.f90:696-700; ex3 :570-572)
@@ -340,7 +376,7 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
 // time after the step and the gate of the next step (the loop test of :112).
 // flags: YS_RHS_ONLY stores RHS into b.R instead (parity tests of the spatial
 // operator); YS_NO_SCALARS leaves the device scalars alone (kernel timing).
-template <int M>
+template <int M, bool LIM>
 __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_in, int parity, int flags) {
   __shared__ YSmem sm;
   DevScalars *g = b.g;
@@ -374,9 +410,12 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
       });
   __syncthreads();
   double G[4] = {0.0, 0.0, 0.0, 0.0};
+  int lim = 0;
   if (i <= k.nx && jf <= k.ny)
-    iface_flux<M, 1, YT_W, YSmem::CS>(k, &sm.pr[0][ty + 2][tx], &sm.fp[0][ty + 2][tx], &sm.fn[0][ty + 2][tx], alpha,
-                                      G);
+    lim = iface_flux<M, 1, YT_W, YSmem::CS, LIM>(k, &sm.pr[0][ty + 2][tx], &sm.fp[0][ty + 2][tx], &sm.fn[0][ty + 2][tx],
+                                            alpha, G);
+  // a tile's first interface is the previous tile's last: counted there
+  if (ty == 0 && blockIdx.y > 0) lim = 0;
 #pragma unroll
   for (int c = 0; c < 4; ++c) sm.G[c][ty][tx] = G[c];
   __syncthreads();
@@ -449,6 +488,87 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
   if (rhs_only) return;
   const double am = block_max<256>(anew, sm.red, tid);
   if (tid == 0 && scalars) b.apart[blockIdx.x + blockIdx.y * gridDim.x] = am;
+  if (LIM && scalars)
+    zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (xf_gx(k) * xf_gy(k) + blockIdx.x + blockIdx.y * gridDim.x),
+             sm.zred, tid);
+}
+
+// One limited half flux outside the flux tiles (use_limiter == 1): the half of
+// cell `a` on line `line` (row for DIR 0, column for DIR 1) with the Roe system
+// of (a, a+1), f+ (RIGHT) or f-, from the stage input.  Returns its limiter
+// change bits (density, pressure).
+template <int M, int DIR, bool RIGHT>
+SWF_D int zs_edge_half(const KCfg &k, const Bufs &b, int line, int a, double alpha, double ia) {
+  constexpr int NS = 5;  // stencil points a-2 .. a+2 (Roe uses a, a+1)
+  double st[4 * NS], prm[4 * NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const long long o = DIR == 0 ? cidx(k, a - 2 + q, line) : cidx(k, line, a - 2 + q);
+    double u[4], fp[4], fn[4], pr[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) u[c] = b.Uin[c * k.plane + o];
+    stage_point<M, DIR>(k, u, alpha, ia, fp, fn, pr);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      st[c * NS + q] = RIGHT ? fp[c] : fn[c];
+      prm[c * NS + q] = pr[c];
+    }
+  }
+  double out[4];
+  half_flux<M, DIR, RIGHT, 1, NS>(k, load_prim<NS>(&prm[2]), load_prim<NS>(&prm[3]), &st[2], out);
+  return zs_limit<NS>(k.gamma1, &st[2], out);
+}
+
+// Limiter counts of the halves that wenofd evaluates (ex3_vortex_accuracy_2d/
+// main.f90:224-340, :428-540 with use_limiter = 1: every cell 0..n+1 of rows
+// 0..ny+1 / columns 0..nx+1, both halves) and the flux tiles do not: the ghost
+// rows (x) and columns (y) in full, and in the interior lines f+ of cell n+1 and
+// f- of cell 0.  One thread per half, zs_edge_halves in all; the stage input's
+// ghost frame must be complete (k_bc after every stage).
+template <int M>
+__global__ __launch_bounds__(256) void k_zs_edges(KCfg k, Bufs b, int slot_in, int parity) {
+  __shared__ int zred[8];
+  DevScalars *g = b.g;
+  if (!g->active[parity] || g->status != ST_OK) return;
+  const int tid = threadIdx.x;
+  const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
+  const double ia = 1.0 / alpha;
+  const int nx = k.nx, ny = k.ny;
+  int t = blockIdx.x * 256 + tid;
+  int lim = 0;
+  if (t < zs_edge_halves(k)) {
+    int dir, line, a;
+    bool right;
+    if (t < 4 * (nx + 2)) {  // x: rows 0 and ny+1, cells 0..nx+1
+      dir = 0;
+      line = t < 2 * (nx + 2) ? 0 : ny + 1;
+      const int r = t % (2 * (nx + 2));
+      a = r >> 1;
+      right = (r & 1) == 0;
+    } else if ((t -= 4 * (nx + 2)) < 2 * ny) {  // x: rows 1..ny, f+ of cell nx+1, f- of cell 0
+      dir = 0;
+      line = 1 + (t >> 1);
+      right = (t & 1) == 0;
+      a = right ? nx + 1 : 0;
+    } else if ((t -= 2 * ny) < 4 * (ny + 2)) {  // y: columns 0 and nx+1, cells 0..ny+1
+      dir = 1;
+      line = t < 2 * (ny + 2) ? 0 : nx + 1;
+      const int r = t % (2 * (ny + 2));
+      a = r >> 1;
+      right = (r & 1) == 0;
+    } else {  // y: columns 1..nx, g+ of cell ny+1, g- of cell 0
+      t -= 4 * (ny + 2);
+      dir = 1;
+      line = 1 + (t >> 1);
+      right = (t & 1) == 0;
+      a = right ? ny + 1 : 0;
+    }
+    if (dir == 0)
+      lim = right ? zs_edge_half<M, 0, true>(k, b, line, a, alpha, ia) : zs_edge_half<M, 0, false>(k, b, line, a, alpha, ia);
+    else
+      lim = right ? zs_edge_half<M, 1, true>(k, b, line, a, alpha, ia) : zs_edge_half<M, 1, false>(k, b, line, a, alpha, ia);
+  }
+  zs_store(lim & 1, (lim >> 1) & 1, b.zs + 2 * (xf_gx(k) * xf_gy(k) + ub_gx(k) * ub_gy(k) + blockIdx.x), zred, tid);
 }
 
 #ifdef SWF_COMMON_KERNELS
@@ -463,6 +583,7 @@ __global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, i
                                                     int serial_ubar) {
   __shared__ SweepShared<SW_T> sh;
   __shared__ double red[SW_T / 64];
+  __shared__ long long zc[2 * (SW_T / 64)];
   DevScalars *g = b.g;
   const int tid = threadIdx.x;
   const int nblk = ub_gx(k) * ub_gy(k);
@@ -470,6 +591,31 @@ __global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, i
   double a = 0.0;
   for (int q = tid; q < nblk; q += SW_T) a = fmax2(a, b.apart[q]);
   if (!act || st != ST_OK) return;
+  if (k.use_limiter == 1) {
+    // den_count / pre_count of the stage's flux limiter, from the workgroup pairs
+    long long den = 0, pre = 0;
+    const int np = zs_pairs(k);
+    for (int q = tid; q < np; q += SW_T) {
+      den += b.zs[2 * q];
+      pre += b.zs[2 * q + 1];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      den += __shfl_xor(den, off, 64);
+      pre += __shfl_xor(pre, off, 64);
+    }
+    if ((tid & 63) == 0) {
+      zc[2 * (tid >> 6)] = den;
+      zc[2 * (tid >> 6) + 1] = pre;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 0; w < SW_T / 64; ++w) {
+        g->den_count += zc[2 * w];
+        g->pre_count += zc[2 * w + 1];
+      }
+    }
+  }
   if (k.use_limiter == 2 && flagged) {
     sweep_body<SW_T>(k, b, serial_ubar != 0, sh, tid);
     after_sweep<SW_T>(k, b, sh, red, tid);

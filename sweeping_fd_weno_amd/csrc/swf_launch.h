@@ -12,14 +12,17 @@ namespace swf {
 void launch_xflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity);
 void launch_ystage(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
                    int flags);
+void launch_zs_edges(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity);
 void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, hipStream_t s);
 
 void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity);
 void launch_ystage_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags);
+void launch_zs_edges_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity);
 void launch_weno_points_m0(const double *a, double *r, double *l, long long n, hipStream_t s);
 void launch_xflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity);
 void launch_ystage_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
                        int flags);
+void launch_zs_edges_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity);
 void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, hipStream_t s);
 
 // mode-independent kernels (swf_k_exact.hip)
@@ -37,7 +40,8 @@ void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_
 
 // x-flux tiles: 64 interfaces x 4 rows covering interfaces 0..nx from origin
 // -XT_SHIFT (swf_kernels.h); stage tiles: UB_W x UB_H cells
-inline dim3 grid_xflux(const KCfg &k) { return dim3((k.nx + XT_SHIFT + 64) / 64, (k.ny + 3) / 4); }
+inline dim3 grid_xflux(const KCfg &k) { return dim3(xf_gx(k), xf_gy(k)); }
+inline unsigned grid_zs_edges(const KCfg &k) { return (unsigned)((zs_edge_halves(k) + 255) / 256); }
 inline dim3 grid_ystage(const KCfg &k) { return dim3(ub_gx(k), ub_gy(k)); }
 inline dim3 grid_cells(const KCfg &k) { return dim3((k.nx + 63) / 64, (k.ny + 3) / 4); }
 

@@ -7,11 +7,13 @@
 ! files and the sweep statistics) and replaces its time-stepping procedures (`rk`
 ! and callees) by the C-ABI of libswf.so.
 !
-!   swf_vortex [nx precision new_table]
+!   swf_vortex [nx precision new_table use_limiter]
 !
 ! Runs in the working directory like the reference (fort.7, err_table.dat and the
 ! .dat files there).  precision: 0 = FP64-exact (bit-identical to the reference),
 ! 1 = FP64, 2 = FP32-bulk.  new_table = 1 starts a new err_table.dat (orders 0).
+! use_limiter: the reference's compile-time switch (main.f90:5), 2 = sweeping
+! (as shipped), 1 = Zhang-Shu flux limiter.
 ! Compiled like the reference, with -fdefault-real-8: the d-exponent literals
 ! (loop test, log(2D0)) are binary128 exactly as there.
 program swf_vortex
@@ -19,10 +21,10 @@ program swf_vortex
    use iso_c_binding
    implicit none
    ! parameter surface (ex3_vortex_accuracy_2d/main.f90:5-9; weno5.f90:78 for pi)
-   integer, parameter :: ierror = 1, use_limiter = 2, swp_num = 20000
+   integer, parameter :: ierror = 1, swp_num = 20000
    real(8), parameter :: time_terminal = 0.01, gamma = 1.4, gamma1 = gamma - 1., eps = 1d-13
    real(8), parameter :: pi = 3.14159265358979323846264338327950
-   integer :: nx, ny, n, prec, itab
+   integer :: nx, ny, n, prec, itab, use_limiter
    logical :: new_table
    character(len=64) :: arg
    real(8), allocatable :: u(:, :, :), x(:), y(:)
@@ -38,6 +40,7 @@ program swf_vortex
    nx = 180
    prec = SWF_FP64_EXACT
    itab = 0
+   use_limiter = 2
    nargs = command_argument_count()
    if (nargs >= 1) then
       call get_command_argument(1, arg)
@@ -50,6 +53,10 @@ program swf_vortex
    if (nargs >= 3) then
       call get_command_argument(3, arg)
       read (arg, *) itab
+   end if
+   if (nargs >= 4) then
+      call get_command_argument(4, arg)
+      read (arg, *) use_limiter
    end if
    new_table = itab /= 0
    ny = nx
@@ -189,18 +196,20 @@ program swf_vortex
    call check(rc)
    allocate (tq(max(st%sweep_count, 1_8)), tn(max(st%sweep_count, 1_8)), tp(max(st%sweep_count, 1_8)))
    nstep = swf_track(ctx, tq, tn, tp, st%sweep_count)
+   ! the reference's track arrays are default integers: maxval of none of them
+   ! prints -huge(0) - 1, as there
    print *, 'total sweep = ', sum(tq(1:st%sweep_count))
    avg = real(sum(tq(1:st%sweep_count)), 8)/real(st%sweep_count, 8)
    print *, 'avg sweep = ', avg
-   print *, 'max sweep = ', maxval(tq(1:st%sweep_count))
+   print *, 'max sweep = ', maxval(int(tq(1:st%sweep_count)))
    print *, 'total negative pairs = ', sum(tn(1:st%sweep_count))
    avg = real(sum(tn(1:st%sweep_count)), 8)/real(st%sweep_count, 8)
    print *, 'avg negative pairs = ', avg
-   print *, 'max negative pairs = ', maxval(tn(1:st%sweep_count))
+   print *, 'max negative pairs = ', maxval(int(tn(1:st%sweep_count)))
    print *, 'total positive neighbors = ', sum(tp(1:st%sweep_count))
    avg = real(sum(tp(1:st%sweep_count)), 8)/real(st%sweep_count, 8)
    print *, 'avg positive neighbors = ', avg
-   print *, 'max positive neighbors = ', maxval(tp(1:st%sweep_count))
+   print *, 'max positive neighbors = ', maxval(int(tp(1:st%sweep_count)))
    print *, '# of limiter', st%den_count, st%pre_count
    call swf_destroy(ctx)
 

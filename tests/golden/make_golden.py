@@ -122,6 +122,38 @@ def make_ex3() -> None:
         json.dump(out, f, indent=1)
 
 
+def make_ex3_l1() -> None:
+    """ex3 with use_limiter = 1 (oracle/Makefile target ex3_l1): the Zhang-Shu flux
+    limiter instead of the sweep; pins the limiter's fields and den/pre counts."""
+    rundir = os.path.join(REF, "run_ex3_l1")
+    text = _run(os.path.join(REF, "ex3_l1", "a"), rundir)
+    den = _tokens(os.path.join(rundir, "den.dat"), 5)
+    m1 = _tokens(os.path.join(rundir, "m1.dat"), 3)
+    m2 = _tokens(os.path.join(rundir, "m2.dat"), 3)
+    pre = _tokens(os.path.join(rundir, "pre.dat"), 3)
+    lines = text.splitlines()
+    l1, linf = (float(v) for v in lines[1].split())
+    fields = {"rho": den[:, 2], "m1": m1[:, 2], "m2": m2[:, 2], "p": pre[:, 2]}
+    limiter = [l for l in lines if l.strip().startswith("# of limiter")][0].split()[-2:]
+    out = {
+        "program": "ex3_vortex_accuracy_2d with use_limiter = 1, nx=ny=180, T=0.01, cfl=0.5, mag=10.0828",
+        "layout": "fields in (j outer, i inner) order over the 180x180 interior",
+        "sha256": {k: field_digest(v) for k, v in fields.items()},
+        "sample_stride": 37,
+        "sample": {k: v[::37].tolist() for k, v in fields.items()},
+        "L1": l1, "Linf": linf,
+        "den_count": int(limiter[0]), "pre_count": int(limiter[1]),
+        "err_table_dat": open(os.path.join(rundir, "err_table.dat")).read(),
+        "fort7": open(os.path.join(rundir, "fort.7")).read(),
+        "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
+                        for fn in ("den.dat", "pre.dat", "m1.dat", "m2.dat", "den2.dat", "pre2.dat")},
+        "stdout_head": "\n".join(lines[:4]),
+        "limiter_line": [l for l in lines if l.strip().startswith("# of limiter")][0],
+    }
+    with open(os.path.join(GOLD, "ref_ex3_l1_vortex180.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def make_ex5() -> None:
     rundir = os.path.join(REF, "run_ex5")
     if not os.path.exists(os.path.join(rundir, "DONE")):
@@ -148,11 +180,13 @@ def make_ex5() -> None:
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["weno", "ex3", "ex5"]
+    what = sys.argv[1:] or ["weno", "ex3", "ex3_l1", "ex5"]
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     if "weno" in what:
         make_weno()
     if "ex3" in what:
         make_ex3()
+    if "ex3_l1" in what:
+        make_ex3_l1()
     if "ex5" in what:
         make_ex5()

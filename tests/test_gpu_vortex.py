@@ -38,6 +38,23 @@ def test_vortex_driver_reproduces_reference_output_files(tmp_path):
     assert int(line.split("=")[1]) == g["stats"]["total sweep"]
 
 
+def test_vortex_driver_flux_limiter_reproduces_reference_output_files(tmp_path):
+    """The same program with use_limiter = 1 (4th argument), against the reference
+    built with that switch: output files, printed norms and limiter counts."""
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_ex3_l1_vortex180.json")))
+    (tmp_path / "fort.7").write_text("%13.5e%13.5e\n" % (1.0, 1.0))
+    (tmp_path / "err_table.dat").write_text("")
+    res = subprocess.run([DRIVER, "180", "0", "0", "1"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert (tmp_path / "err_table.dat").read_text() == g["err_table_dat"]
+    assert (tmp_path / "fort.7").read_text() == g["fort7"]
+    assert "\n".join(res.stdout.splitlines()[:4]) == g["stdout_head"]
+    for fn, digest in g["file_sha256"].items():
+        assert hashlib.sha256((tmp_path / fn).read_bytes()).hexdigest() == digest, fn
+    line = [l for l in res.stdout.splitlines() if "# of limiter" in l][0]
+    assert [int(v) for v in line.split()[-2:]] == [g["den_count"], g["pre_count"]]
+
+
 def test_vortex_accuracy_series():
     g = json.load(open(GOLD))
     rows = A.vortex_series((45, 90, 180, 360), ("fp64_exact", "fp64", "fp32_bulk"))

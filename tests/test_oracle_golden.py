@@ -5,6 +5,9 @@
 - ref_ex3_vortex180.json: ex3_vortex_accuracy_2d as shipped (180^2 periodic vortex,
   T = 0.01, sweeping limiter on) -> final rho/m1/m2/p bitwise (SHA-256) and the
   printed sweep statistics and error norms exactly.
+- ref_ex3_l1_vortex180.json: the same program built with its switch use_limiter = 1
+  (the Zhang-Shu flux limiter in wenofd instead of the sweeping post-process) ->
+  final fields bitwise, the printed limiter counts and error norms exactly.
 """
 import hashlib
 import json
@@ -64,3 +67,27 @@ def test_ex3_vortex_run_matches_reference_bitwise():
             eri = max(eri, d)
     er1 = er1 / 180 / 180
     assert er1 == g["L1"] and eri == g["Linf"]
+
+
+@pytest.mark.slow
+def test_ex3_vortex_flux_limiter_run_matches_reference_bitwise():
+    g = json.load(open(os.path.join(GOLD, "ref_ex3_l1_vortex180.json")))
+    cfg = O.vortex_config(180, use_limiter=1)
+    o = O.Oracle(cfg)
+    o.set_state(O.init_vortex(cfg))
+    o.bc()
+    rc, t, steps = o.run(0.0, 0.01)
+    assert rc == 0 and steps == 9
+    U = o.state()[3:-3, 3:-3, :]
+    rho, m1, m2, e = (U[:, :, c].reshape(-1) for c in range(4))
+    p = (1.4 - 1.0) * (e - 0.5 * m1 ** 2 / rho - 0.5 * m2 ** 2 / rho)
+    got = {"rho": rho, "m1": m1, "m2": m2, "p": p}
+    for k in got:
+        assert _digest(got[k]) == g["sha256"][k], k
+    s = o.stats()
+    assert (s["den_count"], s["pre_count"]) == (g["den_count"], g["pre_count"]) == (20, 416)
+    assert s["sweep_count"] == 0
+    ex = O.vortex_exact(cfg, t)[3:-3, 3:-3, 0]
+    d = np.abs(ex - U[:, :, 0])
+    er1 = float(np.cumsum(d.T.ravel())[-1]) / 180 / 180  # i outer, j inner as in `output`
+    assert er1 == g["L1"] and float(d.max()) == g["Linf"]
