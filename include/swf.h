@@ -97,6 +97,8 @@ typedef struct swf_ctx swf_ctx;
 int swf_create(const swf_config *cfg, swf_ctx **out);
 void swf_destroy(swf_ctx *ctx);
 
+/* The state's alpha (findalpha over every cell of the frame) is taken at upload
+ * and again by swf_bc; the first stage's dt follows from it. */
 int swf_upload(swf_ctx *ctx, const double *u);
 int swf_download(swf_ctx *ctx, double *u);
 

@@ -359,6 +359,10 @@ int swf_bc(swf_ctx *c) {
   int rc = set_device(c);
   if (rc) return rc;
   launch_bc(c->k, c->b, c->stream, -1, -1);
+  // findalpha of the next stage sees the new ghosts (a jet's inflow states among
+  // them): the state's alpha is taken again over the whole frame
+  launch_set_alpha(c->b, c->stream, 0, 0ull, -1, 0);
+  launch_alpha_all(c->k, c->b, c->stream, 0);
   HIPC(hipStreamSynchronize(c->stream));
   return check_launch("swf_bc");
 }

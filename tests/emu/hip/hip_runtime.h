@@ -1,0 +1,119 @@
+// CPU emulation of the HIP runtime subset that libswf uses (test infrastructure:
+// tests/emu/Makefile builds the library's own sources against this header with
+// g++).  Kernels run block by block on one OS thread; each thread of a block is
+// a fiber (ucontext) that runs until its next __syncthreads or its end, so the
+// threads of a block interleave deterministically.  __shfl_xor exchanges through
+// a buffer between two barriers (every thread of a block reaches the shuffles of
+// these kernels together); atomics are plain read-modify-writes; streams are
+// synchronous; a captured graph records the launches and replays them.
+// Device memory is host memory filled with 0xA5 bytes at allocation.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+#include <math.h>
+#include <functional>
+#include <vector>
+
+#define __host__
+#define __device__
+#define __global__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+  unsigned x, y, z;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+namespace emu {
+extern dim3 tIdx, bIdx, bDim, gDim;
+void barrier();
+int lin_tid();
+extern unsigned long long shfl_buf[1024];
+void launch(dim3 g, dim3 b, std::function<void()> body);
+}  // namespace emu
+#define threadIdx (emu::tIdx)
+#define blockIdx (emu::bIdx)
+#define blockDim (emu::bDim)
+#define gridDim (emu::gDim)
+
+inline void __syncthreads() { emu::barrier(); }
+
+template <class T>
+T __shfl_xor(T v, int off, int width = 64) {
+  static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+  const int t = emu::lin_tid();
+  unsigned long long w = 0;
+  memcpy(&w, &v, sizeof(T));
+  emu::shfl_buf[t] = w;
+  emu::barrier();
+  const int src = (t & ~(width - 1)) | ((t & (width - 1)) ^ off);
+  unsigned long long r = emu::shfl_buf[src];
+  emu::barrier();
+  T out;
+  memcpy(&out, &r, sizeof(T));
+  return out;
+}
+
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline long long min(long long a, long long b) { return a < b ? a : b; }
+inline long long max(long long a, long long b) { return a > b ? a : b; }
+inline int __ffs(unsigned x) { return __builtin_ffs((int)x); }
+inline int __popc(unsigned x) { return __builtin_popcount(x); }
+inline long long __double_as_longlong(double d) { long long r; memcpy(&r, &d, 8); return r; }
+inline double __longlong_as_double(long long l) { double r; memcpy(&r, &l, 8); return r; }
+
+// single OS thread: atomics are plain read-modify-writes
+template <class T> T atomicOr(T *p, T v) { T o = *p; *p = o | v; return o; }
+template <class T> T atomicAnd(T *p, T v) { T o = *p; *p = o & v; return o; }
+template <class T> T atomicAdd(T *p, T v) { T o = *p; *p = o + v; return o; }
+template <class T> T atomicMax(T *p, T v) { T o = *p; if (v > o) *p = v; return o; }
+
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
+typedef struct emu_stream *hipStream_t;
+typedef struct emu_graph *hipGraph_t;
+typedef struct emu_graph *hipGraphExec_t;
+typedef struct emu_event *hipEvent_t;
+typedef struct emu_node *hipGraphNode_t;
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
+enum { hipStreamNonBlocking = 1 };
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal };
+
+hipError_t hipSetDevice(int);
+hipError_t hipGetDeviceCount(int *);
+hipError_t hipStreamCreateWithFlags(hipStream_t *, unsigned);
+hipError_t hipStreamDestroy(hipStream_t);
+hipError_t hipStreamSynchronize(hipStream_t);
+hipError_t hipMalloc(void *, size_t);
+template <class T> hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void *)p, n); }
+hipError_t hipFree(void *);
+hipError_t hipMemcpy(void *, const void *, size_t, hipMemcpyKind);
+hipError_t hipMemcpyAsync(void *, const void *, size_t, hipMemcpyKind, hipStream_t = nullptr);
+hipError_t hipMemsetAsync(void *, int, size_t, hipStream_t = nullptr);
+hipError_t hipGetLastError();
+const char *hipGetErrorString(hipError_t);
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode);
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *);
+hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, hipGraphNode_t *, char *, size_t);
+hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t);
+hipError_t hipGraphExecDestroy(hipGraphExec_t);
+hipError_t hipGraphDestroy(hipGraph_t);
+hipError_t hipEventCreate(hipEvent_t *);
+hipError_t hipEventDestroy(hipEvent_t);
+hipError_t hipEventRecord(hipEvent_t, hipStream_t = nullptr);
+hipError_t hipEventSynchronize(hipEvent_t);
+hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
+
+namespace emu {
+void trace_launch(const void *fn, dim3 g, dim3 b);
+template <class K, class... A>
+void launch_k(K k, dim3 g, dim3 b, size_t, hipStream_t, A... a) {
+  trace_launch((const void *)k, g, b);
+  launch(g, b, [=]() { k(a...); });
+}
+}  // namespace emu
+#define hipLaunchKernelGGL(K, ...) emu::launch_k(K, __VA_ARGS__)

@@ -198,6 +198,64 @@ def test_jet_half_steps_exact_bitwise():
     assert s.stats()["total_sweep"] == o.stats()["total_sweep"]
 
 
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_bulk"])
+def test_first_dt_after_bc_sees_inflow_ghosts(prec):
+    """upload of the ambient jet state, then swf_bc (which writes the Mach 2000
+    inflow ghosts): the first step's dt comes from findalpha over the whole frame,
+    inflow included (alpha ~ 800), as in the reference, where initdata ends with
+    bc and rk's findalpha runs over every cell.  Taking alpha from the uploaded
+    frame alone (alpha ~ 1.2) gives dt = tfinal and a blow-up."""
+    cfg = S.jet_config(40, 20, prec, cfl=0.5, half=False)
+    o = O.Oracle(oracle_cfg(cfg))
+    o.set_state(S.init_jet(cfg))
+    o.bc()
+    rc, dto = o.step(0.0, 0.001)
+    assert rc == 0 and dto < 0.05 * 0.001  # 0.5 dx / 800 = 1.56e-5
+    s = S.Solver(cfg)
+    s.upload(S.init_jet(cfg))
+    s.bc()
+    dts = s.step(0.0, 0.001)
+    if prec == "fp64_exact":
+        assert dts == dto
+    else:
+        assert abs(dts - dto) <= 1e-12 * dto
+
+
+def test_jet_full_steps_exact_bitwise_and_fast_modes():
+    """Full-domain Mach 2000 jet (inflow for |y| < 0.05, zero-gradient at both y
+    faces), 200x100 at CFL 0.5 for 25 steps; the sweep triggers at every stage
+    from step 18.  Exact mode: fields and sweep statistics equal the oracle's.
+    FP64 / FP32-bulk: finite, and relative L1 density within the stated bounds."""
+    import time
+    cfg = S.jet_config(200, 100, "fp64_exact", cfl=0.5, half=False)
+    o, s, to, ts, no, ns = _run_both(cfg, S.init_jet(cfg), 0.001, 25)
+    assert ns == no == 25 and ts == to
+    assert_bitwise(s.download(), o.state(), "full jet 200x100")
+    so, sg = o.stats(), s.stats()
+    assert so["sweep_count"] >= 20
+    for key in ("sweep_count", "den_count", "pre_count", "total_sweep", "max_sweep", "total_neg", "max_neg",
+                "total_pos", "max_pos"):
+        assert sg[key] == so[key], key
+    ref = s.download()[3:-3, 3:-3, 0]
+    for prec, tol in (("fp64", 1e-8), ("fp32_bulk", 1e-4)):
+        c = S.jet_config(200, 100, prec, cfl=0.5, half=False)
+        f = S.Solver(c)
+        f.upload(S.init_jet(c))
+        f.bc()
+        t0 = time.perf_counter()
+        tf, nf = f.advance(0.0, 0.001, max_steps=25)
+        el = time.perf_counter() - t0
+        r = f.download()[3:-3, 3:-3, 0]
+        st = f.stats()
+        l1 = np.sum(np.abs(r - ref)) / np.sum(np.abs(ref))
+        print(f"full jet 200x100 {prec}: {nf} steps {el:.3f}s L1 {l1:.3e} sweeps {st['sweep_count']} "
+              f"q {st['total_sweep']} neg {st['total_neg']} pos {st['total_pos']} vs exact q {sg['total_sweep']} "
+              f"neg {sg['total_neg']} pos {sg['total_pos']}", flush=True)
+        assert nf == 25 and np.isfinite(r).all()
+        assert l1 < tol
+        f.close()
+
+
 def test_vortex180_exact_reproduces_reference_run():
     """The reference program ex3_vortex_accuracy_2d as shipped, reproduced bit for bit."""
     g = json.load(open(os.path.join(GOLD, "ref_ex3_vortex180.json")))
