@@ -165,9 +165,10 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&trk, sizeof(long long) * 3 * (size_t)kTrackCap);
   double *stage = nullptr;
   if (e == hipSuccess) e = hipMalloc(&stage, sizeof(double) * c->n_fortran);
-  // stage-tile alpha partials, then the flux limiter's (den, pre) count pairs
+  // stage-tile alpha partials, the tiles' weight*u sums (4 each), then the flux
+  // limiter's (den, pre) count pairs
   const size_t nblk = (size_t)ub_gx(k) * ub_gy(k);
-  const size_t part_bytes = sizeof(double) * nblk + sizeof(int) * 2 * (size_t)zs_pairs(k);
+  const size_t part_bytes = sizeof(double) * 5 * nblk + sizeof(int) * 2 * (size_t)zs_pairs(k);
   double *apart = nullptr;
   if (e == hipSuccess) e = hipMalloc(&apart, part_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(apart, 0, part_bytes, c->stream);
@@ -203,7 +204,8 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   c->b.g = g;
   c->b.trk = trk;
   c->b.apart = apart;
-  c->b.zs = reinterpret_cast<int *>(apart + nblk);
+  c->b.upart = apart + nblk;
+  c->b.zs = reinterpret_cast<int *>(apart + 5 * nblk);
   c->d_fortran = stage;
   *out = c;
   return SWF_OK;

@@ -50,6 +50,8 @@ struct DevScalars {
   double tfinal, dt;
   unsigned long long alpha_bits[4];  // alpha max: inputs of stage 1,2,3; [3] queries
   int postag, swept, status, rho_any;
+  int ubar_tiles;            // the stage kernel sums weight*u per workgroup (fast modes,
+                             // after a stage that swept)
   unsigned done_blocks;      // update workgroups finished in this stage
   long long steps_done;
   long long sweep_count, den_count, pre_count, stage_sweeps;
@@ -68,8 +70,9 @@ struct Bufs {
   DevScalars *g;
   long long *trk;                   // 3 * track_cap: q, neg, pos per sweep trigger
   double *apart;                    // alpha max of each update workgroup
+  double *upart;                    // 4 per update workgroup: sum of weight*u over its cells
   int *zs;                          // flux limiter: (den, pre) counts per workgroup of the
-                                    // x-flux, stage and edge kernels (zs_offsets)
+                                    // x-flux, stage and edge kernels (zs_pairs)
 };
 
 // Row layout of every plane: cell i (Fortran index, ghosts from -2) at column

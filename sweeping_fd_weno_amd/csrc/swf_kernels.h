@@ -50,6 +50,7 @@ struct YSmem {
   double pr[4][YT_PTS][YT_W];
   double G[4][YT_H][YT_W];  // net y-interface fluxes of the tile
   double red[4];
+  double ured[4][4];        // weight*u sums of the four waves
   int zred[8];
 };
 
@@ -360,9 +361,14 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
       j = (c < 3) ? 1 + c : k.ny - 5 + c;
     }
     if (i >= 1 && i <= k.nx && j >= 1 && j <= k.ny) {
-      double v[4];
-      load_ij(k, b, i, j, v);
-      write_ghosts(k, b, i, j, v);
+      // only cells of tiles the sweep changed have new images
+      const int blk = ub_of(k, i, j);
+      const bool dirty = sh.all_dirty || (blk < SW_DIRTY_WORDS * 32 && ((sh.dirty[blk >> 5] >> (blk & 31)) & 1u));
+      if (dirty) {
+        double v[4];
+        load_ij(k, b, i, j, v);
+        write_ghosts(k, b, i, j, v);
+      }
     }
   }
   __syncthreads();
@@ -382,6 +388,8 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
   DevScalars *g = b.g;
   if (!g->active[parity] || g->status != ST_OK) return;
   const bool rhs_only = flags & YS_RHS_ONLY, scalars = !(flags & YS_NO_SCALARS);
+  // after a stage that swept, the tile's sums of weight*u for the next sweep's u_bar
+  const bool usum = M != MODE_EXACT && scalars && !rhs_only && g->ubar_tiles;
   const int tid = threadIdx.x;
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
   const double ia = 1.0 / alpha;
@@ -421,6 +429,7 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
   __syncthreads();
   // cell j = jf: Y = G(j+1/2) - G(j-1/2), X = F(i+1/2) - F(i-1/2)
   double anew = 0.0;
+  double wu[4] = {0.0, 0.0, 0.0, 0.0};
   if (in) {
     const long long o = cidx(k, i, j);
     double rhs[4];
@@ -468,6 +477,7 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
         }
         v[c] = un;
         b.U[c * k.plane + o] = un;
+        wu[c] = k.weight * un;
       }
       write_ghosts(k, b, i, j, v);
       // checkpositivity (ex4_sedov_2d/main.f90:779-801)
@@ -488,6 +498,24 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
   if (rhs_only) return;
   const double am = block_max<256>(anew, sm.red, tid);
   if (tid == 0 && scalars) b.apart[blockIdx.x + blockIdx.y * gridDim.x] = am;
+  if (usum) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double v = wu[c];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+      wu[c] = v;
+    }
+    if ((tid & 63) == 0) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sm.ured[c][tid >> 6] = wu[c];
+    }
+    __syncthreads();
+    if (tid < 4) {
+      const int blk = blockIdx.x + blockIdx.y * gridDim.x;
+      b.upart[4 * blk + tid] = (sm.ured[tid][0] + sm.ured[tid][1]) + (sm.ured[tid][2] + sm.ured[tid][3]);
+    }
+  }
   if (LIM && scalars)
     zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (xf_gx(k) * xf_gy(k) + blockIdx.x + blockIdx.y * gridDim.x),
              sm.zred, tid);
@@ -587,7 +615,7 @@ __global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, i
   DevScalars *g = b.g;
   const int tid = threadIdx.x;
   const int nblk = ub_gx(k) * ub_gy(k);
-  const int act = g->active[parity], st = g->status, flagged = g->postag;
+  const int act = g->active[parity], st = g->status, flagged = g->postag, ubt = g->ubar_tiles;
   double a = 0.0;
   for (int q = tid; q < nblk; q += SW_T) a = fmax2(a, b.apart[q]);
   if (!act || st != ST_OK) return;
@@ -617,13 +645,17 @@ __global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, i
     }
   }
   if (k.use_limiter == 2 && flagged) {
-    sweep_body<SW_T>(k, b, serial_ubar != 0, sh, tid);
+    sweep_body<SW_T>(k, b, serial_ubar != 0, (ubt && !serial_ubar) ? b.upart : nullptr, sh, tid);
     after_sweep<SW_T>(k, b, sh, red, tid);
     a = 0.0;
     for (int q = tid; q < nblk; q += SW_T) a = fmax2(a, b.apart[q]);
   }
   a = block_max<SW_T>(a, red, tid);
-  if (tid == 0) g->alpha_bits[slot_out] = (unsigned long long)__double_as_longlong(fmax2(a, k.alpha_const));
+  if (tid == 0) {
+    g->alpha_bits[slot_out] = (unsigned long long)__double_as_longlong(fmax2(a, k.alpha_const));
+    // sweeps tend to follow sweeps: then the next stage kernel sums weight*u per tile
+    g->ubar_tiles = (k.use_limiter == 2 && flagged && !serial_ubar) ? 1 : 0;
+  }
 }
 
 // alpha max over every cell (ghosts included), for a freshly uploaded state

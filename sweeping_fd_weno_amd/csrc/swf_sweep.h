@@ -19,8 +19,10 @@
 //     convergence test (global min P and rho) reduces to "no cell left flagged".
 // Statistics (sweep_count, den_count, pre_count, per-trigger q / negative /
 // positive adjustment counts) are accumulated exactly as the reference does.
-// One workgroup of NT threads runs the whole sweep (all q iterations): either
-// its own single-workgroup kernel, or the last workgroup of the RK update kernel.
+// One workgroup of NT threads runs the whole sweep (all q iterations): its own
+// single-workgroup kernel (swf_sweep) or the stage-end kernel.  A pass first
+// compacts the set bits of its bitmap into an ordered list in LDS (one block
+// scan), so its cost follows the number of flagged cells, not the grid size.
 #pragma once
 
 #include "swf_device.h"
@@ -107,23 +109,44 @@ SWF_D bool nod_lim(double g1, double eps, const double v1[4], const double v2[4]
 }
 
 // Pressure-sweep walk from position f in `order`.  FWD: neighbour f+1, else f-1.
-// WRITE == false: speculative, returns the stop position only.
+// WRITE == false: speculative, returns the stop position only.  The cells of the
+// path are loaded PF positions ahead (clamped at the last position), so a long
+// walk waits on its arithmetic, not on one memory round trip per cell; the
+// values used are those of cells the walk reaches, which no other walk writes.
 template <bool FWD, bool WRITE, int NT>
 SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], unsigned *next_map,
                 long long *npos, long long *nneg, SweepShared<NT> *sh) {
 #pragma clang fp contract(off)
+  constexpr int PF = 3;
   const int N = k.nx * k.ny;
+  const int last = FWD ? N - 1 : 0;
   const double g1 = k.gamma1, eps = k.eps, wr = k.weight / k.weight;
   int p = f, i, j;
   cell_of(k, order, p, i, j);
   double v[4];
   load_ij(k, b, i, j, v);
+  double av[PF][4];  // cells p+1 .. p+PF (p-1 .. p-PF backward)
+#pragma unroll
+  for (int d = 0; d < PF; ++d) {
+    int ci, cj;
+    cell_of(k, order, FWD ? min(p + 1 + d, last) : max(p - 1 - d, last), ci, cj);
+    load_ij(k, b, ci, cj, av[d]);
+  }
   while (true) {
     const int nb = FWD ? p + 1 : p - 1;
-    int ni, nj;
-    cell_of(k, order, nb, ni, nj);
     double vn[4], vt[4];
-    load_ij(k, b, ni, nj, vn);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) vn[c] = av[0][c];
+#pragma unroll
+    for (int d = 0; d + 1 < PF; ++d) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) av[d][c] = av[d + 1][c];
+    }
+    {
+      int ci, cj;
+      cell_of(k, order, FWD ? min(nb + PF, last) : max(nb - PF, last), ci, cj);
+      load_ij(k, b, ci, cj, av[PF - 1]);
+    }
     const bool pos = nod_lim(g1, eps, v, vn, ub, vt);
     double vn2[4];
 #pragma unroll
@@ -138,8 +161,7 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
         ++*nneg;
     }
     p = nb;
-    i = ni;
-    j = nj;
+    cell_of(k, order, p, i, j);
 #pragma unroll
     for (int c = 0; c < 4; ++c) v[c] = vn2[c];
     const bool edge = FWD ? (p == N - 1) : (p == 0);
@@ -227,49 +249,124 @@ SWF_D void collect_chunk(const KCfg &k, const unsigned *map, int w0, SweepShared
   __syncthreads();
 }
 
+// Every set position of `map`, ascending, into sh.list when there are at most
+// SW_WCH*32 of them; returns their number, or -1 (sh.list not filled) when there
+// are more.  Thread t scans the contiguous words [t*wpt, (t+1)*wpt), so thread
+// order is position order, and one block-wide exclusive scan of the per-thread
+// counts gives each thread its place in the list.
+template <int NT>
+SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, int tid) {
+  static_assert(NT / 64 <= SW_WCH, "wave totals live in sh.wscan");
+  constexpr int BW = 16;  // words loaded together
+  const int wpt = (k.nwords + NT - 1) / NT;
+  const int w0 = tid * wpt, w1 = min(w0 + wpt, k.nwords);
+  unsigned v[BW];
+  int cnt = 0;
+  for (int w = w0; w < w1; w += BW) {
+#pragma unroll
+    for (int q = 0; q < BW; ++q) v[q] = (w + q < w1) ? map[w + q] : 0u;
+#pragma unroll
+    for (int q = 0; q < BW; ++q) cnt += __popc(v[q]);
+  }
+  int incl = cnt;  // inclusive scan within the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if ((tid & 63) >= off) incl += v;
+  }
+  if ((tid & 63) == 63) sh.wscan[tid >> 6] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int q = 0; q < NT / 64; ++q) {
+      const int t = sh.wscan[q];
+      sh.wscan[q] = run;
+      run += t;
+    }
+    sh.nlist = run;
+  }
+  __syncthreads();
+  const int total = sh.nlist;
+  if (total > SW_WCH * 32) return -1;
+  if (cnt) {
+    int o = sh.wscan[tid >> 6] + incl - cnt;
+    for (int w = w0; w < w1; w += BW) {
+      if (wpt > BW) {  // more than one batch: load this one again
+#pragma unroll
+        for (int q = 0; q < BW; ++q) v[q] = (w + q < w1) ? map[w + q] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < BW; ++q) {
+        unsigned word = v[q];
+        while (word) {
+          const int bit = __ffs(word) - 1;
+          word &= word - 1;
+          sh.list[o++] = (w + q) * 32 + bit;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  return total;
+}
+
+// Walks of one pass from the n positions in sh.list (ascending; a backward pass
+// takes them from the end), NT candidates at a time.
+template <bool FWD, int NT>
+SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsigned *next, const double ub[4],
+                          SweepShared<NT> &sh, int tid) {
+  const int N = k.nx * k.ny;
+  for (int s0 = 0; s0 < n; s0 += NT) {
+    const int idx = s0 + tid;
+    int f = -1;
+    if (idx < n) f = FWD ? sh.list[idx] : sh.list[n - 1 - idx];
+    const int le = sh.last_end;
+    const bool cand = (f >= 0) && (FWD ? (f > le && f <= N - 2) : (f < le && f >= 1));
+    sh.ends[tid] = cand ? pwalk<FWD, false, NT>(k, b, order, f, ub, nullptr, nullptr, nullptr, nullptr) : -1;
+    sh.genuine[tid] = 0;
+    __syncthreads();
+    if (tid == 0) {
+      int last = sh.last_end;
+      const int m = min(NT, n - s0);
+      for (int q = 0; q < m; ++q) {
+        const int fq = FWD ? sh.list[s0 + q] : sh.list[n - 1 - (s0 + q)];
+        const bool ok = FWD ? (fq > last && fq <= N - 2) : (fq < last && fq >= 1);
+        if (ok) {
+          sh.genuine[q] = 1;
+          last = sh.ends[q];
+        }
+      }
+      sh.last_end = last;
+    }
+    __syncthreads();
+    if (sh.genuine[tid]) {
+      long long np = 0, nn = 0;
+      pwalk<FWD, true, NT>(k, b, order, f, ub, next, &np, &nn, &sh);
+      atomicAdd((unsigned long long *)&sh.tpos, (unsigned long long)np);
+      atomicAdd((unsigned long long *)&sh.tneg, (unsigned long long)nn);
+    }
+    __syncthreads();
+  }
+}
+
 // One directional pass of the pressure sweep over the flagged positions in
 // `map` (order `order`).  Adjusted-but-still-flagged cells are recorded in `next`.
+// More flagged positions than the list holds: chunk by chunk of SW_WCH words.
 template <bool FWD, int NT>
 SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned *map, unsigned *next,
                          const double ub[4], SweepShared<NT> &sh, int tid) {
   const int N = k.nx * k.ny;
   if (tid == 0) sh.last_end = FWD ? -1 : N;
   __syncthreads();
-  const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
-  for (int ci = 0; ci < nchunks; ++ci) {
-    const int c = FWD ? ci : (nchunks - 1 - ci);
-    collect_chunk<NT>(k, map, c * SW_WCH, sh, tid);
-    const int n = sh.nlist;
-    for (int s0 = 0; s0 < n; s0 += NT) {
-      const int idx = s0 + tid;
-      int f = -1;
-      if (idx < n) f = FWD ? sh.list[idx] : sh.list[n - 1 - idx];
-      const int le = sh.last_end;
-      const bool cand = (f >= 0) && (FWD ? (f > le && f <= N - 2) : (f < le && f >= 1));
-      sh.ends[tid] = cand ? pwalk<FWD, false, NT>(k, b, order, f, ub, nullptr, nullptr, nullptr, nullptr) : -1;
-      sh.genuine[tid] = 0;
-      __syncthreads();
-      if (tid == 0) {
-        int last = sh.last_end;
-        const int m = min(NT, n - s0);
-        for (int q = 0; q < m; ++q) {
-          const int fq = FWD ? sh.list[s0 + q] : sh.list[n - 1 - (s0 + q)];
-          const bool ok = FWD ? (fq > last && fq <= N - 2) : (fq < last && fq >= 1);
-          if (ok) {
-            sh.genuine[q] = 1;
-            last = sh.ends[q];
-          }
-        }
-        sh.last_end = last;
-      }
-      __syncthreads();
-      if (sh.genuine[tid]) {
-        long long np = 0, nn = 0;
-        pwalk<FWD, true, NT>(k, b, order, f, ub, next, &np, &nn, &sh);
-        atomicAdd((unsigned long long *)&sh.tpos, (unsigned long long)np);
-        atomicAdd((unsigned long long *)&sh.tneg, (unsigned long long)nn);
-      }
-      __syncthreads();
+  const int n = collect_all<NT>(k, map, sh, tid);
+  if (n >= 0) {
+    pressure_walks<FWD, NT>(k, b, order, n, next, ub, sh, tid);
+  } else {
+    const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
+    for (int ci = 0; ci < nchunks; ++ci) {
+      const int c = FWD ? ci : (nchunks - 1 - ci);
+      collect_chunk<NT>(k, map, c * SW_WCH, sh, tid);
+      pressure_walks<FWD, NT>(k, b, order, sh.nlist, next, ub, sh, tid);
     }
   }
   // the position the pass never examines (N-1 forward, 0 backward)
@@ -284,6 +381,49 @@ SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned
   __syncthreads();
 }
 
+// Forward walks of the density sweep from the n positions in sh.list.
+template <int NT>
+SWF_D void density_walks(const KCfg &k, const Bufs &b, int n, SweepShared<NT> &sh, int tid) {
+  const int N = k.nx * k.ny;
+  for (int s0 = 0; s0 < n; s0 += NT) {
+    const int idx = s0 + tid;
+    const int f = (idx < n) ? sh.list[idx] : -1;
+    const int le = sh.last_end;
+    const bool cand = (f >= 0) && f > le && f <= N - 2;
+    sh.ends[tid] = cand ? swalk<false, NT>(k, b, f, +1, nullptr) : -1;
+    sh.genuine[tid] = 0;
+    __syncthreads();
+    if (tid == 0) {
+      int last = sh.last_end;
+      const int m = min(NT, n - s0);
+      for (int q = 0; q < m; ++q) {
+        const int fq = sh.list[s0 + q];
+        if (fq > last && fq <= N - 2) {
+          sh.genuine[q] = 1;
+          last = sh.ends[q];
+        }
+      }
+      sh.last_end = last;
+    }
+    __syncthreads();
+    if (sh.genuine[tid]) swalk<true, NT>(k, b, f, +1, &sh);
+    __syncthreads();
+  }
+}
+
+// Convergence test of a cell left flagged after a backward pass (:1714-1744):
+// flagged means P <= eps or rho <= eps, so it fails, except under ex5's test
+// (P >= eps and rho >= eps, :1162), which a cell at exactly eps passes.
+SWF_D bool converged_at(const KCfg &k, const Bufs &b, int order, int p) {
+  if (!k.success_ge) return false;
+  int i, j;
+  cell_of(k, order, p, i, j);
+  double v[4];
+  load_ij(k, b, i, j, v);
+  const double pr = sweep_pres(k.gamma1, v[0], v[1], v[2], v[3]);
+  return !((pr < k.eps) || (v[0] < k.eps));
+}
+
 template <int NT>
 SWF_D void clear_map(const KCfg &k, unsigned *m, int tid) {
   for (int w = tid; w < k.nwords; w += NT) m[w] = 0u;
@@ -292,9 +432,11 @@ SWF_D void clear_map(const KCfg &k, unsigned *m, int tid) {
 
 // u_bar = sum of weights*u over the interior.  Exact mode: the reference's serial
 // order (j outer, i inner, ex4_sedov_2d/main.f90:1431-1438), one lane per
-// component.  Otherwise a fixed-order parallel sum.
+// component.  Otherwise a fixed-order parallel sum: of the stage kernel's
+// per-tile sums when upart is given, else over the cells.
 template <int NT>
-SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared<NT> &sh, int tid) {
+SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, const double *upart, SweepShared<NT> &sh,
+                        int tid) {
 #pragma clang fp contract(off)
   const long long N = (long long)k.nx * k.ny;
   const double w = k.weight;
@@ -311,18 +453,34 @@ SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared<N
       __syncthreads();
       if (tid < 4) {
         const int m = (int)min((long long)NT, N - c0);
-        for (int q = 0; q < m; ++q) acc = acc + sh.prod[tid][q];
+        int q = 0;
+        for (; q + 8 <= m; q += 8) {  // eight LDS reads in flight, the sums in order
+          double x[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) x[e] = sh.prod[tid][q + e];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) acc = acc + x[e];
+        }
+        for (; q < m; ++q) acc = acc + sh.prod[tid][q];
       }
       __syncthreads();
     }
     if (tid < 4) sh.ubar[tid] = acc;
   } else {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (long long t = tid; t < N; t += NT) {
-      const int i = 1 + (int)(t % k.nx), j = 1 + (int)(t / k.nx);
-      const long long o = cidx(k, i, j);
+    if (upart) {
+      const int nblk = ub_gx(k) * ub_gy(k);
+      for (int q = tid; q < nblk; q += NT) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] = acc[c] + w * b.U[c * k.plane + o];
+        for (int c = 0; c < 4; ++c) acc[c] = acc[c] + upart[4 * q + c];
+      }
+    } else {
+      for (long long t = tid; t < N; t += NT) {
+        const int i = 1 + (int)(t % k.nx), j = 1 + (int)(t / k.nx);
+        const long long o = cidx(k, i, j);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] = acc[c] + w * b.U[c * k.plane + o];
+      }
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -342,8 +500,10 @@ SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, SweepShared<N
 
 // The sweep of one stage (sweep_lim incl. scalar_sweep), run by one workgroup of
 // NT threads.  Marks the update blocks whose cells it changed in sh.dirty.
+// upart: the stage kernel's per-tile sums of weight*u of the state, or null.
 template <int NT>
-SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, SweepShared<NT> &sh, int tid) {
+SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const double *upart, SweepShared<NT> &sh,
+                      int tid) {
   DevScalars *g = b.g;
   const int N = k.nx * k.ny;
   for (int w = tid; w < SW_DIRTY_WORDS; w += NT) sh.dirty[w] = 0u;
@@ -358,33 +518,14 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, SweepShare
   if (g->rho_any) {
     if (tid == 0) sh.last_end = -1;
     __syncthreads();
-    const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
-    for (int c = 0; c < nchunks; ++c) {
-      collect_chunk<NT>(k, b.bmR, c * SW_WCH, sh, tid);
-      const int n = sh.nlist;
-      for (int s0 = 0; s0 < n; s0 += NT) {
-        const int idx = s0 + tid;
-        const int f = (idx < n) ? sh.list[idx] : -1;
-        const int le = sh.last_end;
-        const bool cand = (f >= 0) && f > le && f <= N - 2;
-        sh.ends[tid] = cand ? swalk<false, NT>(k, b, f, +1, nullptr) : -1;
-        sh.genuine[tid] = 0;
-        __syncthreads();
-        if (tid == 0) {
-          int last = sh.last_end;
-          const int m = min(NT, n - s0);
-          for (int q = 0; q < m; ++q) {
-            const int fq = sh.list[s0 + q];
-            if (fq > last && fq <= N - 2) {
-              sh.genuine[q] = 1;
-              last = sh.ends[q];
-            }
-          }
-          sh.last_end = last;
-        }
-        __syncthreads();
-        if (sh.genuine[tid]) swalk<true, NT>(k, b, f, +1, &sh);
-        __syncthreads();
+    const int n = collect_all<NT>(k, b.bmR, sh, tid);
+    if (n >= 0) {
+      density_walks<NT>(k, b, n, sh, tid);
+    } else {
+      const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
+      for (int c = 0; c < nchunks; ++c) {
+        collect_chunk<NT>(k, b.bmR, c * SW_WCH, sh, tid);
+        density_walks<NT>(k, b, sh.nlist, sh, tid);
       }
     }
     // backward pass: after the forward pass only position N-1 can be below delta
@@ -397,7 +538,8 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, SweepShare
     __syncthreads();
   }
   // ---- u_bar and the P(u_bar) > 0 precondition
-  compute_ubar<NT>(k, b, serial_ubar, sh, tid);
+  // the density sweep changes cells after the stage kernel summed them
+  compute_ubar<NT>(k, b, serial_ubar, g->rho_any ? nullptr : upart, sh, tid);
   double ub[4] = {sh.ubar[0], sh.ubar[1], sh.ubar[2], sh.ubar[3]};
   if (k.pbar_check) {
     const double pb = k.gamma1 * (ub[3] - (0.5 * (ub[1] * ub[1] + ub[2] * ub[2])) / ub[0]);
@@ -431,21 +573,19 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, SweepShare
       g->pre_count++;
     }
     __syncthreads();
-    for (int w = tid; w < k.nwords; w += NT) {
-      unsigned word = A[w];
-      while (word) {
-        const int bit = __ffs(word) - 1;
-        word &= word - 1;
-        bool fail = true;
-        if (k.success_ge) {
-          int i, j;
-          cell_of(k, order, w * 32 + bit, i, j);
-          double v[4];
-          load_ij(k, b, i, j, v);
-          const double p = sweep_pres(k.gamma1, v[0], v[1], v[2], v[3]);
-          fail = (p < k.eps) || (v[0] < k.eps);
+    // the flagged positions once, as a list when they fit
+    const int nf = collect_all<NT>(k, A, sh, tid);
+    if (nf >= 0) {
+      for (int x = tid; x < nf; x += NT)
+        if (!converged_at(k, b, order, sh.list[x])) atomicOr(&sh.flag, 1);
+    } else {
+      for (int w = tid; w < k.nwords; w += NT) {
+        unsigned word = A[w];
+        while (word) {
+          const int bit = __ffs(word) - 1;
+          word &= word - 1;
+          if (!converged_at(k, b, order, w * 32 + bit)) atomicOr(&sh.flag, 1);
         }
-        if (fail) atomicOr(&sh.flag, 1);
       }
     }
     __syncthreads();
@@ -456,14 +596,22 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, SweepShare
     if (q == k.swp_num) break;
     // flagged set of the next pass, in the other snake order
     clear_map<NT>(k, B, tid);
-    for (int w = tid; w < k.nwords; w += NT) {
-      unsigned word = A[w];
-      while (word) {
-        const int bit = __ffs(word) - 1;
-        word &= word - 1;
+    if (nf >= 0) {
+      for (int x = tid; x < nf; x += NT) {
         int i, j;
-        cell_of(k, order, w * 32 + bit, i, j);
+        cell_of(k, order, sh.list[x], i, j);
         set_bit(B, pos_of(k, 1 - order, i, j));
+      }
+    } else {
+      for (int w = tid; w < k.nwords; w += NT) {
+        unsigned word = A[w];
+        while (word) {
+          const int bit = __ffs(word) - 1;
+          word &= word - 1;
+          int i, j;
+          cell_of(k, order, w * 32 + bit, i, j);
+          set_bit(B, pos_of(k, 1 - order, i, j));
+        }
       }
     }
     __syncthreads();
@@ -497,7 +645,7 @@ constexpr int SW_T = 1024;
 // Stand-alone sweep of the current state (the sweep_lim operator on its own).
 __global__ __launch_bounds__(SW_T) void k_sweep(KCfg k, Bufs b, int serial_ubar) {
   __shared__ SweepShared<SW_T> sh;
-  if (k.use_limiter == 2) sweep_body<SW_T>(k, b, serial_ubar != 0, sh, threadIdx.x);
+  if (k.use_limiter == 2) sweep_body<SW_T>(k, b, serial_ubar != 0, nullptr, sh, threadIdx.x);
 }
 #endif
 

@@ -61,6 +61,23 @@ inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline long long min(long long a, long long b) { return a < b ? a : b; }
 inline long long max(long long a, long long b) { return a > b ? a : b; }
+template <class T>
+T __shfl_up(T v, unsigned d, int width = 64) {
+  static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+  const int t = emu::lin_tid();
+  unsigned long long w = 0;
+  memcpy(&w, &v, sizeof(T));
+  emu::shfl_buf[t] = w;
+  emu::barrier();
+  const int lane = t & (width - 1);
+  const int src = lane >= (int)d ? t - (int)d : t;
+  unsigned long long r = emu::shfl_buf[src];
+  emu::barrier();
+  T out;
+  memcpy(&out, &r, sizeof(T));
+  return out;
+}
+
 inline int __ffs(unsigned x) { return __builtin_ffs((int)x); }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline long long __double_as_longlong(double d) { long long r; memcpy(&r, &d, 8); return r; }

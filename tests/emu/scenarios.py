@@ -45,6 +45,23 @@ def run_both(cfg, u, tfinal, nsteps):
             "stats": {k: [so[k], se[k]] for k in STATS}}
 
 
+def fast_vs_oracle(cfg, u, tfinal, nsteps):
+    """A fast mode against the oracle's exact run: relative L1 density difference
+    and both runs' sweep statistics."""
+    o = O.Oracle(ocfg(cfg))
+    o.set_state(u)
+    o.bc()
+    u = o.state()
+    rc, to, no = o.run(0.0, tfinal, max_steps=nsteps)
+    s = S.Solver(cfg)
+    s.upload(u)
+    ts, ns = s.advance(0.0, tfinal, max_steps=nsteps)
+    a, b = s.download()[3:-3, 3:-3, 0], o.state()[3:-3, 3:-3, 0]
+    so, se = o.stats(), s.stats()
+    return {"rc": rc, "steps": [no, ns], "l1": float(np.abs(a - b).sum() / np.abs(b).sum()),
+            "stats": {k: [so[k], se[k]] for k in STATS}}
+
+
 def first_dt(cfg):
     """dt of the first step after upload of the initial state and swf_bc."""
     o = O.Oracle(ocfg(cfg))
@@ -57,6 +74,42 @@ def first_dt(cfg):
     return {"rc": rc, "dt": [dto, s.step(0.0, 0.001)]}
 
 
+def crafted_sweep(n, nbad, seed):
+    """swf_sweep on a state with nbad cells of slightly negative pressure (runs of
+    consecutive flagged cells included) and three near-vacuum cells, against the
+    oracle's sweep: fields bitwise and statistics equal."""
+    cfg = S.sedov_config(n, "fp64_exact")
+    rng = np.random.default_rng(seed)
+    u = np.zeros(S.state_shape(cfg))
+    u[:, :, 0] = 1.0 + 0.1 * rng.random(u.shape[:2])
+    u[:, :, 1] = 0.05 * rng.standard_normal(u.shape[:2])
+    u[:, :, 2] = 0.05 * rng.standard_normal(u.shape[:2])
+    u[:, :, 3] = 2.5 + rng.random(u.shape[:2])
+    placed = 0
+    while placed < nbad:
+        i, j = int(rng.integers(1, n + 1)), int(rng.integers(1, n + 1))
+        for q in range(int(rng.integers(1, 5))):
+            ii = min(i + q, n)
+            u[j + 2, ii + 2, 3] = 0.5 * (u[j + 2, ii + 2, 1] ** 2 + u[j + 2, ii + 2, 2] ** 2) / u[j + 2, ii + 2, 0] \
+                - 1e-3 * rng.random()
+            placed += 1
+    for _ in range(3):
+        i, j = int(rng.integers(1, n + 1)), int(rng.integers(1, n + 1))
+        u[j + 2, i + 2, 0] = 1e-14
+        u[j + 2, i + 2, 1:3] = 0.0
+        u[j + 2, i + 2, 3] = 1e-15
+    o = O.Oracle(ocfg(cfg))
+    o.set_state(u)
+    rc = o.sweep()
+    s = S.Solver(cfg)
+    s.upload(u)
+    s.sweep()
+    a, b = s.download()[3:-3, 3:-3], o.state()[3:-3, 3:-3]
+    so, se = o.stats(), s.stats()
+    return {"rc": rc, "steps": [0, 0], "time": [0.0, 0.0], "differ": int((a.view(np.int64) != b.view(np.int64)).sum()),
+            "stats": {k: [so[k], se[k]] for k in STATS}}
+
+
 def main(name):
     if name == "sedov_sweeps":
         cfg = S.sedov_config(32, "fp64_exact")
@@ -64,12 +117,18 @@ def main(name):
     elif name == "jet_full_sweeps":
         cfg = S.jet_config(64, 32, "fp64_exact", cfl=0.5, half=False)
         out = run_both(cfg, S.init_jet(cfg), 0.001, 30)
+    elif name.startswith("jet_full_sweeps_"):
+        cfg = S.jet_config(64, 32, name[len("jet_full_sweeps_"):], cfl=0.5, half=False)
+        out = fast_vs_oracle(cfg, S.init_jet(cfg), 0.001, 25)
     elif name == "jet_half":
         cfg = S.jet_config(80, 20, "fp64_exact", cfl=0.25, half=True)
         out = run_both(cfg, S.init_jet(cfg), 0.001, 40)
     elif name == "vortex_flux_limiter":
         cfg = S.vortex_config(90, "fp64_exact", use_limiter=1)
         out = run_both(cfg, S.init_vortex(cfg), 0.01, 100)
+    elif name.startswith("crafted_sweep_"):
+        n, nbad = (int(v) for v in name[len("crafted_sweep_"):].split("_"))
+        out = crafted_sweep(n, nbad, 7)
     elif name.startswith("first_dt_"):
         out = first_dt(S.jet_config(40, 20, name[len("first_dt_"):], cfl=0.5, half=False))
     else:

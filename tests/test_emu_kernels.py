@@ -10,6 +10,10 @@ arithmetic and control flow, not races; the GPU tests check the real device.
 - Full-domain jet 64x32 at CFL 0.5, 30 steps, 40 sweeps (q up to 2): bitwise.
 - Half-domain jet 80x20 at CFL 0.25, 40 steps: bitwise.
 - Vortex 90^2 with the Zhang-Shu flux limiter: bitwise, limiter counts equal.
+- swf_sweep on crafted states with 14, 3000 and 12000 cells of negative
+  pressure (the last more than one ordered list of flagged positions holds, so
+  its passes go chunk by chunk): fields bitwise, statistics equal.
+- FP64 / FP32-bulk full jet 64x32 with sweeps (u_bar from per-tile sums).
 - After upload of the ambient jet and swf_bc, the first dt sees the Mach 2000
   inflow ghosts (alpha ~ 800) in every precision mode.
 """
@@ -39,7 +43,8 @@ def scenario(name):
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
-@pytest.mark.parametrize("name", ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter"])
+@pytest.mark.parametrize("name", ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter",
+                                  "crafted_sweep_24_14", "crafted_sweep_128_3000", "crafted_sweep_160_12000"])
 def test_emulated_kernels_match_oracle(emu_lib, name):
     r = scenario(name)
     assert r["rc"] == 0
@@ -51,6 +56,21 @@ def test_emulated_kernels_match_oracle(emu_lib, name):
         assert r["stats"]["sweep_count"][0] >= 30
     if name == "vortex_flux_limiter":
         assert r["stats"]["pre_count"][0] > 0
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-7), ("fp32_bulk", 1e-4)])
+def test_emulated_fast_modes_jet_sweeps(emu_lib, prec, tol):
+    """FP64 / FP32-bulk full jet 64x32, 25 steps with sweeps at every stage from
+    step 18 (u_bar from the stage kernel's per-tile sums): the same number of
+    sweep passes as the oracle's exact run, and relative L1 density within the
+    stated bound.  The coarse jet amplifies rounding quickly (FP32-bulk: 1.3e-6
+    at 25 steps, 2e-3 at 30), so the bounds are loose; the GPU tests hold the
+    200x100 jet to 1e-8 / 1e-4."""
+    r = scenario("jet_full_sweeps_" + prec)
+    assert r["rc"] == 0 and r["steps"] == [25, 25]
+    assert r["stats"]["sweep_count"][0] == r["stats"]["sweep_count"][1] >= 20
+    assert r["stats"]["total_sweep"][0] == r["stats"]["total_sweep"][1]
+    assert r["l1"] < tol
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_bulk"])

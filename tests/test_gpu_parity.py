@@ -145,6 +145,39 @@ def _crafted_sweep_state(cfg, seed):
     return u
 
 
+@pytest.mark.parametrize("n,nbad", [(128, 3000), (160, 12000)])
+def test_sweep_many_flagged_matches_oracle(n, nbad):
+    """Thousands of flagged cells; at 12000 a pass holds more flagged positions
+    than its ordered list and goes chunk by chunk."""
+    cfg = S.sedov_config(n, "fp64_exact")
+    rng = np.random.default_rng(7)
+    u = np.zeros(S.state_shape(cfg))
+    u[:, :, 0] = 1.0 + 0.1 * rng.random(u.shape[:2])
+    u[:, :, 1] = 0.05 * rng.standard_normal(u.shape[:2])
+    u[:, :, 2] = 0.05 * rng.standard_normal(u.shape[:2])
+    u[:, :, 3] = 2.5 + rng.random(u.shape[:2])
+    placed = 0
+    while placed < nbad:
+        i, j = int(rng.integers(1, n + 1)), int(rng.integers(1, n + 1))
+        for q in range(int(rng.integers(1, 5))):
+            ii = min(i + q, n)
+            u[j + 2, ii + 2, 3] = 0.5 * (u[j + 2, ii + 2, 1] ** 2 + u[j + 2, ii + 2, 2] ** 2) / u[j + 2, ii + 2, 0] \
+                - 1e-3 * rng.random()
+            placed += 1
+    o = O.Oracle(oracle_cfg(cfg))
+    o.set_state(u)
+    assert o.sweep() == 0
+    s = S.Solver(cfg)
+    s.upload(u)
+    s.sweep()
+    assert_bitwise(s.download()[3:-3, 3:-3], o.state()[3:-3, 3:-3], "swept field")
+    so, sg = o.stats(), s.stats()
+    assert so["total_neg"] > nbad // 2
+    for key in ("sweep_count", "den_count", "pre_count", "total_sweep", "max_sweep", "total_neg", "max_neg",
+                "total_pos", "max_pos"):
+        assert sg[key] == so[key], key
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_sweep_matches_oracle(seed):
     cfg = S.sedov_config(24, "fp64_exact")
