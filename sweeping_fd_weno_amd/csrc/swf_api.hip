@@ -172,6 +172,10 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   double *apart = nullptr;
   if (e == hipSuccess) e = hipMalloc(&apart, part_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(apart, 0, part_bytes, c->stream);
+  // the sweep's walk records (use_limiter = 2)
+  double *wrec = nullptr;
+  const size_t rec_bytes = sizeof(double) * 4 * SW_REC * SW_T + sizeof(unsigned) * (SW_REC / 32) * SW_T;
+  if (e == hipSuccess && k.use_limiter == 2) e = hipMalloc(&wrec, rec_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(pool, 0, 5 * pbytes, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(maps, 0, sizeof(unsigned) * 4 * (size_t)k.nwords, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(g, 0, sizeof(DevScalars), c->stream);
@@ -185,6 +189,7 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
     if (trk) (void)hipFree(trk);
     if (stage) (void)hipFree(stage);
     if (apart) (void)hipFree(apart);
+    if (wrec) (void)hipFree(wrec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return rc;
@@ -205,6 +210,8 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   c->b.trk = trk;
   c->b.apart = apart;
   c->b.upart = apart + nblk;
+  c->b.wrec = wrec;
+  c->b.wpos = wrec ? reinterpret_cast<unsigned *>(wrec + 4 * SW_REC * SW_T) : nullptr;
   c->b.zs = reinterpret_cast<int *>(apart + 5 * nblk);
   c->d_fortran = stage;
   *out = c;
@@ -221,6 +228,7 @@ void swf_destroy(swf_ctx *c) {
   (void)hipFree(c->b.trk);
   (void)hipFree(c->d_fortran);
   (void)hipFree(c->b.apart);
+  if (c->b.wrec) (void)hipFree(c->b.wrec);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   (void)hipStreamDestroy(c->stream);
   delete c;

@@ -60,6 +60,11 @@ struct DevScalars {
   double pbar;               // P(u_bar) of the last pbar failure
 };
 
+// The sweep's workgroup size, and the steps of a speculative walk it records per
+// thread (longer walks are computed again when they write; 64 MB of records).
+constexpr int SW_T = 512;
+constexpr int SW_REC = 4096;
+
 // Device buffers of one solver context.  A step cycles three state buffers
 // S0 -> S1 -> S2 -> S0: stage s reads its stencils from Uin and writes U; the
 // RK combinations of stages 2 and 3 read the step's initial state U0 (= S0)
@@ -71,6 +76,8 @@ struct Bufs {
   long long *trk;                   // 3 * track_cap: q, neg, pos per sweep trigger
   double *apart;                    // alpha max of each update workgroup
   double *upart;                    // 4 per update workgroup: sum of weight*u over its cells
+  double *wrec;                     // sweep walk records: SW_REC rows x 4 x SW_T slots
+  unsigned *wpos;                   // their branch bits: SW_REC/32 words x SW_T slots
   int *zs;                          // flux limiter: (den, pre) counts per workgroup of the
                                     // x-flux, stage and edge kernels (zs_pairs)
 };

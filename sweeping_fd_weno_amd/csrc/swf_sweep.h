@@ -33,12 +33,47 @@ constexpr int SW_WCH = 256;          // bitmap words per scan chunk (8192 positi
 constexpr int SW_DIRTY_WORDS = 256;  // dirty-tile bitmap: up to 8192 update tiles
 
 SWF_D void cell_of(const KCfg &k, int order, int p, int &i, int &j) {
-  if (order == 0)
-    cellI(k, p, i, j);
-  else
-    cellII(k, p, i, j);
+  // cellI / cellII as one computation with selected operands (no address of i or
+  // j chosen at run time, which would put them in scratch memory)
+  const int n = order == 0 ? k.nx : k.ny;  // length of a snake line
+  const int a = p / n + 1;                 // line
+  const int r = p - (a - 1) * n;
+  const int c = (a & 1) ? r + 1 : n - r;   // cell within the line
+  i = order == 0 ? c : a;
+  j = order == 0 ? a : c;
 }
 SWF_D int pos_of(const KCfg &k, int order, int i, int j) { return order == 0 ? posI(k, i, j) : posII(k, i, j); }
+
+// A position on a snake path kept as (line, cell within the line), stepped
+// without division: odd lines run up, even lines down, and a line end steps to
+// the next line at the same cell.
+struct Snake {
+  int a, c;
+};
+SWF_D Snake snake_at(const KCfg &k, int order, int p) {
+  const int n = order == 0 ? k.nx : k.ny;
+  const int a = p / n + 1, r = p - (a - 1) * n;
+  return Snake{a, (a & 1) ? r + 1 : n - r};
+}
+template <bool FWD>
+SWF_D void snake_step(Snake &s, int n) {
+  const bool up = s.a & 1;
+  if (FWD) {
+    if (up ? s.c < n : s.c > 1)
+      s.c += up ? 1 : -1;
+    else
+      ++s.a;
+  } else {
+    if (up ? s.c > 1 : s.c < n)
+      s.c += up ? -1 : 1;
+    else
+      --s.a;
+  }
+}
+SWF_D void snake_ij(const Snake &s, int order, int &i, int &j) {
+  i = order == 0 ? s.c : s.a;
+  j = order == 0 ? s.a : s.c;
+}
 
 SWF_D void set_bit(unsigned *m, int p) { atomicOr(&m[p >> 5], 1u << (p & 31)); }
 SWF_D void clear_bit(unsigned *m, int p) { atomicAnd(&m[p >> 5], ~(1u << (p & 31))); }
@@ -51,10 +86,11 @@ struct SweepShared {
     double prod[4][NT];     // u_bar partial products
   };
   int ends[NT];
+  int woff[NT];  // first replay step of each slot's genuine walk
   int wscan[SW_WCH];
   unsigned dirty[SW_DIRTY_WORDS];
   unsigned char genuine[NT];
-  int nlist, last_end, flag, all_dirty;
+  int nlist, last_end, flag, all_dirty, nsteps;
   long long tpos, tneg;
   double ubar[4];
   double red[4][NT / 64];
@@ -80,19 +116,19 @@ SWF_D void store_ij(const KCfg &k, const Bufs &b, int i, int j, const double v[4
   for (int c = 0; c < 4; ++c) b.U[c * k.plane + o] = v[c];
 }
 
-// nod_lim (ex4_sedov_2d/main.f90:1359-1396); returns true for the positive branch
-SWF_D bool nod_lim(double g1, double eps, const double v1[4], const double v2[4], const double vb[4],
-                   double out[4]) {
+// nod_lim (ex4_sedov_2d/main.f90:1359-1396); returns true for the positive branch.
+// p1, p2, pb: the pressures of v1, v2 and vb, each g1*(E - (0.5*(m1^2 + m2^2))/rho)
+// as sweep_pres forms them (the reference's expression), passed in so that a walk
+// forms each cell's pressure once.
+SWF_D bool nod_lim(double eps, double p1, double p2, double pb, const double v1[4], const double v2[4],
+                   const double vb[4], double out[4]) {
 #pragma clang fp contract(off)
-  double p1 = g1 * (v1[3] - (0.5 * (v1[1] * v1[1] + v1[2] * v1[2])) / v1[0]);
-  double p2 = g1 * (v2[3] - (0.5 * (v2[1] * v2[1] + v2[2] * v2[2])) / v2[0]);
   double t;
   bool pos;
   if (p2 > eps) {
     t = (p1 - eps) / (p1 - p2);
     pos = true;
   } else {
-    double pb = g1 * (vb[3] - (0.5 * (vb[1] * vb[1] + vb[2] * vb[2])) / vb[0]);
     double t_1 = (p1 - eps) / (p1 - pb);
     double d12 = sqrt((((v2[3] - v1[3]) * (v2[3] - v1[3]) + (v2[2] - v1[2]) * (v2[2] - v1[2])) +
                        (v2[1] - v1[1]) * (v2[1] - v1[1])) +
@@ -109,45 +145,67 @@ SWF_D bool nod_lim(double g1, double eps, const double v1[4], const double v2[4]
 }
 
 // Pressure-sweep walk from position f in `order`.  FWD: neighbour f+1, else f-1.
-// WRITE == false: speculative, returns the stop position only.  The cells of the
-// path are loaded PF positions ahead (clamped at the last position), so a long
-// walk waits on its arithmetic, not on one memory round trip per cell; the
-// values used are those of cells the walk reaches, which no other walk writes.
+// WRITE == false: speculative, returns the stop position only, and records each
+// step's adjusted state (and branch) in the slot's rows of the walk record, the
+// value handed to the stop position after the last step, up to SW_REC rows.
+// The cells of the path are loaded PF = 6 positions ahead (clamped at the last
+// position); the values used are those of cells the walk reaches, which no other
+// walk writes.  Each cell's pressure is formed once (sweep_pres = nod_lim's form).
 template <bool FWD, bool WRITE, int NT>
-SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], unsigned *next_map,
-                long long *npos, long long *nneg, SweepShared<NT> *sh) {
+SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], double pb, unsigned *next_map,
+                long long *npos, long long *nneg, SweepShared<NT> *sh, int slot) {
 #pragma clang fp contract(off)
-  constexpr int PF = 3;
+  static_assert(NT <= SW_T, "walk records hold SW_T slots");
   const int N = k.nx * k.ny;
   const int last = FWD ? N - 1 : 0;
+  const int nline = order == 0 ? k.nx : k.ny;
   const double g1 = k.gamma1, eps = k.eps, wr = k.weight / k.weight;
   int p = f, i, j;
-  cell_of(k, order, p, i, j);
+  Snake cur = snake_at(k, order, p);  // the walk's current cell
+  snake_ij(cur, order, i, j);
   double v[4];
   load_ij(k, b, i, j, v);
-  double av[PF][4];  // cells p+1 .. p+PF (p-1 .. p-PF backward)
+  // cells p+1 .. p+PF (p-1 ... backward) in PF slots used in rotation: the loop
+  // body is unrolled PF times, so no slot is copied while its load is in flight;
+  // the path is stepped by snake cursors, without division
+  constexpr int PF = 6;
+  double a0[4], a1[4], a2[4], a3[4], a4[4], a5[4];
+  int q = p;           // the cell loaded last, PF ahead of p (clamped at `last`)
+  Snake pre = cur;
+  {
+    double(*slots[PF])[4] = {&a0, &a1, &a2, &a3, &a4, &a5};
 #pragma unroll
-  for (int d = 0; d < PF; ++d) {
-    int ci, cj;
-    cell_of(k, order, FWD ? min(p + 1 + d, last) : max(p - 1 - d, last), ci, cj);
-    load_ij(k, b, ci, cj, av[d]);
+    for (int d = 0; d < PF; ++d) {
+      if (q != last) {
+        snake_step<FWD>(pre, nline);
+        q += FWD ? 1 : -1;
+      }
+      int ci, cj;
+      snake_ij(pre, order, ci, cj);
+      load_ij(k, b, ci, cj, *slots[d]);
+    }
   }
-  while (true) {
+  double pv = sweep_pres(g1, v[0], v[1], v[2], v[3]);
+  unsigned posw = 0u;  // branch bits of the current 32 steps
+  int s = 0;
+  // one step with the neighbour in `slot_v`, which then receives cell nb+3;
+  // returns true at the stop position
+  auto step = [&](double(&slot_v)[4]) __attribute__((always_inline)) -> bool {
     const int nb = FWD ? p + 1 : p - 1;
     double vn[4], vt[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) vn[c] = av[0][c];
-#pragma unroll
-    for (int d = 0; d + 1 < PF; ++d) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) av[d][c] = av[d + 1][c];
-    }
+    for (int c = 0; c < 4; ++c) vn[c] = slot_v[c];
     {
+      if (q != last) {
+        snake_step<FWD>(pre, nline);
+        q += FWD ? 1 : -1;
+      }
       int ci, cj;
-      cell_of(k, order, FWD ? min(nb + PF, last) : max(nb - PF, last), ci, cj);
-      load_ij(k, b, ci, cj, av[PF - 1]);
+      snake_ij(pre, order, ci, cj);
+      load_ij(k, b, ci, cj, slot_v);
     }
-    const bool pos = nod_lim(g1, eps, v, vn, ub, vt);
+    const double pn = sweep_pres(g1, vn[0], vn[1], vn[2], vn[3]);
+    const bool pos = nod_lim(eps, pv, pn, pb, v, vn, ub, vt);
     double vn2[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) vn2[c] = vn[c] + wr * (v[c] - vt[c]);
@@ -159,20 +217,71 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
         ++*npos;
       else
         ++*nneg;
+    } else if (s < SW_REC) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b.wrec[((long long)s * 4 + c) * NT + slot] = vt[c];
+      if (pos) posw |= 1u << (s & 31);
+      if ((s & 31) == 31) {
+        b.wpos[(s >> 5) * NT + slot] = posw;
+        posw = 0u;
+      }
     }
     p = nb;
-    cell_of(k, order, p, i, j);
+    if (WRITE) {
+      snake_step<FWD>(cur, nline);
+      snake_ij(cur, order, i, j);
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) v[c] = vn2[c];
+    pv = sweep_pres(g1, v[0], v[1], v[2], v[3]);
     const bool edge = FWD ? (p == N - 1) : (p == 0);
-    if (edge || !sweep_flag(g1, eps, v[0], v[1], v[2], v[3])) {
+    if (edge || !((pv <= eps) || (v[0] <= eps))) {  // stop at the first cell not flagged
       if (WRITE) {
         store_ij(k, b, i, j, v);
         mark_dirty(k, *sh, i, j);
+      } else if (s + 1 < SW_REC) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b.wrec[((long long)(s + 1) * 4 + c) * NT + slot] = v[c];
+        b.wpos[((s + 1) >> 5) * NT + slot] = posw;  // the block of steps not yet stored
       }
-      return p;
+      return true;
     }
+    ++s;
+    return false;
+  };
+  while (true) {
+    if (step(a0)) return p;
+    if (step(a1)) return p;
+    if (step(a2)) return p;
+    if (step(a3)) return p;
+    if (step(a4)) return p;
+    if (step(a5)) return p;
   }
+}
+
+// Step st of a genuine walk from f with adj adjustments, from the speculative
+// record of slot `slot`: st < adj writes the adjusted cell f+st (f-st backward),
+// flags it for the next pass and counts its branch; st == adj writes the value
+// handed to the stop position.  The same stores, flags and counts as the walk.
+template <bool FWD, int NT>
+SWF_D void replay_step(const KCfg &k, const Bufs &b, int order, int f, int st, int adj, int slot, unsigned *next_map,
+                       long long *npos, long long *nneg, SweepShared<NT> *sh) {
+#pragma clang fp contract(off)
+  double v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = b.wrec[((long long)st * 4 + c) * NT + slot];
+  const unsigned pw = b.wpos[(st >> 5) * NT + slot];
+  const int p = FWD ? f + st : f - st;
+  int i, j;
+  cell_of(k, order, p, i, j);
+  store_ij(k, b, i, j, v);
+  mark_dirty(k, *sh, i, j);
+  if (st == adj) return;
+  if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) set_bit(next_map, p);
+  if ((pw >> (st & 31)) & 1u)
+    ++*npos;
+  else
+    ++*nneg;
 }
 
 // Density-sweep walk (scalar_sweep) in order I; dir +1 forward, -1 backward.
@@ -249,26 +358,12 @@ SWF_D void collect_chunk(const KCfg &k, const unsigned *map, int w0, SweepShared
   __syncthreads();
 }
 
-// Every set position of `map`, ascending, into sh.list when there are at most
-// SW_WCH*32 of them; returns their number, or -1 (sh.list not filled) when there
-// are more.  Thread t scans the contiguous words [t*wpt, (t+1)*wpt), so thread
-// order is position order, and one block-wide exclusive scan of the per-thread
-// counts gives each thread its place in the list.
+// Exclusive prefix sum of one int per thread over the workgroup, in thread
+// order; the total goes to *total (in LDS).  Uses sh.wscan.
 template <int NT>
-SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, int tid) {
+SWF_D int block_exclusive_scan(int x, SweepShared<NT> &sh, int *total, int tid) {
   static_assert(NT / 64 <= SW_WCH, "wave totals live in sh.wscan");
-  constexpr int BW = 16;  // words loaded together
-  const int wpt = (k.nwords + NT - 1) / NT;
-  const int w0 = tid * wpt, w1 = min(w0 + wpt, k.nwords);
-  unsigned v[BW];
-  int cnt = 0;
-  for (int w = w0; w < w1; w += BW) {
-#pragma unroll
-    for (int q = 0; q < BW; ++q) v[q] = (w + q < w1) ? map[w + q] : 0u;
-#pragma unroll
-    for (int q = 0; q < BW; ++q) cnt += __popc(v[q]);
-  }
-  int incl = cnt;  // inclusive scan within the wave
+  int incl = x;  // inclusive scan within the wave
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const int v = __shfl_up(incl, off, 64);
@@ -283,13 +378,35 @@ SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, i
       sh.wscan[q] = run;
       run += t;
     }
-    sh.nlist = run;
+    *total = run;
   }
   __syncthreads();
+  return sh.wscan[tid >> 6] + incl - x;
+}
+
+// Every set position of `map`, ascending, into sh.list when there are at most
+// SW_WCH*32 of them; returns their number, or -1 (sh.list not filled) when there
+// are more.  Thread t scans the contiguous words [t*wpt, (t+1)*wpt), so thread
+// order is position order, and one block-wide exclusive scan of the per-thread
+// counts gives each thread its place in the list.
+template <int NT>
+SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, int tid) {
+  constexpr int BW = 16;  // words loaded together
+  const int wpt = (k.nwords + NT - 1) / NT;
+  const int w0 = tid * wpt, w1 = min(w0 + wpt, k.nwords);
+  unsigned v[BW];
+  int cnt = 0;
+  for (int w = w0; w < w1; w += BW) {
+#pragma unroll
+    for (int q = 0; q < BW; ++q) v[q] = (w + q < w1) ? map[w + q] : 0u;
+#pragma unroll
+    for (int q = 0; q < BW; ++q) cnt += __popc(v[q]);
+  }
+  const int first = block_exclusive_scan<NT>(cnt, sh, &sh.nlist, tid);
   const int total = sh.nlist;
   if (total > SW_WCH * 32) return -1;
   if (cnt) {
-    int o = sh.wscan[tid >> 6] + incl - cnt;
+    int o = first;
     for (int w = w0; w < w1; w += BW) {
       if (wpt > BW) {  // more than one batch: load this one again
 #pragma unroll
@@ -314,7 +431,7 @@ SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, i
 // takes them from the end), NT candidates at a time.
 template <bool FWD, int NT>
 SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsigned *next, const double ub[4],
-                          SweepShared<NT> &sh, int tid) {
+                          double pb, SweepShared<NT> &sh, int tid) {
   const int N = k.nx * k.ny;
   for (int s0 = 0; s0 < n; s0 += NT) {
     const int idx = s0 + tid;
@@ -322,7 +439,7 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
     if (idx < n) f = FWD ? sh.list[idx] : sh.list[n - 1 - idx];
     const int le = sh.last_end;
     const bool cand = (f >= 0) && (FWD ? (f > le && f <= N - 2) : (f < le && f >= 1));
-    sh.ends[tid] = cand ? pwalk<FWD, false, NT>(k, b, order, f, ub, nullptr, nullptr, nullptr, nullptr) : -1;
+    sh.ends[tid] = cand ? pwalk<FWD, false, NT>(k, b, order, f, ub, pb, nullptr, nullptr, nullptr, nullptr, tid) : -1;
     sh.genuine[tid] = 0;
     __syncthreads();
     if (tid == 0) {
@@ -339,12 +456,39 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
       sh.last_end = last;
     }
     __syncthreads();
+    // genuine walks: recorded ones are replayed by the whole workgroup (their
+    // steps are independent), longer ones are walked again by their thread
+    long long np = 0, nn = 0;
+    int len = 0;  // cells written: the adjustments and the stop position
     if (sh.genuine[tid]) {
-      long long np = 0, nn = 0;
-      pwalk<FWD, true, NT>(k, b, order, f, ub, next, &np, &nn, &sh);
-      atomicAdd((unsigned long long *)&sh.tpos, (unsigned long long)np);
-      atomicAdd((unsigned long long *)&sh.tneg, (unsigned long long)nn);
+      const int end = sh.ends[tid];
+      const int adj = FWD ? end - f : f - end;
+      if (adj < SW_REC)
+        len = adj + 1;
+      else
+        pwalk<FWD, true, NT>(k, b, order, f, ub, pb, next, &np, &nn, &sh, tid);
     }
+    sh.woff[tid] = block_exclusive_scan<NT>(len, sh, &sh.nsteps, tid);
+    __syncthreads();
+    const int total = sh.nsteps;
+    const int m = min(NT, n - s0);
+    for (int x = tid; x < total; x += NT) {
+      int lo = 0, hi = m - 1;  // the last slot whose replay starts at or before x
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sh.woff[mid] <= x)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int w = lo, st = x - sh.woff[w];
+      const int fw = FWD ? sh.list[s0 + w] : sh.list[n - 1 - (s0 + w)];
+      const int endw = sh.ends[w];
+      const int adj = FWD ? endw - fw : fw - endw;
+      replay_step<FWD, NT>(k, b, order, fw, st, adj, w, next, &np, &nn, &sh);
+    }
+    if (np) atomicAdd((unsigned long long *)&sh.tpos, (unsigned long long)np);
+    if (nn) atomicAdd((unsigned long long *)&sh.tneg, (unsigned long long)nn);
     __syncthreads();
   }
 }
@@ -354,19 +498,19 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
 // More flagged positions than the list holds: chunk by chunk of SW_WCH words.
 template <bool FWD, int NT>
 SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned *map, unsigned *next,
-                         const double ub[4], SweepShared<NT> &sh, int tid) {
+                         const double ub[4], double pb, SweepShared<NT> &sh, int tid) {
   const int N = k.nx * k.ny;
   if (tid == 0) sh.last_end = FWD ? -1 : N;
   __syncthreads();
   const int n = collect_all<NT>(k, map, sh, tid);
   if (n >= 0) {
-    pressure_walks<FWD, NT>(k, b, order, n, next, ub, sh, tid);
+    pressure_walks<FWD, NT>(k, b, order, n, next, ub, pb, sh, tid);
   } else {
     const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
     for (int ci = 0; ci < nchunks; ++ci) {
       const int c = FWD ? ci : (nchunks - 1 - ci);
       collect_chunk<NT>(k, map, c * SW_WCH, sh, tid);
-      pressure_walks<FWD, NT>(k, b, order, sh.nlist, next, ub, sh, tid);
+      pressure_walks<FWD, NT>(k, b, order, sh.nlist, next, ub, pb, sh, tid);
     }
   }
   // the position the pass never examines (N-1 forward, 0 backward)
@@ -541,8 +685,8 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
   // the density sweep changes cells after the stage kernel summed them
   compute_ubar<NT>(k, b, serial_ubar, g->rho_any ? nullptr : upart, sh, tid);
   double ub[4] = {sh.ubar[0], sh.ubar[1], sh.ubar[2], sh.ubar[3]};
+  const double pb = sweep_pres(k.gamma1, ub[0], ub[1], ub[2], ub[3]);  // P(u_bar)
   if (k.pbar_check) {
-    const double pb = k.gamma1 * (ub[3] - (0.5 * (ub[1] * ub[1] + ub[2] * ub[2])) / ub[0]);
     if (pb <= 0.) {
       if (tid == 0) {
         g->status = ST_PBAR;
@@ -563,9 +707,9 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
   for (int q = 1; q <= k.swp_num; ++q) {
     const int order = (q & 1) ? 0 : 1;
     clear_map<NT>(k, B, tid);
-    pressure_pass<true, NT>(k, b, order, A, B, ub, sh, tid);
+    pressure_pass<true, NT>(k, b, order, A, B, ub, pb, sh, tid);
     clear_map<NT>(k, A, tid);
-    pressure_pass<false, NT>(k, b, order, B, A, ub, sh, tid);
+    pressure_pass<false, NT>(k, b, order, B, A, ub, pb, sh, tid);
     // convergence test (:1714-1744): no cell left with P <= eps or rho <= eps
     // (ex5: P < eps or rho < eps)
     if (tid == 0) {
@@ -640,7 +784,6 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
   __syncthreads();
 }
 
-constexpr int SW_T = 1024;
 #ifdef SWF_COMMON_KERNELS
 // Stand-alone sweep of the current state (the sweep_lim operator on its own).
 __global__ __launch_bounds__(SW_T) void k_sweep(KCfg k, Bufs b, int serial_ubar) {

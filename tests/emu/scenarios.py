@@ -74,7 +74,7 @@ def first_dt(cfg):
     return {"rc": rc, "dt": [dto, s.step(0.0, 0.001)]}
 
 
-def crafted_sweep(n, nbad, seed):
+def crafted_sweep(n, nbad, seed, chain=0):
     """swf_sweep on a state with nbad cells of slightly negative pressure (runs of
     consecutive flagged cells included) and three near-vacuum cells, against the
     oracle's sweep: fields bitwise and statistics equal."""
@@ -93,6 +93,11 @@ def crafted_sweep(n, nbad, seed):
             u[j + 2, ii + 2, 3] = 0.5 * (u[j + 2, ii + 2, 1] ** 2 + u[j + 2, ii + 2, 2] ** 2) / u[j + 2, ii + 2, 0] \
                 - 1e-3 * rng.random()
             placed += 1
+    for pos in range(n * n // 3, n * n // 3 + chain):  # consecutive snake-order positions
+        j = pos // n + 1
+        r = pos - (j - 1) * n
+        i = r + 1 if j % 2 == 1 else n - r
+        u[j + 2, i + 2, 3] = 0.5 * (u[j + 2, i + 2, 1] ** 2 + u[j + 2, i + 2, 2] ** 2) / u[j + 2, i + 2, 0] - 1e-4
     for _ in range(3):
         i, j = int(rng.integers(1, n + 1)), int(rng.integers(1, n + 1))
         u[j + 2, i + 2, 0] = 1e-14
@@ -127,8 +132,8 @@ def main(name):
         cfg = S.vortex_config(90, "fp64_exact", use_limiter=1)
         out = run_both(cfg, S.init_vortex(cfg), 0.01, 100)
     elif name.startswith("crafted_sweep_"):
-        n, nbad = (int(v) for v in name[len("crafted_sweep_"):].split("_"))
-        out = crafted_sweep(n, nbad, 7)
+        n, nbad, *rest = (int(v) for v in name[len("crafted_sweep_"):].split("_"))
+        out = crafted_sweep(n, nbad, 7, rest[0] if rest else 0)
     elif name.startswith("first_dt_"):
         out = first_dt(S.jet_config(40, 20, name[len("first_dt_"):], cfl=0.5, half=False))
     else:

@@ -12,7 +12,9 @@ arithmetic and control flow, not races; the GPU tests check the real device.
 - Vortex 90^2 with the Zhang-Shu flux limiter: bitwise, limiter counts equal.
 - swf_sweep on crafted states with 14, 3000 and 12000 cells of negative
   pressure (the last more than one ordered list of flagged positions holds, so
-  its passes go chunk by chunk): fields bitwise, statistics equal.
+  its passes go chunk by chunk), and with a chain of 100 or 300 consecutive
+  flagged snake positions (walks shorter and longer than a walk record):
+  fields bitwise, statistics equal.
 - FP64 / FP32-bulk full jet 64x32 with sweeps (u_bar from per-tile sums).
 - After upload of the ambient jet and swf_bc, the first dt sees the Mach 2000
   inflow ghosts (alpha ~ 800) in every precision mode.
@@ -44,7 +46,8 @@ def scenario(name):
 
 
 @pytest.mark.parametrize("name", ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter",
-                                  "crafted_sweep_24_14", "crafted_sweep_128_3000", "crafted_sweep_160_12000"])
+                                  "crafted_sweep_24_14", "crafted_sweep_128_3000", "crafted_sweep_160_12000",
+                                  "crafted_sweep_64_20_100", "crafted_sweep_64_20_300"])
 def test_emulated_kernels_match_oracle(emu_lib, name):
     r = scenario(name)
     assert r["rc"] == 0

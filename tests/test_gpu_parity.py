@@ -145,10 +145,12 @@ def _crafted_sweep_state(cfg, seed):
     return u
 
 
-@pytest.mark.parametrize("n,nbad", [(128, 3000), (160, 12000)])
-def test_sweep_many_flagged_matches_oracle(n, nbad):
-    """Thousands of flagged cells; at 12000 a pass holds more flagged positions
-    than its ordered list and goes chunk by chunk."""
+@pytest.mark.parametrize("n,nbad,chain", [(128, 3000, 0), (160, 12000, 0), (64, 20, 100), (64, 20, 300)])
+def test_sweep_many_flagged_matches_oracle(n, nbad, chain):
+    """Thousands of flagged cells (at 12000 a pass holds more flagged positions
+    than its ordered list and goes chunk by chunk), or a chain of consecutive
+    flagged snake positions whose walks are shorter (100) or longer (300) than
+    the record a speculative walk keeps for its writes."""
     cfg = S.sedov_config(n, "fp64_exact")
     rng = np.random.default_rng(7)
     u = np.zeros(S.state_shape(cfg))
@@ -164,6 +166,11 @@ def test_sweep_many_flagged_matches_oracle(n, nbad):
             u[j + 2, ii + 2, 3] = 0.5 * (u[j + 2, ii + 2, 1] ** 2 + u[j + 2, ii + 2, 2] ** 2) / u[j + 2, ii + 2, 0] \
                 - 1e-3 * rng.random()
             placed += 1
+    for pos in range(n * n // 3, n * n // 3 + chain):  # consecutive snake-order positions (order I)
+        j = pos // n + 1
+        r = pos - (j - 1) * n
+        i = r + 1 if j % 2 == 1 else n - r
+        u[j + 2, i + 2, 3] = 0.5 * (u[j + 2, i + 2, 1] ** 2 + u[j + 2, i + 2, 2] ** 2) / u[j + 2, i + 2, 0] - 1e-4
     o = O.Oracle(oracle_cfg(cfg))
     o.set_state(u)
     assert o.sweep() == 0
@@ -172,7 +179,7 @@ def test_sweep_many_flagged_matches_oracle(n, nbad):
     s.sweep()
     assert_bitwise(s.download()[3:-3, 3:-3], o.state()[3:-3, 3:-3], "swept field")
     so, sg = o.stats(), s.stats()
-    assert so["total_neg"] > nbad // 2
+    assert so["total_neg"] > nbad // 2 + chain // 2
     for key in ("sweep_count", "den_count", "pre_count", "total_sweep", "max_sweep", "total_neg", "max_neg",
                 "total_pos", "max_pos"):
         assert sg[key] == so[key], key
