@@ -18,6 +18,8 @@
  *   swf_stats / swf_track      the statistics printed by `output` (:1326-1351)
  *   swf_weno_points            weno_right / weno_left (ex4_sedov_2d/weno5.f90:86-146)
  *   swf_time_continue          the binary128 loop test of :112
+ *   swf_batch_*                many instances advanced together (no counterpart: the
+ *                              reference runs one problem per program; see below)
  *
  * Arrays crossing the boundary are host memory in the reference's Fortran layout
  * u(4, -2:nx+3, -2:ny+3): column-major, component fastest.  The caller owns them;
@@ -130,6 +132,35 @@ int swf_weno_points(const double *a, double *right, double *left, int64_t n, int
  * ghost images, run as stage 1 into a scratch state).  No counterpart in the
  * reference: measurement support. */
 int swf_time_kernel(swf_ctx *ctx, int32_t which, int32_t nreps, double *avg_ms);
+
+/*
+ * Batches.  No counterpart in the reference, which runs one problem per program:
+ * n independent instances (any grids and problems, one precision mode, use_limiter
+ * 0 or 2) advanced together, each RK stage one launch of each stage kernel over the
+ * tiles of every instance and one sweep workgroup per instance.  Each instance
+ * keeps its own time step, time, sweep and statistics; its results are those of
+ * the same configuration run alone (the same kernels).
+ *   swf_batch_create      one swf_ctx per instance (swf_create of cfgs[i])
+ *   swf_batch_instance    instance i's context: swf_upload, swf_bc, swf_download,
+ *                         swf_stats and swf_track work on it; it belongs to the batch
+ *   swf_batch_advance     every instance's time loop up to max_steps steps: time_inout,
+ *                         tfinal, steps_out, status_out are arrays of n; returns the
+ *                         first non-zero status (0 if none)
+ *   swf_batch_set_schedule / swf_batch_tune
+ *                         the order of all instances' tiles in a launch (0: instance by
+ *                         instance, 1: round robin, 2: round robin largest first);
+ *                         swf_batch_tune times the stage kernels under each (reps
+ *                         launches, no effect on the states), keeps the fastest and
+ *                         reports it and the three times (ms_out[3], may be NULL)
+ */
+typedef struct swf_batch swf_batch;
+int swf_batch_create(const swf_config *cfgs, int32_t n, swf_batch **out);
+void swf_batch_destroy(swf_batch *batch);
+swf_ctx *swf_batch_instance(swf_batch *batch, int32_t i);
+int swf_batch_advance(swf_batch *batch, double *time_inout, const double *tfinal, int64_t max_steps,
+                      int64_t *steps_out, int32_t *status_out);
+int swf_batch_set_schedule(swf_batch *batch, int32_t schedule);
+int swf_batch_tune(swf_batch *batch, int32_t reps, int32_t *schedule_out, float *ms_out);
 
 int swf_time_continue(double time, double tfinal);
 int swf_sync(swf_ctx *ctx);

@@ -10,13 +10,13 @@ without a HIP device, raises.
     cfg = sedov_config(640, precision="fp64")
     s = Solver(cfg); s.upload(init_sedov(cfg)); t, n = s.advance(0.0, 1.0)
 """
-from ._lib import (SwfConfig, SwfError, Solver, lib, library_path, sedov_config, vortex_config,
+from ._lib import (SwfConfig, SwfError, Solver, Batch, lib, library_path, sedov_config, vortex_config,
                    jet_config, weno_points, time_continue, state_shape, PRECISIONS, BC_SEDOV,
                    BC_PERIODIC, BC_JET_HALF, BC_JET_FULL)
 from .problems import init_sedov, init_vortex, init_jet, vortex_exact
 
 __all__ = [
-    "SwfConfig", "SwfError", "Solver", "lib", "library_path", "sedov_config", "vortex_config", "jet_config",
+    "SwfConfig", "SwfError", "Solver", "Batch", "lib", "library_path", "sedov_config", "vortex_config", "jet_config",
     "weno_points", "time_continue", "state_shape", "PRECISIONS", "BC_SEDOV", "BC_PERIODIC", "BC_JET_HALF",
     "BC_JET_FULL", "init_sedov", "init_vortex", "init_jet", "vortex_exact",
 ]

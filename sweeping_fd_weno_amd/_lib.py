@@ -87,6 +87,14 @@ def lib():
         L.swf_time_continue.argtypes = [C.c_double, C.c_double]
         L.swf_sync.argtypes = [vp]
         L.swf_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, dp]
+        L.swf_batch_create.argtypes = [P(SwfConfig), C.c_int32, P(vp)]
+        L.swf_batch_destroy.argtypes = [vp]
+        L.swf_batch_destroy.restype = None
+        L.swf_batch_instance.argtypes = [vp, C.c_int32]
+        L.swf_batch_instance.restype = vp
+        L.swf_batch_advance.argtypes = [vp, dp, dp, C.c_int64, P(C.c_int64), P(C.c_int32)]
+        L.swf_batch_set_schedule.argtypes = [vp, C.c_int32]
+        L.swf_batch_tune.argtypes = [vp, C.c_int32, P(C.c_int32), P(C.c_float)]
         L.swf_strerror.argtypes = [C.c_int]
         L.swf_strerror.restype = C.c_char_p
         L.swf_last_error.argtypes = []
@@ -175,15 +183,20 @@ def weno_points(a: np.ndarray, precision="fp64_exact", device: int = 0):
 class Solver:
     """One device solver context (the reference's global state and `rk` call tree)."""
 
-    def __init__(self, cfg: SwfConfig):
+    def __init__(self, cfg: SwfConfig, _borrowed=None):
         self.cfg = cfg
+        self._owned = _borrowed is None
+        if _borrowed is not None:  # an instance of a Batch: the batch owns it
+            self._h = C.c_void_p(_borrowed)
+            return
         h = C.c_void_p()
         _check(lib().swf_create(C.byref(cfg), C.byref(h)), "swf_create")
         self._h = h
 
     def close(self) -> None:
         if getattr(self, "_h", None):
-            lib().swf_destroy(self._h)
+            if self._owned:
+                lib().swf_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -272,3 +285,59 @@ class Solver:
         P = C.POINTER(C.c_int64)
         lib().swf_track(self._h, q.ctypes.data_as(P), ng.ctypes.data_as(P), ps.ctypes.data_as(P), n)
         return q[:n], ng[:n], ps[:n]
+
+
+class Batch:
+    """Independent instances advanced together (swf_batch_*): one launch of each
+    stage kernel per RK stage over the tiles of all instances, one sweep workgroup
+    per instance.  ``instance(i)`` is instance i's Solver (upload, bc, download,
+    stats); ``advance`` runs every instance's own time loop."""
+
+    def __init__(self, cfgs):
+        self.cfgs = list(cfgs)
+        n = len(self.cfgs)
+        arr = (SwfConfig * n)(*self.cfgs)
+        h = C.c_void_p()
+        _check(lib().swf_batch_create(arr, n, C.byref(h)), "swf_batch_create")
+        self._h = h
+        self.n = n
+        self._inst = [Solver(self.cfgs[i], _borrowed=lib().swf_batch_instance(h, i)) for i in range(n)]
+
+    def instance(self, i: int) -> "Solver":
+        return self._inst[i]
+
+    def advance(self, times, tfinals, max_steps: int = 1 << 40):
+        """Returns (times, steps, statuses) as numpy arrays of n."""
+        t = np.ascontiguousarray(times, dtype=np.float64).copy()
+        tf = np.ascontiguousarray(tfinals, dtype=np.float64)
+        steps = np.zeros(self.n, dtype=np.int64)
+        st = np.zeros(self.n, dtype=np.int32)
+        rc = lib().swf_batch_advance(self._h, _dp(t), _dp(tf), max_steps,
+                                     steps.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     st.ctypes.data_as(C.POINTER(C.c_int32)))
+        if rc < 0:
+            _check(rc, "swf_batch_advance")
+        return t, steps, st
+
+    def set_schedule(self, schedule: int) -> None:
+        _check(lib().swf_batch_set_schedule(self._h, schedule), "swf_batch_set_schedule")
+
+    def tune(self, reps: int = 10):
+        """Times the stage kernels under each schedule, keeps the fastest: (schedule, ms[3])."""
+        pick = C.c_int32(0)
+        ms = (C.c_float * 3)()
+        _check(lib().swf_batch_tune(self._h, reps, C.byref(pick), ms), "swf_batch_tune")
+        return pick.value, list(ms)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            for s in self._inst:
+                s._h = None
+            lib().swf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -7,7 +7,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
+#include <vector>
 
 #include "../../include/swf.h"
 #include "swf_launch.h"
@@ -540,6 +542,269 @@ int swf_weno_points(const double *a, double *right, double *left, int64_t n, int
   (void)hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "swf_weno_points");
   return SWF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent instances
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct swf_batch {
+  int n = 0, mode = 0, schedule = 0;
+  swf_ctx **ctx = nullptr;     // the instances (their own buffers, time and statistics)
+  BatchInst *d_inst = nullptr;
+  int2 *d_xmap = nullptr, *d_ymap = nullptr;  // block -> (instance, tile)
+  int nxb = 0, nyb = 0;
+  double *d_time = nullptr, *d_tfinal = nullptr;
+  BatchStatus *d_status = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int device = 0;
+};
+
+namespace {
+
+// The order of the tiles of all instances in a batch launch.  0: instance by
+// instance; 1: round robin over the instances (tile t of every instance before
+// tile t+1); 2: round robin with the largest instances first.
+void batch_map(const swf_batch *bt, bool ytiles, int schedule, std::vector<int2> &map) {
+  map.clear();
+  std::vector<int> cnt(bt->n), order(bt->n);
+  for (int i = 0; i < bt->n; ++i) {
+    const KCfg &k = bt->ctx[i]->k;
+    cnt[i] = ytiles ? ub_gx(k) * ub_gy(k) : xf_gx(k) * xf_gy(k);
+    order[i] = i;
+  }
+  if (schedule == 2)
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+  if (schedule == 0) {
+    for (int i = 0; i < bt->n; ++i)
+      for (int t = 0; t < cnt[i]; ++t) map.push_back(make_int2(i, t));
+    return;
+  }
+  int most = 0;
+  for (int i = 0; i < bt->n; ++i) most = cnt[i] > most ? cnt[i] : most;
+  for (int t = 0; t < most; ++t)
+    for (int q = 0; q < bt->n; ++q) {
+      const int i = order[q];
+      if (t < cnt[i]) map.push_back(make_int2(i, t));
+    }
+}
+
+int batch_set_maps(swf_batch *bt, int schedule) {
+  std::vector<int2> xm, ym;
+  batch_map(bt, false, schedule, xm);
+  batch_map(bt, true, schedule, ym);
+  if (bt->d_xmap) (void)hipFree(bt->d_xmap);
+  if (bt->d_ymap) (void)hipFree(bt->d_ymap);
+  bt->d_xmap = bt->d_ymap = nullptr;
+  HIPC(hipMalloc(&bt->d_xmap, sizeof(int2) * xm.size()));
+  HIPC(hipMalloc(&bt->d_ymap, sizeof(int2) * ym.size()));
+  HIPC(hipMemcpy(bt->d_xmap, xm.data(), sizeof(int2) * xm.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(bt->d_ymap, ym.data(), sizeof(int2) * ym.size(), hipMemcpyHostToDevice));
+  bt->nxb = (int)xm.size();
+  bt->nyb = (int)ym.size();
+  bt->schedule = schedule;
+  if (bt->gexec) {
+    (void)hipGraphExecDestroy(bt->gexec);
+    bt->gexec = nullptr;
+  }
+  return SWF_OK;
+}
+
+// one RK stage of every instance: three launches
+void batch_enqueue_stage(swf_batch *bt, int s, int p) {
+  const int slot_in = s - 1, slot_out = s % 3;
+  launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap, bt->nxb, bt->stream, s, slot_in, p);
+  launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap, bt->nyb, bt->stream, s, slot_in, p, 0);
+  launch_stage_end_batch(bt->d_inst, bt->n, bt->stream, s, slot_out, p, bt->mode == MODE_EXACT ? 1 : 0);
+}
+
+void batch_enqueue_step(swf_batch *bt, int p) {
+  batch_enqueue_stage(bt, 1, p);
+  batch_enqueue_stage(bt, 2, p);
+  batch_enqueue_stage(bt, 3, p);
+}
+
+int batch_ensure_graph(swf_batch *bt) {
+  if (bt->gexec) return SWF_OK;
+  hipGraph_t graph = nullptr;
+  HIPC(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+  for (int q = 0; q < kGraphSteps; ++q) batch_enqueue_step(bt, q & 1);
+  HIPC(hipStreamEndCapture(bt->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&bt->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate (batch)");
+  return SWF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swf_batch_create(const swf_config *cfgs, int32_t n, swf_batch **out) {
+  if (!cfgs || !out || n < 1) return SWF_E_INVALID;
+  *out = nullptr;
+  for (int i = 0; i < n; ++i)
+    if (cfgs[i].precision != cfgs[0].precision || cfgs[i].device != cfgs[0].device || cfgs[i].use_limiter == 1)
+      return SWF_E_INVALID;
+  swf_batch *bt = new (std::nothrow) swf_batch;
+  if (!bt) return SWF_E_NOMEM;
+  bt->n = n;
+  bt->mode = cfgs[0].precision;
+  bt->device = cfgs[0].device;
+  bt->ctx = new (std::nothrow) swf_ctx *[n]();
+  int rc = bt->ctx ? SWF_OK : SWF_E_NOMEM;
+  for (int i = 0; rc == SWF_OK && i < n; ++i) rc = swf_create(&cfgs[i], &bt->ctx[i]);
+  if (rc == SWF_OK) {
+    std::vector<BatchInst> h(n);
+    for (int i = 0; i < n; ++i) {
+      h[i].k = bt->ctx[i]->k;
+      h[i].b = bt->ctx[i]->b;
+      h[i].S1 = bt->ctx[i]->S1;
+      h[i].S2 = bt->ctx[i]->S2;
+    }
+    hipError_t e = hipSetDevice(bt->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&bt->d_inst, sizeof(BatchInst) * n);
+    if (e == hipSuccess) e = hipMemcpy(bt->d_inst, h.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&bt->d_time, sizeof(double) * 2 * n);
+    if (e == hipSuccess) e = hipMalloc(&bt->d_status, sizeof(BatchStatus) * n);
+    if (e != hipSuccess) rc = (e == hipErrorOutOfMemory) ? SWF_E_NOMEM : hip_fail(e, "swf_batch_create");
+    bt->d_tfinal = bt->d_time ? bt->d_time + n : nullptr;
+  }
+  if (rc == SWF_OK) rc = batch_set_maps(bt, 1);
+  if (rc != SWF_OK) {
+    swf_batch_destroy(bt);
+    return rc;
+  }
+  *out = bt;
+  return SWF_OK;
+}
+
+void swf_batch_destroy(swf_batch *bt) {
+  if (!bt) return;
+  (void)hipSetDevice(bt->device);
+  if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+  if (bt->gexec) (void)hipGraphExecDestroy(bt->gexec);
+  (void)hipFree(bt->d_inst);
+  (void)hipFree(bt->d_xmap);
+  (void)hipFree(bt->d_ymap);
+  (void)hipFree(bt->d_time);
+  (void)hipFree(bt->d_status);
+  if (bt->stream) (void)hipStreamDestroy(bt->stream);
+  if (bt->ctx)
+    for (int i = 0; i < bt->n; ++i) swf_destroy(bt->ctx[i]);
+  delete[] bt->ctx;
+  delete bt;
+}
+
+swf_ctx *swf_batch_instance(swf_batch *bt, int32_t i) {
+  if (!bt || i < 0 || i >= bt->n) return nullptr;
+  return bt->ctx[i];
+}
+
+int swf_batch_set_schedule(swf_batch *bt, int32_t schedule) {
+  if (!bt || schedule < 0 || schedule > 2) return SWF_E_INVALID;
+  HIPC(hipSetDevice(bt->device));
+  return batch_set_maps(bt, schedule);
+}
+
+int swf_batch_advance(swf_batch *bt, double *time_inout, const double *tfinal, int64_t max_steps, int64_t *steps_out,
+                      int32_t *status_out) {
+  if (!bt || !time_inout || !tfinal || max_steps < 0) return SWF_E_INVALID;
+  HIPC(hipSetDevice(bt->device));
+  int rc = batch_ensure_graph(bt);
+  if (rc) return rc;
+  const int n = bt->n;
+  std::vector<BatchStatus> h(n);
+  HIPC(hipMemcpyAsync(bt->d_time, time_inout, sizeof(double) * n, hipMemcpyHostToDevice, bt->stream));
+  HIPC(hipMemcpyAsync(bt->d_tfinal, tfinal, sizeof(double) * n, hipMemcpyHostToDevice, bt->stream));
+  launch_begin_run_batch(bt->d_inst, n, bt->d_time, bt->d_tfinal, bt->stream);
+  int64_t launched = 0, since_check = 0;
+  while (launched < max_steps) {
+    if (max_steps - launched >= kGraphSteps) {
+      HIPC(hipGraphLaunch(bt->gexec, bt->stream));
+      launched += kGraphSteps;
+      since_check += kGraphSteps;
+    } else {
+      for (; launched < max_steps; ++launched, ++since_check) batch_enqueue_step(bt, (int)(launched & 1));
+    }
+    rc = check_launch("swf_batch_advance");
+    if (rc) return rc;
+    if (since_check >= kCheckEvery || launched >= max_steps) {
+      since_check = 0;
+      launch_batch_status(bt->d_inst, n, bt->d_status, bt->stream);
+      HIPC(hipMemcpyAsync(h.data(), bt->d_status, sizeof(BatchStatus) * n, hipMemcpyDeviceToHost, bt->stream));
+      HIPC(hipStreamSynchronize(bt->stream));
+      bool any = false;  // an instance still stepping
+      for (int i = 0; i < n; ++i) any = any || (h[i].status == ST_OK && h[i].steps_done >= launched);
+      if (!any) break;
+    }
+  }
+  launch_batch_status(bt->d_inst, n, bt->d_status, bt->stream);
+  HIPC(hipMemcpyAsync(h.data(), bt->d_status, sizeof(BatchStatus) * n, hipMemcpyDeviceToHost, bt->stream));
+  HIPC(hipStreamSynchronize(bt->stream));
+  int worst = SWF_OK;
+  for (int i = 0; i < n; ++i) {
+    time_inout[i] = h[i].time;
+    if (steps_out) steps_out[i] = h[i].steps_done;
+    if (status_out) status_out[i] = h[i].status;
+    bt->ctx[i]->steps_total += h[i].steps_done;
+    if (h[i].status != SWF_OK && worst == SWF_OK) worst = h[i].status;
+  }
+  return worst;
+}
+
+int swf_batch_tune(swf_batch *bt, int32_t reps, int32_t *schedule_out, float *ms_out) {
+  if (!bt || reps < 1) return SWF_E_INVALID;
+  HIPC(hipSetDevice(bt->device));
+  // the stage-1 kernels of every instance without side effects on the state:
+  // x-fluxes into Fx, stage results into S1, no scalars; every instance's
+  // step gate is opened for the timing and restored after it
+  std::vector<DevScalars> h0(bt->n);
+  for (int i = 0; i < bt->n; ++i) {
+    int rc = read_scalars(bt->ctx[i], &h0[i]);
+    if (rc) return rc;
+    launch_set_alpha(bt->ctx[i]->b, bt->stream, 0, h0[i].alpha_bits[0], 0, 1);
+  }
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  float best = -1.f;
+  int pick = 0;
+  for (int sched = 0; sched < 3; ++sched) {
+    int rc = batch_set_maps(bt, sched);
+    if (rc) return rc;
+    for (int w = 0; w < 2; ++w) {
+      launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap, bt->nxb, bt->stream, 0, 0, 0);
+      launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap, bt->nyb, bt->stream, 1, 0, 0, YS_NO_SCALARS);
+    }
+    HIPC(hipEventRecord(e0, bt->stream));
+    for (int r = 0; r < reps; ++r) {
+      launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap, bt->nxb, bt->stream, 0, 0, 0);
+      launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap, bt->nyb, bt->stream, 1, 0, 0, YS_NO_SCALARS);
+    }
+    HIPC(hipEventRecord(e1, bt->stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    if (ms_out) ms_out[sched] = ms;
+    if (best < 0.f || ms < best) {
+      best = ms;
+      pick = sched;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int i = 0; i < bt->n; ++i) launch_set_alpha(bt->ctx[i]->b, bt->stream, 0, h0[i].alpha_bits[0], 0, h0[i].active[0]);
+  HIPC(hipStreamSynchronize(bt->stream));
+  int rc = check_launch("swf_batch_tune");
+  if (rc) return rc;
+  if (schedule_out) *schedule_out = pick;
+  return batch_set_maps(bt, pick);
 }
 
 }  // extern "C"

@@ -82,6 +82,21 @@ struct Bufs {
                                     // x-flux, stage and edge kernels (zs_pairs)
 };
 
+// One instance of a batch: its configuration, its buffers outside a stage (Uin ==
+// U == U0 == S0) and the two other state buffers of the stage cycle.
+struct BatchInst {
+  KCfg k;
+  Bufs b;
+  double *S1, *S2;
+};
+
+// What the host's time loop reads of each instance of a batch.
+struct BatchStatus {
+  double time;
+  long long steps_done;
+  int status, active;
+};
+
 // Row layout of every plane: cell i (Fortran index, ghosts from -2) at column
 // i + XOFF, so interior cell 1 starts a 128-byte line and each 16-cell tile row of
 // the stage kernel is exactly one cache line; the pitch P is a multiple of 16.

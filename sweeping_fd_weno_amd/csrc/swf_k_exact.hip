@@ -42,6 +42,39 @@ void launch_ystage(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int st
   else
     launch_ystage_m12(mode, k, b, s, stage, slot_in, parity, flags);
 }
+void launch_xflux_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                           int slot_in, int parity) {
+  hipLaunchKernelGGL(k_xflux_batch<MODE_EXACT>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity);
+}
+void launch_ystage_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                            int slot_in, int parity, int flags) {
+  hipLaunchKernelGGL(k_ystage_batch<MODE_EXACT>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
+                     flags);
+}
+void launch_xflux_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                        int slot_in, int parity) {
+  if (mode == MODE_EXACT)
+    launch_xflux_batch_m0(inst, map, nblocks, s, stage, slot_in, parity);
+  else
+    launch_xflux_batch_m12(mode, inst, map, nblocks, s, stage, slot_in, parity);
+}
+void launch_ystage_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                         int slot_in, int parity, int flags) {
+  if (mode == MODE_EXACT)
+    launch_ystage_batch_m0(inst, map, nblocks, s, stage, slot_in, parity, flags);
+  else
+    launch_ystage_batch_m12(mode, inst, map, nblocks, s, stage, slot_in, parity, flags);
+}
+void launch_stage_end_batch(const BatchInst *inst, int n, hipStream_t s, int stage, int slot_out, int parity,
+                            int serial_ubar) {
+  hipLaunchKernelGGL(k_stage_end_batch, dim3(n), dim3(SW_T), 0, s, inst, stage, slot_out, parity, serial_ubar);
+}
+void launch_begin_run_batch(const BatchInst *inst, int n, const double *time, const double *tfinal, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin_run_batch, dim3((n + 63) / 64), dim3(64), 0, s, inst, n, time, tfinal);
+}
+void launch_batch_status(const BatchInst *inst, int n, BatchStatus *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_status, dim3((n + 63) / 64), dim3(64), 0, s, inst, n, out);
+}
 void launch_zs_edges(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
   if (mode == MODE_EXACT)
     launch_zs_edges_m0(k, b, s, slot_in, parity);

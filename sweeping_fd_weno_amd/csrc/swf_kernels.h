@@ -162,20 +162,21 @@ SWF_D void zs_store(int den, int pre, int *pair, int *red, int tid) {
 // (stage = 1, 2, 3; 0 for a stand-alone evaluation) the launch also resets the
 // stage's sweep bitmaps and flags (the stage kernel sets them), and at stage 1
 // of an inactive step keeps the next step inactive.
+// Tile (bx, by) of a gx x gy grid of x-flux tiles.
 template <int M, bool LIM>
-__global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_in, int parity) {
-  __shared__ XSmem sm;
+SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int parity, int bx, int by, int gx,
+                      int gy, XSmem &sm) {
   constexpr int CS = XSmem::CS;
   DevScalars *g = b.g;
   const int tid = threadIdx.x;
-  const bool first = blockIdx.x == 0 && blockIdx.y == 0 && tid == 0;
+  const bool first = bx == 0 && by == 0 && tid == 0;
   if (!g->active[parity] || g->status != ST_OK) {
     if (stage == 1 && first) g->active[1 - parity] = 0;
     return;
   }
   if (stage > 0) {
-    const long long nthreads = (long long)gridDim.x * gridDim.y * 256;
-    for (long long w = (long long)(blockIdx.x + blockIdx.y * gridDim.x) * 256 + tid; w < k.nwords; w += nthreads) {
+    const long long nthreads = (long long)gx * gy * 256;
+    for (long long w = (long long)(bx + by * gx) * 256 + tid; w < k.nwords; w += nthreads) {
       b.bmR[w] = 0u;
       b.bmP[w] = 0u;
     }
@@ -186,7 +187,7 @@ __global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_i
   }
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
   const double ia = 1.0 / alpha;
-  const int k0 = blockIdx.x * XT_W - XT_SHIFT, j0 = 1 + blockIdx.y * XT_H;
+  const int k0 = bx * XT_W - XT_SHIFT, j0 = 1 + by * XT_H;
   stage_tile<M, 0, XT_H * XT_PTS>(
       k, b, alpha, ia, tid,
       [&](int p, int &i, int &j) {
@@ -215,8 +216,13 @@ __global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_i
 #pragma unroll
     for (int c = 0; c < 4; ++c) b.Fx[c * k.plane + o] = F[c];
   }
-  if (LIM && stage > 0)
-    zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (blockIdx.x + blockIdx.y * gridDim.x), sm.zred, tid);
+  if (LIM && stage > 0) zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (bx + by * gx), sm.zred, tid);
+}
+
+template <int M, bool LIM>
+__global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_in, int parity) {
+  __shared__ XSmem sm;
+  xflux_tile<M, LIM>(k, b, stage, slot_in, parity, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, sm);
 }
 
 // dt of the step (ex4_sedov_2d/main.f90:696-700; ex3 :570-572)
@@ -382,9 +388,10 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
 // time after the step and the gate of the next step (the loop test of :112).
 // flags: YS_RHS_ONLY stores RHS into b.R instead (parity tests of the spatial
 // operator); YS_NO_SCALARS leaves the device scalars alone (kernel timing).
+// Stage tile (bx, by) of a grid gx tiles wide.
 template <int M, bool LIM>
-__global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_in, int parity, int flags) {
-  __shared__ YSmem sm;
+SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int parity, int flags, int bx, int by,
+                       int gx, YSmem &sm) {
   DevScalars *g = b.g;
   if (!g->active[parity] || g->status != ST_OK) return;
   const bool rhs_only = flags & YS_RHS_ONLY, scalars = !(flags & YS_NO_SCALARS);
@@ -393,7 +400,7 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
   const int tid = threadIdx.x;
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
   const double ia = 1.0 / alpha;
-  const int i0 = 1 + blockIdx.x * UB_W, j0 = 1 + blockIdx.y * UB_H;  // first cell
+  const int i0 = 1 + bx * UB_W, j0 = 1 + by * UB_H;  // first cell
   const int jf0 = j0 - 1;                                            // first interface
   const int tx = tid & (YT_W - 1), ty = tid / YT_W;
   const int i = i0 + tx, jf = jf0 + ty;
@@ -423,7 +430,7 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
     lim = iface_flux<M, 1, YT_W, YSmem::CS, LIM>(k, &sm.pr[0][ty + 2][tx], &sm.fp[0][ty + 2][tx], &sm.fn[0][ty + 2][tx],
                                             alpha, G);
   // a tile's first interface is the previous tile's last: counted there
-  if (ty == 0 && blockIdx.y > 0) lim = 0;
+  if (ty == 0 && by > 0) lim = 0;
 #pragma unroll
   for (int c = 0; c < 4; ++c) sm.G[c][ty][tx] = G[c];
   __syncthreads();
@@ -497,7 +504,7 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
   }
   if (rhs_only) return;
   const double am = block_max<256>(anew, sm.red, tid);
-  if (tid == 0 && scalars) b.apart[blockIdx.x + blockIdx.y * gridDim.x] = am;
+  if (tid == 0 && scalars) b.apart[bx + by * gx] = am;
   if (usum) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -511,14 +518,17 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_
       for (int c = 0; c < 4; ++c) sm.ured[c][tid >> 6] = wu[c];
     }
     __syncthreads();
-    if (tid < 4) {
-      const int blk = blockIdx.x + blockIdx.y * gridDim.x;
-      b.upart[4 * blk + tid] = (sm.ured[tid][0] + sm.ured[tid][1]) + (sm.ured[tid][2] + sm.ured[tid][3]);
-    }
+    if (tid < 4)
+      b.upart[4 * (bx + by * gx) + tid] = (sm.ured[tid][0] + sm.ured[tid][1]) + (sm.ured[tid][2] + sm.ured[tid][3]);
   }
   if (LIM && scalars)
-    zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (xf_gx(k) * xf_gy(k) + blockIdx.x + blockIdx.y * gridDim.x),
-             sm.zred, tid);
+    zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (xf_gx(k) * xf_gy(k) + bx + by * gx), sm.zred, tid);
+}
+
+template <int M, bool LIM>
+__global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_in, int parity, int flags) {
+  __shared__ YSmem sm;
+  ystage_tile<M, LIM>(k, b, stage, slot_in, parity, flags, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
 // One limited half flux outside the flux tiles (use_limiter == 1): the half of
@@ -599,6 +609,42 @@ __global__ __launch_bounds__(256) void k_zs_edges(KCfg k, Bufs b, int slot_in, i
   zs_store(lim & 1, (lim >> 1) & 1, b.zs + 2 * (xf_gx(k) * xf_gy(k) + ub_gx(k) * ub_gy(k) + blockIdx.x), zred, tid);
 }
 
+// ---------------------------------------------------------------------------
+// Batches: independent instances (own grid, configuration, buffers, time and
+// sweep) advanced together, one launch per kernel of an RK stage for all of
+// them.  A block of a batch launch finds its instance and tile in a map built by
+// the host (the schedule: the order of the tiles of all instances).
+// ---------------------------------------------------------------------------
+
+// buffers of stage s (1, 2, 3): stencils from S0, S1, S2, results into S1, S2, S0
+SWF_D Bufs batch_stage_bufs(const BatchInst &I, int s) {
+  Bufs b = I.b;
+  b.Uin = (s == 1) ? I.b.U0 : (s == 2) ? I.S1 : I.S2;
+  b.U = (s == 1) ? I.S1 : (s == 2) ? I.S2 : I.b.U0;
+  return b;
+}
+
+// map[block] = (instance, tile of that instance)
+template <int M>
+__global__ SWF_FLUX_BOUNDS(M) void k_xflux_batch(const BatchInst *inst, const int2 *map, int stage, int slot_in,
+                                                 int parity) {
+  __shared__ XSmem sm;
+  const int2 e = map[blockIdx.x];
+  const BatchInst &I = inst[e.x];
+  const int gx = xf_gx(I.k), gy = xf_gy(I.k);
+  xflux_tile<M, false>(I.k, batch_stage_bufs(I, stage), stage, slot_in, parity, e.y % gx, e.y / gx, gx, gy, sm);
+}
+
+template <int M>
+__global__ SWF_FLUX_BOUNDS(M) void k_ystage_batch(const BatchInst *inst, const int2 *map, int stage, int slot_in,
+                                                  int parity, int flags) {
+  __shared__ YSmem sm;
+  const int2 e = map[blockIdx.x];
+  const BatchInst &I = inst[e.x];
+  const int gx = ub_gx(I.k);
+  ystage_tile<M, false>(I.k, batch_stage_bufs(I, stage), stage, slot_in, parity, flags, e.y % gx, e.y / gx, gx, sm);
+}
+
 #ifdef SWF_COMMON_KERNELS
 // Mode-independent grid kernels, compiled once (swf_k_exact.hip).
 
@@ -607,11 +653,17 @@ __global__ __launch_bounds__(256) void k_zs_edges(KCfg k, Bufs b, int slot_in, i
 // (recomputed for the tiles a sweep changed) and the prescribed boundary states.
 // The gate, the flag and the partials are loaded together (one memory round
 // trip when no sweep is needed).
-__global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, int slot_out, int parity,
-                                                    int serial_ubar) {
-  __shared__ SweepShared<SW_T> sh;
-  __shared__ double red[SW_T / 64];
-  __shared__ long long zc[2 * (SW_T / 64)];
+struct StageEndShared {
+  SweepShared<SW_T> sh;
+  double red[SW_T / 64];
+  long long zc[2 * (SW_T / 64)];
+};
+
+SWF_D void stage_end_body(const KCfg &k, const Bufs &b, int stage, int slot_out, int parity, int serial_ubar,
+                          StageEndShared &se) {
+  SweepShared<SW_T> &sh = se.sh;
+  double *red = se.red;
+  long long *zc = se.zc;
   DevScalars *g = b.g;
   const int tid = threadIdx.x;
   const int nblk = ub_gx(k) * ub_gy(k);
@@ -656,6 +708,42 @@ __global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, i
     // sweeps tend to follow sweeps: then the next stage kernel sums weight*u per tile
     g->ubar_tiles = (k.use_limiter == 2 && flagged && !serial_ubar) ? 1 : 0;
   }
+}
+
+__global__ __launch_bounds__(SW_T) void k_stage_end(KCfg k, Bufs b, int stage, int slot_out, int parity,
+                                                    int serial_ubar) {
+  __shared__ StageEndShared se;
+  stage_end_body(k, b, stage, slot_out, parity, serial_ubar, se);
+}
+
+// one workgroup per instance
+__global__ __launch_bounds__(SW_T) void k_stage_end_batch(const BatchInst *inst, int stage, int slot_out, int parity,
+                                                          int serial_ubar) {
+  __shared__ StageEndShared se;
+  const BatchInst &I = inst[blockIdx.x];
+  stage_end_body(I.k, batch_stage_bufs(I, stage), stage, slot_out, parity, serial_ubar, se);
+}
+
+// the time loop's start for every instance (k_begin_run of each)
+__global__ void k_begin_run_batch(const BatchInst *inst, int n, const double *time, const double *tfinal) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  DevScalars *g = inst[i].b.g;
+  g->time[0] = time[i];
+  g->tfinal = tfinal[i];
+  g->active[0] = time_continue_q(time[i], tfinal[i]);
+  g->steps_done = 0;
+}
+
+__global__ void k_batch_status(const BatchInst *inst, int n, BatchStatus *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevScalars *g = inst[i].b.g;
+  const long long s = g->steps_done;
+  out[i].time = g->time[s & 1];
+  out[i].steps_done = s;
+  out[i].status = g->status;
+  out[i].active = g->active[s & 1];
 }
 
 // alpha max over every cell (ghosts included), for a freshly uploaded state

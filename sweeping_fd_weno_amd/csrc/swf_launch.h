@@ -25,6 +25,24 @@ void launch_ystage_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, in
 void launch_zs_edges_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity);
 void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, hipStream_t s);
 
+// batch kernels: blocks of the x-flux / stage launches from the schedule maps
+void launch_xflux_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                        int slot_in, int parity);
+void launch_ystage_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                         int slot_in, int parity, int flags);
+void launch_xflux_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                           int slot_in, int parity);
+void launch_ystage_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                            int slot_in, int parity, int flags);
+void launch_xflux_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                            int slot_in, int parity);
+void launch_ystage_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                             int slot_in, int parity, int flags);
+void launch_stage_end_batch(const BatchInst *inst, int n, hipStream_t s, int stage, int slot_out, int parity,
+                            int serial_ubar);
+void launch_begin_run_batch(const BatchInst *inst, int n, const double *time, const double *tfinal, hipStream_t s);
+void launch_batch_status(const BatchInst *inst, int n, BatchStatus *out, hipStream_t s);
+
 // mode-independent kernels (swf_k_exact.hip)
 void launch_sweep(const KCfg &k, const Bufs &b, hipStream_t s, int serial_ubar);
 void launch_stage_end(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_out, int parity,

@@ -115,6 +115,32 @@ def crafted_sweep(n, nbad, seed, chain=0):
             "stats": {k: [so[k], se[k]] for k in STATS}}
 
 
+def batch_vs_single(prec, nsteps):
+    """A batch of a Sedov, a vortex and a full jet instance against each run alone
+    (same precision): every field bitwise, the same time and steps."""
+    cfgs = [S.sedov_config(40, prec), S.vortex_config(36, prec), S.jet_config(48, 24, prec, cfl=0.5, half=False)]
+    inits = [S.init_sedov(cfgs[0]), S.init_vortex(cfgs[1]), S.init_jet(cfgs[2])]
+    tf = np.array([1.0, 0.01, 0.001])
+    bt = S.Batch(cfgs)
+    for i in range(3):
+        bt.instance(i).upload(inits[i])
+        bt.instance(i).bc()
+    tb, nb, stb = bt.advance(np.zeros(3), tf, max_steps=nsteps)
+    out = {"rc": int(stb.max()), "steps": nb.tolist(), "differ": 0, "time_equal": True}
+    for i in range(3):
+        s = S.Solver(cfgs[i])
+        s.upload(inits[i])
+        s.bc()
+        ts, ns = s.advance(0.0, tf[i], max_steps=nsteps)
+        a, b = bt.instance(i).download(), s.download()
+        out["differ"] += int((a.view(np.int64) != b.view(np.int64)).sum())
+        out["time_equal"] = bool(out["time_equal"] and ts == tb[i] and ns == nb[i])
+        out.setdefault("sweeps", []).append([s.stats()["sweep_count"], bt.instance(i).stats()["sweep_count"]])
+        s.close()
+    bt.close()
+    return out
+
+
 def main(name):
     if name == "sedov_sweeps":
         cfg = S.sedov_config(32, "fp64_exact")
@@ -134,6 +160,8 @@ def main(name):
     elif name.startswith("crafted_sweep_"):
         n, nbad, *rest = (int(v) for v in name[len("crafted_sweep_"):].split("_"))
         out = crafted_sweep(n, nbad, 7, rest[0] if rest else 0)
+    elif name.startswith("batch_"):
+        out = batch_vs_single(name[len("batch_"):], 25)
     elif name.startswith("first_dt_"):
         out = first_dt(S.jet_config(40, 20, name[len("first_dt_"):], cfl=0.5, half=False))
     else:

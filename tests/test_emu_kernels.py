@@ -16,6 +16,7 @@ arithmetic and control flow, not races; the GPU tests check the real device.
   flagged snake positions (walks shorter and longer than a walk record):
   fields bitwise, statistics equal.
 - FP64 / FP32-bulk full jet 64x32 with sweeps (u_bar from per-tile sums).
+- A batch of three instances against each run alone: bitwise.
 - After upload of the ambient jet and swf_bc, the first dt sees the Mach 2000
   inflow ghosts (alpha ~ 800) in every precision mode.
 """
@@ -74,6 +75,19 @@ def test_emulated_fast_modes_jet_sweeps(emu_lib, prec, tol):
     assert r["stats"]["sweep_count"][0] == r["stats"]["sweep_count"][1] >= 20
     assert r["stats"]["total_sweep"][0] == r["stats"]["total_sweep"][1]
     assert r["l1"] < tol
+
+
+@pytest.mark.parametrize("prec", ["fp64_exact", "fp32_bulk"])
+def test_emulated_batch_matches_single_runs(emu_lib, prec):
+    """A batch of Sedov 40^2, vortex 36^2 and full jet 48x24 (the vortex ends after
+    one step while the others go on): every instance bitwise equal to its run alone,
+    with the same steps, time and sweeps."""
+    r = scenario("batch_" + prec)
+    assert r["rc"] == 0 and r["steps"] == [25, 1, 25]
+    assert r["differ"] == 0 and r["time_equal"]
+    for o, e in r["sweeps"]:
+        assert o == e
+    assert r["sweeps"][2][0] > 10
 
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp64", "fp32_bulk"])
