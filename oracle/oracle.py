@@ -5,7 +5,8 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and th
 imports this module.
 
 The oracle restates ex4_sedov_2d/main.f90 (Sedov), ex3_vortex_accuracy_2d/main.f90
-(vortex) and ex5_mach2000_jet/main.f90 (jet) of the reference; see swf_oracle.h.
+(vortex), ex5_mach2000_jet/main.f90 (jet) and ex2_sedov_1d/main.f90 (the 1D program)
+of the reference; see swf_oracle.h.
 """
 from __future__ import annotations
 
@@ -41,6 +42,17 @@ class SwoStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "sweep_count", "den_count", "pre_count", "total_sweep", "max_sweep", "total_neg", "max_neg",
         "total_pos", "max_pos", "stage_sweeps")]
+
+
+class Swo1dConfig(C.Structure):
+    _fields_ = [("n", C.c_int32), ("xlo", C.c_double), ("dx", C.c_double), ("gamma", C.c_double),
+                ("cfl", C.c_double), ("eps", C.c_double), ("use_limiter", C.c_int32), ("swp_num", C.c_int32)]
+
+
+class Swo1dStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "sweep_count", "den_count", "pre_count", "total_sweep", "max_sweep", "total_neg", "max_neg",
+        "total_pos", "max_pos", "steps")]
 
 
 _lib = None
@@ -86,6 +98,16 @@ def lib():
         L.swo_init_vortex.argtypes = [P(SwoConfig), C.c_double, C.c_double, dp]
         L.swo_vortex_exact.argtypes = [P(SwoConfig), C.c_double, C.c_double, dp]
         L.swo_init_jet.argtypes = [P(SwoConfig), dp]
+        L.swo1d_create.restype = C.c_void_p
+        L.swo1d_create.argtypes = [P(Swo1dConfig)]
+        L.swo1d_destroy.argtypes = [C.c_void_p]
+        L.swo1d_set_state.argtypes = [C.c_void_p, dp]
+        L.swo1d_get_state.argtypes = [C.c_void_p, dp]
+        L.swo1d_get_stats.argtypes = [C.c_void_p, P(Swo1dStats)]
+        L.swo1d_track.restype = C.c_int64
+        L.swo1d_track.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int64), C.c_int64]
+        L.swo1d_init_sedov.argtypes = [P(Swo1dConfig), C.c_int, C.c_double, dp]
+        L.swo1d_run.argtypes = [C.c_void_p, P(C.c_double), C.c_double, C.c_int64, P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -258,3 +280,77 @@ def init_jet(cfg: SwoConfig) -> np.ndarray:
     u = np.zeros(state_shape(cfg))
     lib().swo_init_jet(C.byref(cfg), _dp(u))
     return u
+
+
+# ---------------------------------------------------------------------------
+# the 1D program (ex2_sedov_1d/main.f90)
+# ---------------------------------------------------------------------------
+
+def sedov1d_config(n: int = 800, cfl: float = 0.6, swp_num: int = 2, use_limiter: int = 2) -> Swo1dConfig:
+    """ex2_sedov_1d/main.f90:4-7,399-405: [-2,2], dx = 4/n, gamma 1.4, eps 1d-13.
+
+    The shipped program has cfl 1.2 and swp_num 50; the 1D benchmark case sets 0.6 and 2.
+    """
+    return Swo1dConfig(n=n, xlo=-2.0, dx=(2.0 - (-2.0)) / n, gamma=1.4, cfl=cfl, eps=1e-13,
+                       use_limiter=use_limiter, swp_num=swp_num)
+
+
+def state1d_shape(cfg: Swo1dConfig) -> tuple:
+    return (cfg.n + 6, 3)
+
+
+def init_sedov1d(cfg: Swo1dConfig, shipped_energy: bool = False, e_amb: float = 1e-12) -> np.ndarray:
+    """initdata (:398-420); the shipped program's ambient E is 1d-9/gamma1 (shipped_energy)."""
+    u = np.zeros(state1d_shape(cfg))
+    lib().swo1d_init_sedov(C.byref(cfg), int(shipped_energy), e_amb, _dp(u))
+    return u
+
+
+class Oracle1D:
+    """One context of the 1D program: state u(3, -2:n+3) stored as rows (n+6, 3)."""
+
+    def __init__(self, cfg: Swo1dConfig):
+        self.cfg = cfg
+        self._p = lib().swo1d_create(C.byref(cfg))
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().swo1d_destroy(self._p)
+            self._p = None
+
+    def set_state(self, u: np.ndarray) -> None:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == state1d_shape(self.cfg)
+        lib().swo1d_set_state(self._p, _dp(u))
+
+    def state(self) -> np.ndarray:
+        u = np.empty(state1d_shape(self.cfg))
+        lib().swo1d_get_state(self._p, _dp(u))
+        return u
+
+    def stats(self) -> dict:
+        s = Swo1dStats()
+        lib().swo1d_get_stats(self._p, C.byref(s))
+        return {n: getattr(s, n) for n, _ in Swo1dStats._fields_}
+
+    def track(self):
+        n = lib().swo1d_track(self._p, None, None, None, 0)
+        q = np.zeros(max(n, 1), np.int64)
+        ng = np.zeros_like(q)
+        ps = np.zeros_like(q)
+        P = C.POINTER(C.c_int64)
+        lib().swo1d_track(self._p, q.ctypes.data_as(P), ng.ctypes.data_as(P), ps.ctypes.data_as(P), n)
+        return q[:n], ng[:n], ps[:n]
+
+    def run(self, t: float, tfinal: float, max_steps: int = 1 << 40):
+        """Reference loop `DO WHILE (time <= time_terminal - 1d-10)` (:38-45)."""
+        tt = C.c_double(t)
+        k = C.c_int64(0)
+        lib().swo1d_run(self._p, C.byref(tt), tfinal, max_steps, C.byref(k))
+        return tt.value, k.value
+
+
+def pressure1d(u: np.ndarray, gamma: float = 1.4) -> np.ndarray:
+    """The pre.dat column: gamma1*(E - 0.5*m**2/rho) (:481)."""
+    g1 = gamma - 1.0
+    return g1 * (u[:, 2] - (0.5 * (u[:, 1] * u[:, 1])) / u[:, 0])

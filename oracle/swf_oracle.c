@@ -953,3 +953,308 @@ void swo_init_jet(const swo_config *cfg, double *u) {
             UI(3, i, j) = cfg->amb_p / g1;
         }
 }
+
+/* ========================================================================= */
+/* The 1D program (ex2_sedov_1d/main.f90), built with -fdefault-real-8: the   */
+/* state u(3, -2:n+3), local (cell) eigenvectors, no boundary conditions (the */
+/* ghost cells keep their initial values: RHS is 0 there), sweep over every   */
+/* cell including the ghosts, unconditional scalar (density) sweep, no stop   */
+/* on a sweep failure.                                                         */
+/* ========================================================================= */
+
+struct swo1d_ctx {
+    swo1d_config cfg;
+    int n, N;            /* interior cells; cells with ghosts (n + 6) */
+    double gamma1, weight, alphamax, dt;
+    int postag;
+    double *u, *u0, *rhs, *hp, *hn, *hpf, *hnf, *flus;
+    swo1d_stats st;
+    int64_t *trk_q, *trk_n, *trk_p, trk_cap;
+    int64_t t_pos, t_neg;
+};
+
+#define U1(a, c, i) ((a)[3 * ((i) + 2) + (c)])
+
+swo1d_ctx *swo1d_create(const swo1d_config *cfg) {
+    swo1d_ctx *C = (swo1d_ctx *)calloc(1, sizeof(swo1d_ctx));
+    C->cfg = *cfg;
+    C->n = cfg->n;
+    C->N = cfg->n + 6;
+    C->gamma1 = cfg->gamma - 1.;
+    /* weights(i) = 1/(nSteps + 6), nSteps = n as REAL (ex2_sedov_1d/main.f90:5,34-36) */
+    C->weight = 1. / ((double)cfg->n + 6.);
+    double **arrs[] = {&C->u, &C->u0, &C->rhs, &C->hp, &C->hn, &C->hpf, &C->hnf, &C->flus};
+    for (int a = 0; a < 8; ++a) *arrs[a] = (double *)calloc((size_t)3 * C->N, sizeof(double));
+    C->trk_cap = 1 << 16;
+    C->trk_q = (int64_t *)calloc(C->trk_cap, sizeof(int64_t));
+    C->trk_n = (int64_t *)calloc(C->trk_cap, sizeof(int64_t));
+    C->trk_p = (int64_t *)calloc(C->trk_cap, sizeof(int64_t));
+    return C;
+}
+
+void swo1d_destroy(swo1d_ctx *C) {
+    if (!C) return;
+    free(C->u); free(C->u0); free(C->rhs); free(C->hp); free(C->hn); free(C->hpf); free(C->hnf); free(C->flus);
+    free(C->trk_q); free(C->trk_n); free(C->trk_p);
+    free(C);
+}
+
+void swo1d_set_state(swo1d_ctx *C, const double *u) { memcpy(C->u, u, sizeof(double) * 3 * C->N); }
+void swo1d_get_state(const swo1d_ctx *C, double *u) { memcpy(u, C->u, sizeof(double) * 3 * C->N); }
+void swo1d_get_stats(const swo1d_ctx *C, swo1d_stats *s) { *s = C->st; }
+int64_t swo1d_track(const swo1d_ctx *C, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap) {
+    const int64_t m = C->st.sweep_count < cap ? C->st.sweep_count : cap;
+    for (int64_t k = 0; k < m && k < C->trk_cap; ++k) {
+        q[k] = C->trk_q[k];
+        neg[k] = C->trk_n[k];
+        pos[k] = C->trk_p[k];
+    }
+    return C->st.sweep_count;
+}
+
+/* initdata (:398-420): rho = 1, m = 0, E = e_amb everywhere (the shipped program's
+ * 1d-9/gamma1 in binary128 when shipped_energy), E(n/2+1) = 3200000.0/dx */
+void swo1d_init_sedov(const swo1d_config *cfg, int shipped_energy, double e_amb, double *u) {
+    const int N = cfg->n + 6;
+    const double g1 = cfg->gamma - 1.;
+    const double e = shipped_energy ? D(1e-9Q / Q(g1)) : e_amb;
+    for (int i = 0; i < N; ++i) {
+        u[3 * i + 0] = 1.;
+        u[3 * i + 1] = 0.;
+        u[3 * i + 2] = e;
+    }
+    u[3 * (cfg->n / 2 + 1 + 2) + 2] = 3200000.0 / cfg->dx;
+}
+
+/* findalpha (:285-296) */
+static double alpha1d(const swo1d_ctx *C) {
+    double amax = 0.;
+    for (int i = -2; i <= C->n + 3; ++i) {
+        const double den = U1(C->u, 0, i);
+        const double pres = C->gamma1 * (U1(C->u, 2, i) - ((U1(C->u, 1, i) * U1(C->u, 1, i)) / U1(C->u, 0, i)) / 2.);
+        const double sound = sqrt((C->cfg.gamma * pres) / den);
+        amax = fmax2(amax, fabs(U1(C->u, 1, i) / U1(C->u, 0, i)) + sound);
+    }
+    return amax;
+}
+
+/* fflux (:59-73): vx = m/rho when rho > 1d-8 (binary128 literal), else 0 */
+static void flux1d(double g1, double rho, double mx, double e, double f[3]) {
+    const double vx = (Q(rho) > 1e-8Q) ? mx / rho : 0.;
+    const double pres = g1 * (e - (0.5 * rho) * (vx * vx));
+    f[0] = mx;
+    f[1] = rho * (vx * vx) + pres;
+    f[2] = (e + pres) * vx;
+}
+
+/* checkpositivity (:380-394) over the interior */
+static int pos1d(const swo1d_ctx *C) {
+    int tag = 0;
+    for (int m = 1; m <= C->n; ++m) {
+        if (U1(C->u, 0, m) <= C->cfg.eps) tag = 1;
+        if ((U1(C->u, 2, m) - ((U1(C->u, 1, m) * U1(C->u, 1, m)) / U1(C->u, 0, m)) / 2.) <= C->cfg.eps) tag = 1;
+    }
+    return tag;
+}
+
+/* wenofd (:75-244) without the use_limiter = 1 branch: split, then per cell m in
+ * 0..n+1 the cell's own eigenvectors, WENO of each characteristic field */
+static void wenofd1d(swo1d_ctx *C) {
+    const double g1 = C->gamma1;
+    C->alphamax = alpha1d(C);
+    for (int i = -2; i <= C->n + 3; ++i) {
+        double f[3];
+        flux1d(g1, U1(C->u, 0, i), U1(C->u, 1, i), U1(C->u, 2, i), f);
+        for (int c = 0; c < 3; ++c) {
+            U1(C->hp, c, i) = 0.5 * (U1(C->u, c, i) + f[c] / C->alphamax);
+            U1(C->hn, c, i) = 0.5 * (U1(C->u, c, i) - f[c] / C->alphamax);
+        }
+    }
+    C->postag = pos1d(C);
+    for (int m = 0; m <= C->n + 1; ++m) {
+        const double r = U1(C->u, 0, m), mx = U1(C->u, 1, m), e = U1(C->u, 2, m);
+        const double vxm = mx / r;
+        const double hm = (g1 * (e - (0.5 * (mx * mx)) / r) + e) / r;
+        const double qm = (0.5 * vxm) * vxm;
+        const double cm = sqrt(g1 * (hm - qm));
+        double t0 = vxm * cm;
+        const double evr[3][3] = {{1.0, 1.0, 1.0}, {vxm - cm, vxm, vxm + cm}, {hm - t0, qm, hm + t0}};
+        const double rcm = 1. / cm;
+        const double b1 = (g1 * rcm) * rcm, b2 = qm * b1;
+        t0 = vxm * rcm;
+        const double t1 = b1 * vxm, t2 = 0.5 * b1;
+        const double evl[3][3] = {{0.5 * (b2 + t0), -0.5 * (t1 + rcm), t2},
+                                  {1. - b2, t1, -b1},
+                                  {0.5 * (b2 - t0), -0.5 * (t1 - rcm), t2}};
+        double cup[3][5], cun[3][5];
+        for (int j = -2; j <= 2; ++j)
+            for (int k = 0; k < 3; ++k) {
+                cup[k][j + 2] = (evl[k][0] * U1(C->hp, 0, m + j) + evl[k][1] * U1(C->hp, 1, m + j)) +
+                                evl[k][2] * U1(C->hp, 2, m + j);
+                cun[k][j + 2] = (evl[k][0] * U1(C->hn, 0, m + j) + evl[k][1] * U1(C->hn, 1, m + j)) +
+                                evl[k][2] * U1(C->hn, 2, m + j);
+            }
+        double ur[3], ul[3];
+        for (int k = 0; k < 3; ++k) {
+            ur[k] = weno_right(&cup[k][2]);
+            ul[k] = weno_left(&cun[k][2]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            U1(C->hpf, k, m) = (evr[k][0] * ur[0] + evr[k][1] * ur[1]) + evr[k][2] * ur[2];
+            U1(C->hnf, k, m) = (evr[k][0] * ul[0] + evr[k][1] * ul[1]) + evr[k][2] * ul[2];
+        }
+    }
+}
+
+/* scheme (:263-280) */
+static void scheme1d(swo1d_ctx *C) {
+    for (int i = 1; i <= C->n + 1; ++i)
+        for (int c = 0; c < 3; ++c) U1(C->flus, c, i) = C->alphamax * (U1(C->hpf, c, i - 1) - U1(C->hnf, c, i));
+    memset(C->rhs, 0, sizeof(double) * 3 * C->N);
+    for (int i = 1; i <= C->n; ++i)
+        for (int c = 0; c < 3; ++c) U1(C->rhs, c, i) = -(U1(C->flus, c, i + 1) - U1(C->flus, c, i)) / C->cfg.dx;
+}
+
+/* nod_lim (:488-532): note the parenthesisation 0.5*(m**2/rho) */
+static void nodlim1d(swo1d_ctx *C, const double v1[3], const double v2[3], const double vb[3], double out[3]) {
+    const double g1 = C->gamma1, eps = C->cfg.eps;
+    const double p1 = g1 * (v1[2] - 0.5 * ((v1[1] * v1[1]) / v1[0]));
+    const double p2 = g1 * (v2[2] - 0.5 * ((v2[1] * v2[1]) / v2[0]));
+    double t;
+    if (p2 > eps) {
+        t = (p1 - eps) / (p1 - p2);
+        C->t_pos++;
+    } else {
+        const double pb = g1 * (vb[2] - 0.5 * ((vb[1] * vb[1]) / vb[0]));
+        const double t_1 = (p1 - eps) / (p1 - pb);
+        const double d12 = sqrt(((v2[2] - v1[2]) * (v2[2] - v1[2]) + (v2[1] - v1[1]) * (v2[1] - v1[1])) +
+                                (v2[0] - v1[0]) * (v2[0] - v1[0]));
+        const double d1b = sqrt(((vb[2] - v1[2]) * (vb[2] - v1[2]) + (vb[1] - v1[1]) * (vb[1] - v1[1])) +
+                                (vb[0] - v1[0]) * (vb[0] - v1[0]));
+        t = fmin2((t_1 * d1b) / d12, 0.25);
+        C->t_neg++;
+    }
+    for (int c = 0; c < 3; ++c) out[c] = (1 - t) * v1[c] + t * v2[c];
+}
+
+/* the sweep's flag and pressure: gamma1*(E - 0.5*(m**2)/rho) (:562, :645) */
+static inline double pres1d(double g1, const double *v) { return g1 * (v[2] - (0.5 * (v[1] * v[1])) / v[0]); }
+
+/* scalar_sweep (:729-789), unconditional inside sweep_lim; delta = 1d-13 */
+static void scalar1d(swo1d_ctx *C) {
+    const double delta = 1e-13, wr = C->weight / C->weight;
+    const int n = C->n;
+    for (int j = -2; j <= n + 2; ++j) {
+        const double vj = U1(C->u, 0, j), vjp1 = U1(C->u, 0, j + 1);
+        if (vj < delta) {
+            U1(C->u, 0, j + 1) = vjp1 + wr * (vj - delta);
+            U1(C->u, 0, j) = delta;
+        }
+    }
+    for (int j = n + 3; j >= -1; --j) {
+        const double vj = U1(C->u, 0, j), vjm1 = U1(C->u, 0, j - 1);
+        if (vj < delta) {
+            U1(C->u, 0, j - 1) = vjm1 + wr * (vj - delta);
+            U1(C->u, 0, j) = delta;
+        }
+    }
+    C->st.den_count++;
+}
+
+/* sweep_lim (:538-727) */
+static void sweep1d(swo1d_ctx *C) {
+    const double g1 = C->gamma1, eps = C->cfg.eps, w = C->weight, wr = w / w;
+    const int n = C->n;
+    scalar1d(C);
+    double ub[3] = {0., 0., 0.};  /* dot_product(rho_t, weights) ... in index order */
+    for (int i = -2; i <= n + 3; ++i)
+        for (int c = 0; c < 3; ++c) ub[c] = ub[c] + U1(C->u, c, i) * w;
+    C->st.sweep_count++;
+    const int64_t sc = C->st.sweep_count;
+    C->t_pos = C->t_neg = 0;
+    for (int q = 1; q <= C->cfg.swp_num; ++q) {
+        for (int j = -2; j <= n + 2; ++j) {
+            double vj[3] = {U1(C->u, 0, j), U1(C->u, 1, j), U1(C->u, 2, j)};
+            if (pres1d(g1, vj) <= eps || vj[0] <= eps) {
+                double vp[3] = {U1(C->u, 0, j + 1), U1(C->u, 1, j + 1), U1(C->u, 2, j + 1)}, vt[3];
+                nodlim1d(C, vj, vp, ub, vt);
+                for (int c = 0; c < 3; ++c) {
+                    U1(C->u, c, j) = vt[c];
+                    U1(C->u, c, j + 1) = vp[c] + wr * (vj[c] - vt[c]);
+                }
+            }
+        }
+        for (int j = n + 3; j >= -1; --j) {
+            double vj[3] = {U1(C->u, 0, j), U1(C->u, 1, j), U1(C->u, 2, j)};
+            if (pres1d(g1, vj) <= eps || vj[0] <= eps) {
+                double vp[3] = {U1(C->u, 0, j - 1), U1(C->u, 1, j - 1), U1(C->u, 2, j - 1)}, vt[3];
+                nodlim1d(C, vj, vp, ub, vt);
+                for (int c = 0; c < 3; ++c) {
+                    U1(C->u, c, j) = vt[c];
+                    U1(C->u, c, j - 1) = vp[c] + wr * (vj[c] - vt[c]);
+                }
+            }
+        }
+        double pmin = 1e20, rmin = 1e20;
+        for (int i = -2; i <= n + 3; ++i) {
+            const double v[3] = {U1(C->u, 0, i), U1(C->u, 1, i), U1(C->u, 2, i)};
+            pmin = fmin2(pres1d(g1, v), pmin);
+            rmin = fmin2(v[0], rmin);
+        }
+        C->st.pre_count++;
+        if (pmin > eps && rmin > eps) {
+            C->postag = 0;
+            if (sc - 1 < C->trk_cap) {
+                C->trk_q[sc - 1] = q;
+                C->trk_n[sc - 1] = C->t_neg;
+                C->trk_p[sc - 1] = C->t_pos;
+            }
+            C->st.total_sweep += q;
+            C->st.total_neg += C->t_neg;
+            C->st.total_pos += C->t_pos;
+            if (q > C->st.max_sweep) C->st.max_sweep = q;
+            if (C->t_neg > C->st.max_neg) C->st.max_neg = C->t_neg;
+            if (C->t_pos > C->st.max_pos) C->st.max_pos = C->t_pos;
+            return;
+        }
+    }
+}
+
+/* rk3 (:298-332): dt = dx*cfl/alphamax, no clamp to the final time */
+static void rk3_1d(swo1d_ctx *C) {
+    const int N3 = 3 * C->N;
+    for (int s = 1; s <= 3; ++s) {
+        wenofd1d(C);
+        if (s == 1) {
+            C->dt = (C->cfg.dx * C->cfg.cfl) / C->alphamax;
+            C->postag = 0;
+            memcpy(C->u0, C->u, sizeof(double) * N3);
+        }
+        scheme1d(C);
+        for (int k = 0; k < N3; ++k) {
+            const double x = C->u[k] + C->dt * C->rhs[k];
+            if (s == 1)
+                C->u[k] = x;
+            else if (s == 2)
+                C->u[k] = D(Q(0.75) * Q(C->u0[k]) + Q(0.25) * Q(x));
+            else
+                C->u[k] = D((Q(C->u0[k]) + Q(2) * Q(x)) / Q(3));
+        }
+        C->postag = pos1d(C);
+        if (C->postag == 1 && C->cfg.use_limiter == 2) sweep1d(C);
+    }
+}
+
+int swo1d_run(swo1d_ctx *C, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out) {
+    double t = *time_inout;
+    int64_t k = 0;
+    while (k < max_steps && swo_time_continue(t, tfinal)) {
+        rk3_1d(C);
+        t = t + C->dt;
+        ++k;
+    }
+    C->st.steps += k;
+    *time_inout = t;
+    if (steps_out) *steps_out = k;
+    return 0;
+}

@@ -10,6 +10,7 @@
  *   ex3_vortex_accuracy_2d/main.f90 (periodic vortex, alpha applied in `scheme`)
  *   ex5_mach2000_jet/main.f90 (jet inflow, `>=` sweep success test)
  *   ex4_sedov_2d/weno5.f90:86-146 (weno_left / weno_right)
+ *   ex2_sedov_1d/main.f90   (the 1D program: swo1d_*)
  * Every `d`-exponent literal of the reference is REAL(16) under -fdefault-real-8,
  * so those sites are evaluated here in __float128 and rounded once to double.
  *
@@ -97,6 +98,27 @@ void swo_init_sedov(const swo_config *cfg, double e_corner_num, double e_amb, do
 void swo_init_vortex(const swo_config *cfg, double mag, double t, double *u);
 void swo_vortex_exact(const swo_config *cfg, double mag, double t, double *u);
 void swo_init_jet(const swo_config *cfg, double *u);
+
+/* The 1D program ex2_sedov_1d/main.f90 (built with -fdefault-real-8): u(3, -2:n+3). */
+typedef struct swo1d_config {
+    int32_t n;
+    double xlo, dx, gamma, cfl, eps;
+    int32_t use_limiter;  /* 2 = sweeping (scalar + pressure sweeps), 0 = none */
+    int32_t swp_num;
+} swo1d_config;
+typedef struct swo1d_stats {
+    int64_t sweep_count, den_count, pre_count, total_sweep, max_sweep, total_neg, max_neg, total_pos, max_pos,
+        steps;
+} swo1d_stats;
+typedef struct swo1d_ctx swo1d_ctx;
+swo1d_ctx *swo1d_create(const swo1d_config *cfg);
+void swo1d_destroy(swo1d_ctx *ctx);
+void swo1d_set_state(swo1d_ctx *ctx, const double *u);
+void swo1d_get_state(const swo1d_ctx *ctx, double *u);
+void swo1d_get_stats(const swo1d_ctx *ctx, swo1d_stats *out);
+int64_t swo1d_track(const swo1d_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap);
+void swo1d_init_sedov(const swo1d_config *cfg, int shipped_energy, double e_amb, double *u);
+int swo1d_run(swo1d_ctx *ctx, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,12 @@ oracle/_ref/, run there, and only its OUTPUTS are stored here:
   ref_ex5_jet.json        ex5_mach2000_jet as shipped (800x200, T = 0.001), when its
                           (hour-long) run has finished: log-field hashes + statistics.
 
-Usage:  python tests/golden/make_golden.py [weno] [ex3] [ex5]
+  ref_ex2_sedov1d.json    ex2_sedov_1d (800 points, T = 0.001) in three builds: as shipped
+                          (cfl 1.2), the 1D benchmark case (cfl 0.6, ambient E 1e-12,
+                          swp_num 2) and a 40-step cfl 1.5 run stopped by the program's
+                          max_iter branch: SHA-256 of den/pre/vel, samples, statistics.
+
+Usage:  python tests/golden/make_golden.py [weno] [ex3] [ex3_l1] [ex2] [ex5]
 """
 from __future__ import annotations
 
@@ -154,6 +159,41 @@ def make_ex3_l1() -> None:
         json.dump(out, f, indent=1)
 
 
+def make_ex2() -> None:
+    """The three ex2 builds of oracle/Makefile (ex2, ex2_cfg1, ex2_stress)."""
+    cases = {
+        "shipped": ("ex2", "as shipped: n=800, cfl=1.2, swp_num=50, E_amb=1d-9/gamma1, T=0.001"),
+        "cfg1": ("ex2_cfg1", "cfl=0.6, swp_num=2, E_amb=1d-12, otherwise as shipped"),
+        "stress": ("ex2_stress", "cfl=1.5, max_iter=40, otherwise as shipped (stops after 40 steps)"),
+    }
+    out = {"layout": "den/pre/vel columns of den.dat/pre.dat/vel.dat over the 800 interior cells",
+           "sample_stride": 23}
+    for key, (build, desc) in cases.items():
+        rundir = os.path.join(REF, "run_" + build)
+        text = _run(os.path.join(REF, build, "a"), rundir)
+        fields = {nm: _tokens(os.path.join(rundir, nm + ".dat"), 2)[:, 1] for nm in ("den", "pre", "vel")}
+        lines = text.splitlines()
+        steps = [int(l.split()[0]) for l in lines if l.strip().endswith("%")]
+        at = [i for i, l in enumerate(lines) if l.strip().startswith("L1 =")][0]
+        l1, linf = (float(v) for v in lines[at + 1].split())
+        rec = {
+            "program": "ex2_sedov_1d, " + desc,
+            "sha256": {k: field_digest(v) for k, v in fields.items()},
+            "sample": {k: v[::23].tolist() for k, v in fields.items()},
+            "L1": l1, "Linf": linf,
+            "stats": _parse_stats(text),
+            "last_progress_step": steps[-1] if steps else 0,
+            "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
+                            for fn in ("den.dat", "pre.dat", "vel.dat")},
+        }
+        mr = [l for l in lines if l.strip().startswith("min_rho:")]
+        if mr:
+            rec["min_rho"] = float(mr[0].split()[-1])
+        out[key] = rec
+    with open(os.path.join(GOLD, "ref_ex2_sedov1d.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def make_ex5() -> None:
     rundir = os.path.join(REF, "run_ex5")
     if not os.path.exists(os.path.join(rundir, "DONE")):
@@ -180,7 +220,7 @@ def make_ex5() -> None:
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["weno", "ex3", "ex3_l1", "ex5"]
+    what = sys.argv[1:] or ["weno", "ex3", "ex3_l1", "ex2", "ex5"]
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     if "weno" in what:
         make_weno()
@@ -188,5 +228,7 @@ if __name__ == "__main__":
         make_ex3()
     if "ex3_l1" in what:
         make_ex3_l1()
+    if "ex2" in what:
+        make_ex2()
     if "ex5" in what:
         make_ex5()

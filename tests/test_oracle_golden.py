@@ -8,6 +8,10 @@
 - ref_ex3_l1_vortex180.json: the same program built with its switch use_limiter = 1
   (the Zhang-Shu flux limiter in wenofd instead of the sweeping post-process) ->
   final fields bitwise, the printed limiter counts and error norms exactly.
+- ref_ex2_sedov1d.json: ex2_sedov_1d (the 1D program, 800 points) as shipped, at the
+  1D benchmark settings (cfl 0.6, ambient E 1e-12, swp_num 2) and as a 40-step cfl 1.5
+  run with 16 sweeps -> den/pre/vel bitwise, the sweep statistics exactly, the step
+  count through the program's progress lines and its min_rho line.
 """
 import hashlib
 import json
@@ -91,3 +95,47 @@ def test_ex3_vortex_flux_limiter_run_matches_reference_bitwise():
     d = np.abs(ex - U[:, :, 0])
     er1 = float(np.cumsum(d.T.ravel())[-1]) / 180 / 180  # i outer, j inner as in `output`
     assert er1 == g["L1"] and float(d.max()) == g["Linf"]
+
+
+_EX2 = {
+    # key: (cfl, swp_num, shipped ambient energy, max steps)
+    "shipped": (1.2, 50, True, None),
+    "cfg1": (0.6, 2, False, None),
+    "stress": (1.5, 50, True, 40),
+}
+
+
+def _stats1d_as_printed(st):
+    """The program's printout: maxval over an empty track prints as -huge-1 (int32)."""
+    none = st["sweep_count"] == 0
+    return {
+        "total sweep": st["total_sweep"], "max sweep": -2147483648 if none else st["max_sweep"],
+        "total negative pairs": st["total_neg"], "max negative pairs": -2147483648 if none else st["max_neg"],
+        "total positive neighbors": st["total_pos"],
+        "max positive neighbors": -2147483648 if none else st["max_pos"],
+        "den_count": st["den_count"], "pre_count": st["pre_count"],
+    }
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("case", list(_EX2))
+def test_ex2_1d_run_matches_reference_bitwise(case):
+    g = json.load(open(os.path.join(GOLD, "ref_ex2_sedov1d.json")))[case]
+    cfl, swp, shipped, max_steps = _EX2[case]
+    cfg = O.sedov1d_config(800, cfl=cfl, swp_num=swp)
+    o = O.Oracle1D(cfg)
+    o.set_state(O.init_sedov1d(cfg, shipped_energy=shipped))
+    t, k = o.run(0.0, 0.001, max_steps if max_steps else 1 << 40)
+    u = o.state()
+    inner = u[3:3 + 800]
+    fields = {"den": inner[:, 0], "pre": O.pressure1d(u)[3:3 + 800], "vel": inner[:, 1] / inner[:, 0]}
+    for name, v in fields.items():
+        assert _digest(v) == g["sha256"][name], name
+    assert _stats1d_as_printed(o.stats()) == g["stats"]
+    if max_steps:
+        assert k == max_steps
+        assert float(u[:, 0].min()) == g["min_rho"]
+    else:
+        assert k // 50 * 50 == g["last_progress_step"]  # progress printed every 50 steps
+    q, neg, pos = o.track()
+    assert int(q.sum()) == g["stats"]["total sweep"] and int(pos.sum()) == g["stats"]["total positive neighbors"]
