@@ -1,6 +1,7 @@
 """Benchmark: cell-stage updates/s on 2D Sedov 640^2 to T = 1, FP64 vs FP32-bulk.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --workload sedov1d     (the 1D program's case: 800 cells to T = 1e-3)
 
 One step = one SSP-RK3 step (three stages: WENO fluxes, RK update with the
 positivity check, sweep when flagged) over the whole 640^2 domain, inputs
@@ -37,6 +38,17 @@ FLOPS_PER_CELL_DIR = 906.5
 FLOPS_PER_CELL_UPDATE = 77.0
 FLOPS_PER_CELL_STAGE = 2 * FLOPS_PER_CELL_DIR + FLOPS_PER_CELL_UPDATE
 FP64_VALU_PEAK = 78.6e12
+N_CU = 256
+
+# The 1D program (ex2_sedov_1d): 800 cells on [-2, 2], cfl 0.6, ambient E 1e-12,
+# swp_num 2, T = 1e-3 (2617 steps).  Algorithmic flops per cell and stage, counted
+# the same way: split fluxes 26, local eigenvectors 25, 2 x 5-point projections of
+# 3 fields 150, 6 WENO reconstructions 6 x 67, back-projection 30, scheme 12, RK 15,
+# findalpha and checkpositivity 13.
+N1D = 800
+T1D = 1e-3
+STEPS_1D = 2617
+FLOPS_PER_CELL_STAGE_1D = 673.0
 
 
 def parse():
@@ -47,6 +59,7 @@ def parse():
     ap.add_argument("--modes", default="fp64,fp32_bulk,fp64_exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--workload", default="sedov2d", choices=["sedov2d", "sedov1d"])
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM counters under rocprofv3)")
     return ap.parse_args()
@@ -100,6 +113,68 @@ def traffic(kernel: str):
     return None
 
 
+def run_line(S, prec: str, warmup: int):
+    cfg = S.sedov1d_config(N1D, prec)
+    u0 = S.init_sedov1d(cfg)
+    with S.Solver1D(cfg) as s:
+        if warmup:
+            s.upload(u0)
+            s.advance(0.0, T1D, max_steps=warmup)
+        s.upload(u0)
+        t0 = time.perf_counter()
+        t, k = s.advance(0.0, T1D)
+        wall = time.perf_counter() - t0
+        ms = s.last_ms()
+        rho = s.download()[3:-3, 0].copy()
+        stats = s.stats()
+    return dict(steps=k, t_end=t, seconds=wall, kernel_ms=ms, value=N1D * 3 * k / wall,
+                ms_per_step=1e3 * wall / max(k, 1), stats=stats), rho
+
+
+def cpu_baseline_line(steps: int):
+    from oracle import oracle as O  # the checker, timed as the CPU baseline
+    oc = O.sedov1d_config(N1D)
+    o = O.Oracle1D(oc)
+    o.set_state(O.init_sedov1d(oc))
+    t0 = time.perf_counter()
+    t, k = o.run(0.0, T1D, steps)
+    wall = time.perf_counter() - t0
+    return {"value": N1D * 3 * k / wall, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
+            "sample": f"{k} SSP-RK3 steps of the 1D case from t=0 by the C oracle (FP64 with binary128 sites as "
+                      f"the reference build), single thread, {wall:.1f} s"}
+
+
+def main_line(a, S):
+    modes = a.modes.split(",")
+    res, rho = {}, {}
+    for m in modes:
+        res[m], rho[m] = run_line(S, m, min(a.warmup, 50))
+    err = {}
+    if "fp64_exact" in rho:
+        for m in modes:
+            if m != "fp64_exact":
+                err[m] = float(np.abs(rho[m] - rho["fp64_exact"]).sum() / np.abs(rho["fp64_exact"]).sum())
+    head = res.get("fp64") or res[modes[0]]
+    ach = FLOPS_PER_CELL_STAGE_1D * (N1D + 6) * 3 * head["steps"] / (head["kernel_ms"] * 1e-3)
+    out = {
+        "metric": "cell-stage updates/s, 1D Sedov 800 cells to T=1e-3",
+        "value": head["value"], "unit": "cell-stage updates/s", "n_gpus": 1, "steps": head["steps"],
+        "warmup": min(a.warmup, 50), "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "sedov1d_800_to_T1e-3", "cells": N1D, "cfl": 0.6, "swp_num": 2, "t_end": T1D,
+                   "precision": "fp64", "parallelism": "one workgroup, whole time loop in one launch"},
+        "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": ach / FP64_VALU_PEAK, "frac_of_one_cu": ach / (FP64_VALU_PEAK / N_CU),
+                     "traffic": None, "kernel": "k_run1d<fp64> (every stage of every step)",
+                     "kernel_ms": head["kernel_ms"]},
+        "modes": res,
+        "l1_density_rel_vs_fp64_exact": err,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_line(a.cpu_steps if a.cpu_steps > 1 else 600)
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -107,6 +182,10 @@ def main():
                                    "sharding)", "n_gpus": a.gpus}))
         sys.exit(2)
     import sweeping_fd_weno_amd as S
+
+    if a.workload == "sedov1d":
+        main_line(a, S)
+        return
 
     if a.copy_probe_mb:
         import torch

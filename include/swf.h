@@ -162,6 +162,50 @@ int swf_batch_advance(swf_batch *batch, double *time_inout, const double *tfinal
 int swf_batch_set_schedule(swf_batch *batch, int32_t schedule);
 int swf_batch_tune(swf_batch *batch, int32_t reps, int32_t *schedule_out, float *ms_out);
 
+/*
+ * The 1D program (ex2_sedov_1d/main.f90): one line of n cells with three ghost
+ * cells each side, state u(3, -2:n+3) in the program's layout (component fastest).
+ *   swf1d_create / swf1d_destroy   the PARAMETER block, arrays and weights (:4-36)
+ *   swf1d_upload / swf1d_download  the state array u (:10)
+ *   swf1d_advance                  the time loop `DO WHILE (time <= time_terminal - 1d-10)`
+ *                                  around `rk3` (:38-45, :298-332), i.e. wenofd, scheme, the
+ *                                  RK combination, checkpositivity and sweep_lim per stage
+ *   swf1d_stats / swf1d_track      the statistics printed by `output` (:487-511)
+ * Semantics of that program: eigenvectors of each cell's own state; no boundary
+ * conditions (ghost cells keep their values apart from the RK combination and the
+ * sweep); the sweep walks every cell including the ghosts, its density pass runs
+ * unconditionally with delta = 1d-13; a sweep that reaches swp_num is silent (no
+ * error code); dt = dx*cfl/alphamax is not clamped to tfinal.  use_limiter 1 (the
+ * flux limiter branch of that program's wenofd) is not provided: SWF_E_INVALID.
+ * One workgroup runs the whole loop; n + 6 <= 2048 (and the device's LDS per
+ * workgroup must hold 72*(n+6) + 1024 bytes).
+ */
+typedef struct swf1d_config {
+  int32_t n;            /* interior cells (ex2: 800) */
+  double dx;            /* cell size (ex2: 4/n) */
+  double gamma, cfl;    /* gamma1 = gamma - 1 is formed in binary64 */
+  double eps;           /* positivity threshold, 1d-13 */
+  int32_t use_limiter;  /* 2 = sweeping post-process, 0 = none */
+  int32_t swp_num;      /* sweep iteration cap */
+  int32_t precision;    /* SWF_FP64_EXACT / SWF_FP64 / SWF_FP32_BULK */
+  int32_t device;       /* HIP device ordinal */
+} swf1d_config;
+
+typedef struct swf1d_ctx swf1d_ctx;
+int swf1d_create(const swf1d_config *cfg, swf1d_ctx **out);
+void swf1d_destroy(swf1d_ctx *ctx);
+int swf1d_upload(swf1d_ctx *ctx, const double *u);
+int swf1d_download(swf1d_ctx *ctx, double *u);
+/* Up to max_steps steps while time <= tfinal - 1d-10; *time_inout advances and
+ * *steps_out (may be NULL) receives the steps taken.  Synchronous. */
+int swf1d_advance(swf1d_ctx *ctx, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out);
+int swf1d_stats(swf1d_ctx *ctx, swf_sweep_stats *out);
+/* per-trigger (q, negative pairs, positive neighbours); returns sweep_count */
+int64_t swf1d_track(swf1d_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap);
+/* Device time (ms, HIP events) of the kernels of the last swf1d_advance.  No
+ * counterpart in the reference: measurement support. */
+int swf1d_last_ms(swf1d_ctx *ctx, double *ms);
+
 int swf_time_continue(double time, double tfinal);
 int swf_sync(swf_ctx *ctx);
 const char *swf_strerror(int code);

@@ -4,6 +4,7 @@ The parameter constructors restate the PARAMETER blocks of the reference program
   sedov_config  ex4_sedov_2d/main.f90:15-20,50,687 at cut_x = cut_y = 1
   vortex_config ex3_vortex_accuracy_2d/main.f90:5-7,570-572,671-697
   jet_config    ex5_mach2000_jet/main.f90:4-6,597,686-768
+  sedov1d_config ex2_sedov_1d/main.f90:4-7,398-405 (the 1D program)
 """
 from __future__ import annotations
 
@@ -45,6 +46,16 @@ class SwfStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "sweep_count", "den_count", "pre_count", "total_sweep", "max_sweep", "total_neg", "max_neg",
         "total_pos", "max_pos", "stage_sweeps", "steps")]
+
+
+class Swf1dConfig(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("dx", C.c_double), ("gamma", C.c_double), ("cfl", C.c_double), ("eps", C.c_double),
+        ("use_limiter", C.c_int32), ("swp_num", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class SwfError(RuntimeError):
@@ -95,6 +106,16 @@ def lib():
         L.swf_batch_advance.argtypes = [vp, dp, dp, C.c_int64, P(C.c_int64), P(C.c_int32)]
         L.swf_batch_set_schedule.argtypes = [vp, C.c_int32]
         L.swf_batch_tune.argtypes = [vp, C.c_int32, P(C.c_int32), P(C.c_float)]
+        L.swf1d_create.argtypes = [P(Swf1dConfig), P(vp)]
+        L.swf1d_destroy.argtypes = [vp]
+        L.swf1d_destroy.restype = None
+        L.swf1d_upload.argtypes = [vp, dp]
+        L.swf1d_download.argtypes = [vp, dp]
+        L.swf1d_advance.argtypes = [vp, dp, C.c_double, C.c_int64, P(C.c_int64)]
+        L.swf1d_stats.argtypes = [vp, P(SwfStats)]
+        L.swf1d_track.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int64), C.c_int64]
+        L.swf1d_track.restype = C.c_int64
+        L.swf1d_last_ms.argtypes = [vp, dp]
         L.swf_strerror.argtypes = [C.c_int]
         L.swf_strerror.restype = C.c_char_p
         L.swf_last_error.argtypes = []
@@ -341,3 +362,91 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# the 1D program (ex2_sedov_1d/main.f90)
+# ---------------------------------------------------------------------------
+
+def sedov1d_config(n: int = 800, precision="fp64_exact", cfl: float = 0.6, swp_num: int = 2,
+                   use_limiter: int = 2, device: int = 0) -> Swf1dConfig:
+    """ex2_sedov_1d/main.f90:4-7,399-405: [-2, 2] with dx = 4/n, gamma 1.4, eps 1d-13.
+
+    Defaults are the 1D benchmark case (cfl 0.6, swp_num 2); the program as shipped has
+    cfl 1.2 and swp_num 50."""
+    c = Swf1dConfig()
+    c.n, c.dx, c.gamma, c.cfl, c.eps = n, (2.0 - (-2.0)) / n, 1.4, cfl, 1e-13
+    c.use_limiter, c.swp_num, c.precision, c.device = use_limiter, swp_num, _prec(precision), device
+    return c
+
+
+def state1d_shape(cfg: Swf1dConfig) -> tuple:
+    """numpy shape (n+6, 3) of the Fortran array u(3, -2:n+3)."""
+    return (cfg.n + 6, 3)
+
+
+class Solver1D:
+    """One device context of the 1D program: state, `rk3` time loop and statistics."""
+
+    def __init__(self, cfg: Swf1dConfig):
+        self.cfg = cfg
+        h = C.c_void_p()
+        _check(lib().swf1d_create(C.byref(cfg), C.byref(h)), "swf1d_create")
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().swf1d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, u: np.ndarray) -> None:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != state1d_shape(self.cfg):
+            raise ValueError(f"state shape {u.shape} != {state1d_shape(self.cfg)}")
+        _check(lib().swf1d_upload(self._h, _dp(u)), "swf1d_upload")
+
+    def download(self) -> np.ndarray:
+        u = np.empty(state1d_shape(self.cfg))
+        _check(lib().swf1d_download(self._h, _dp(u)), "swf1d_download")
+        return u
+
+    def advance(self, t: float, tfinal: float, max_steps: int = 1 << 40):
+        """The program's time loop: steps while time <= tfinal - 1d-10.  Returns (time, steps)."""
+        tt = C.c_double(t)
+        n = C.c_int64(0)
+        _check(lib().swf1d_advance(self._h, C.byref(tt), tfinal, max_steps, C.byref(n)), "swf1d_advance")
+        return tt.value, n.value
+
+    def last_ms(self) -> float:
+        """Device time (HIP events) of the last advance."""
+        ms = C.c_double(0.0)
+        _check(lib().swf1d_last_ms(self._h, C.byref(ms)), "swf1d_last_ms")
+        return ms.value
+
+    def stats(self) -> dict:
+        s = SwfStats()
+        _check(lib().swf1d_stats(self._h, C.byref(s)), "swf1d_stats")
+        return {n: getattr(s, n) for n, _ in SwfStats._fields_}
+
+    def track(self):
+        n = lib().swf1d_track(self._h, None, None, None, 0)
+        if n < 0:
+            _check(int(n), "swf1d_track")
+        q = np.zeros(max(n, 1), np.int64)
+        ng = np.zeros_like(q)
+        ps = np.zeros_like(q)
+        P = C.POINTER(C.c_int64)
+        lib().swf1d_track(self._h, q.ctypes.data_as(P), ng.ctypes.data_as(P), ps.ctypes.data_as(P), n)
+        return q[:n], ng[:n], ps[:n]

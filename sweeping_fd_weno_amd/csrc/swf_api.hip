@@ -808,3 +808,199 @@ int swf_batch_tune(swf_batch *bt, int32_t reps, int32_t *schedule_out, float *ms
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The 1D program: one workgroup runs the time loop (swf_1d.h)
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr int kTrackCap1d = 1 << 16;
+constexpr long long kStepsPerLaunch1d = 4096;  // bounds one launch's duration
+}  // namespace
+
+struct swf1d_ctx {
+  swf1d_config cfg;
+  K1Cfg k;
+  B1 b;
+  int mode;
+  hipStream_t stream;
+  hipEvent_t e0, e1;
+  double last_ms;
+  long long steps_total;
+};
+
+extern "C" {
+
+void swf1d_destroy(swf1d_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->cfg.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->b.u) (void)hipFree(c->b.u);
+  if (c->b.g) (void)hipFree(c->b.g);
+  if (c->b.trk) (void)hipFree(c->b.trk);
+  if (c->e0) (void)hipEventDestroy(c->e0);
+  if (c->e1) (void)hipEventDestroy(c->e1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int swf1d_create(const swf1d_config *cfg, swf1d_ctx **out) {
+  if (!cfg || !out) return SWF_E_INVALID;
+  *out = nullptr;
+  const int N = cfg->n + 6;
+  if (cfg->n < 1 || N > D1_MAXN || !(cfg->dx > 0.0) || !(cfg->gamma > 1.0) || !(cfg->cfl > 0.0) ||
+      (cfg->use_limiter != 0 && cfg->use_limiter != 2) || (cfg->use_limiter == 2 && cfg->swp_num < 1) ||
+      cfg->precision < SWF_FP64_EXACT || cfg->precision > SWF_FP32_BULK) {
+    snprintf(g_last_error, sizeof(g_last_error),
+             "swf1d_create: need 1 <= n <= %d, dx > 0, gamma > 1, cfl > 0, use_limiter 0 or 2 (swp_num >= 1), "
+             "precision 0..2",
+             D1_MAXN - 6);
+    return SWF_E_INVALID;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) return SWF_E_NODEVICE;
+  HIPC(hipSetDevice(cfg->device));
+  int lds_max = 0;
+  HIPC(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, cfg->device));
+  if ((size_t)lds_max < d1_lds_bytes(N)) {
+    snprintf(g_last_error, sizeof(g_last_error), "swf1d_create: n = %d needs %zu bytes of LDS, the device has %d",
+             cfg->n, d1_lds_bytes(N), lds_max);
+    return SWF_E_INVALID;
+  }
+  swf1d_ctx *c = new (std::nothrow) swf1d_ctx();
+  if (!c) return SWF_E_NOMEM;
+  c->cfg = *cfg;
+  c->mode = cfg->precision;
+  K1Cfg &k = c->k;
+  k.n = cfg->n;
+  k.N = N;
+  k.use_limiter = cfg->use_limiter;
+  k.swp_num = cfg->swp_num;
+  k.track_cap = kTrackCap1d;
+  k.dx = cfg->dx;
+  k.idx = 1.0 / cfg->dx;
+  k.gamma = cfg->gamma;
+  k.gamma1 = cfg->gamma - 1.;
+  k.eps = cfg->eps;
+  // weights(i) = 1/(nSteps + 6) with nSteps = n as REAL (ex2_sedov_1d/main.f90:5,34-36)
+  k.weight = 1. / ((double)cfg->n + 6.);
+  k.dt_num = cfg->dx * cfg->cfl;
+  if (hipMalloc(&c->b.u, sizeof(double) * 3 * N) != hipSuccess ||
+      hipMalloc(&c->b.g, sizeof(D1Scalars)) != hipSuccess ||
+      hipMalloc(&c->b.trk, sizeof(long long) * 3 * kTrackCap1d) != hipSuccess) {
+    swf1d_destroy(c);
+    return SWF_E_NOMEM;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
+      hipMemsetAsync(c->b.g, 0, sizeof(D1Scalars), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->b.trk, 0, sizeof(long long) * 3 * kTrackCap1d, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->b.u, 0, sizeof(double) * 3 * N, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    const hipError_t e = hipGetLastError();
+    hip_fail(e, "swf1d_create");
+    swf1d_destroy(c);
+    return SWF_E_HIP;
+  }
+  *out = c;
+  return SWF_OK;
+}
+
+int swf1d_upload(swf1d_ctx *c, const double *u) {
+  if (!c || !u) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  HIPC(hipMemcpyAsync(c->b.u, u, sizeof(double) * 3 * c->k.N, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return SWF_OK;
+}
+
+int swf1d_download(swf1d_ctx *c, double *u) {
+  if (!c || !u) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  HIPC(hipMemcpyAsync(u, c->b.u, sizeof(double) * 3 * c->k.N, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return SWF_OK;
+}
+
+static int read_scalars1d(swf1d_ctx *c, D1Scalars *h) {
+  HIPC(hipMemcpyAsync(h, c->b.g, sizeof(D1Scalars), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return SWF_OK;
+}
+
+int swf1d_advance(swf1d_ctx *c, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out) {
+  if (!c || !time_inout || max_steps < 0) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  D1Scalars h;
+  int rc = read_scalars1d(c, &h);
+  if (rc) return rc;
+  h.time = *time_inout;
+  h.tfinal = tfinal;
+  HIPC(hipMemcpyAsync(c->b.g, &h, sizeof(D1Scalars), hipMemcpyHostToDevice, c->stream));
+  long long done = 0;
+  const long long start = h.steps_done;
+  HIPC(hipEventRecord(c->e0, c->stream));
+  while (done < max_steps) {
+    const long long chunk = std::min<long long>(max_steps - done, kStepsPerLaunch1d);
+    const hipError_t e = launch_run1d(c->mode, c->k, c->b, c->stream, chunk);
+    if (e != hipSuccess) return hip_fail(e, "k_run1d launch");
+    rc = read_scalars1d(c, &h);
+    if (rc) return rc;
+    const long long took = h.steps_done - start - done;
+    done += took;
+    if (took < chunk) break;  // the time test ended the loop
+  }
+  HIPC(hipEventRecord(c->e1, c->stream));
+  HIPC(hipEventSynchronize(c->e1));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, c->e0, c->e1));
+  c->last_ms = ms;
+  c->steps_total += done;
+  *time_inout = h.time;
+  if (steps_out) *steps_out = done;
+  return check_launch("swf1d_advance");
+}
+
+int swf1d_stats(swf1d_ctx *c, swf_sweep_stats *s) {
+  if (!c || !s) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  D1Scalars h;
+  const int rc = read_scalars1d(c, &h);
+  if (rc) return rc;
+  s->sweep_count = h.sweep_count;
+  s->den_count = h.den_count;
+  s->pre_count = h.pre_count;
+  s->total_sweep = h.total_sweep;
+  s->max_sweep = h.max_sweep;
+  s->total_neg = h.total_neg;
+  s->max_neg = h.max_neg;
+  s->total_pos = h.total_pos;
+  s->max_pos = h.max_pos;
+  s->stage_sweeps = h.stage_sweeps;
+  s->steps = c->steps_total;
+  return SWF_OK;
+}
+
+int64_t swf1d_track(swf1d_ctx *c, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap) {
+  if (!c) return SWF_E_INVALID;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SWF_E_HIP;
+  D1Scalars h;
+  if (read_scalars1d(c, &h)) return SWF_E_HIP;
+  const int64_t n = h.sweep_count < cap ? h.sweep_count : cap;
+  const int64_t m = n < kTrackCap1d ? n : kTrackCap1d;
+  if (m > 0 && q && neg && pos) {
+    if (hipMemcpy(q, c->b.trk, sizeof(int64_t) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(neg, c->b.trk + kTrackCap1d, sizeof(int64_t) * m, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(pos, c->b.trk + 2 * kTrackCap1d, sizeof(int64_t) * m, hipMemcpyDeviceToHost) != hipSuccess)
+      return SWF_E_HIP;
+  }
+  return h.sweep_count;
+}
+
+int swf1d_last_ms(swf1d_ctx *c, double *ms) {
+  if (!c || !ms) return SWF_E_INVALID;
+  *ms = c->last_ms;
+  return SWF_OK;
+}
+
+}  // extern "C"

@@ -8,6 +8,25 @@
 
 namespace swf {
 
+template <int M, int CPT>
+static hipError_t run1d(const K1Cfg &k, const B1 &b, hipStream_t s, long long max_steps) {
+  const size_t lds = d1_lds_bytes(k.N);
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute((const void *)k_run1d<M, CPT>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_run1d<M, CPT>), dim3(1), dim3(run1d_threads(k.N)), lds, s, k, b, max_steps);
+  return hipGetLastError();
+}
+
+hipError_t launch_run1d_m0(const K1Cfg &k, const B1 &b, hipStream_t s, long long max_steps) {
+  return run1d_cpt(k.N) == 2 ? run1d<MODE_EXACT, 2>(k, b, s, max_steps) : run1d<MODE_EXACT, 4>(k, b, s, max_steps);
+}
+hipError_t launch_run1d(int mode, const K1Cfg &k, const B1 &b, hipStream_t s, long long max_steps) {
+  return mode == MODE_EXACT ? launch_run1d_m0(k, b, s, max_steps) : launch_run1d_m12(mode, k, b, s, max_steps);
+}
+
 void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
   if (k.use_limiter == 1)
     hipLaunchKernelGGL((k_xflux<MODE_EXACT, true>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);

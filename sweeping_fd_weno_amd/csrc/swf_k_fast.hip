@@ -5,6 +5,24 @@
 
 namespace swf {
 
+template <int M, int CPT>
+static hipError_t run1d(const K1Cfg &k, const B1 &b, hipStream_t s, long long max_steps) {
+  const size_t lds = d1_lds_bytes(k.N);
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute((const void *)k_run1d<M, CPT>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_run1d<M, CPT>), dim3(1), dim3(run1d_threads(k.N)), lds, s, k, b, max_steps);
+  return hipGetLastError();
+}
+
+hipError_t launch_run1d_m12(int mode, const K1Cfg &k, const B1 &b, hipStream_t s, long long max_steps) {
+  if (mode == MODE_FP64)
+    return run1d_cpt(k.N) == 2 ? run1d<MODE_FP64, 2>(k, b, s, max_steps) : run1d<MODE_FP64, 4>(k, b, s, max_steps);
+  return run1d_cpt(k.N) == 2 ? run1d<MODE_FP32, 2>(k, b, s, max_steps) : run1d<MODE_FP32, 4>(k, b, s, max_steps);
+}
+
 // the flux limiter (use_limiter = 1) has kernels of its own, so that the others
 // carry none of its registers
 template <int M, bool LIM>

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "swf_1d.h"
 #include "swf_device.h"
 
 namespace swf {
@@ -55,6 +56,18 @@ void launch_set_alpha(const Bufs &b, hipStream_t s, int slot, unsigned long long
 void launch_upload_reset(const Bufs &b, hipStream_t s);
 void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
+
+// the 1D time loop (swf_1d.h): one workgroup of nt threads, cpt cells per thread
+// and lds bytes of dynamic LDS; mode 0 in swf_k_exact.hip, 1 and 2 in swf_k_fast.hip
+hipError_t launch_run1d(int mode, const K1Cfg &k, const B1 &b, hipStream_t s, long long max_steps);
+hipError_t launch_run1d_m0(const K1Cfg &k, const B1 &b, hipStream_t s, long long max_steps);
+hipError_t launch_run1d_m12(int mode, const K1Cfg &k, const B1 &b, hipStream_t s, long long max_steps);
+// threads per 1D workgroup for N cells: two or four cells per thread, whole waves
+inline int run1d_cpt(int N) { return N <= 2 * D1_NT ? 2 : 4; }
+inline int run1d_threads(int N) {
+  const int per = (N + run1d_cpt(N) - 1) / run1d_cpt(N);
+  return (per + 63) / 64 * 64;
+}
 
 // x-flux tiles: 64 interfaces x 4 rows covering interfaces 0..nx from origin
 // -XT_SHIFT (swf_kernels.h); stage tiles: UB_W x UB_H cells

@@ -93,3 +93,39 @@ def init_jet(cfg) -> np.ndarray:
     u[:, :, 0] = cfg.amb_rho
     u[:, :, 3] = cfg.amb_p / g1
     return u
+
+
+def _round_bits(x, p: int):
+    """x (a positive Fraction) rounded to p significant bits, ties to even."""
+    from fractions import Fraction
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    if Fraction(2) ** e > x:
+        e -= 1
+    scale = Fraction(2) ** (p - 1 - e)
+    m = x * scale
+    fl = m.numerator // m.denominator
+    rem = m - fl
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and fl % 2 == 1):
+        fl += 1
+    return Fraction(fl) / scale
+
+
+def init_sedov1d(cfg, shipped_energy: bool = False, e_amb: float = 1e-12) -> np.ndarray:
+    """initdata of the 1D program (ex2_sedov_1d/main.f90:398-425): rho = 1, m = 0,
+    E = e_amb everywhere (ghosts included), and E = 3200000.0/dx in cell n/2+1.
+
+    shipped_energy: the program's own ambient E = 1d-9/gamma1 (:423), the binary128
+    literal 1d-9 divided in binary128 by gamma1 = gamma - 1 (binary64), then rounded
+    to binary64."""
+    from fractions import Fraction
+    n = cfg.n
+    u = np.zeros((n + 6, 3))
+    u[:, 0] = 1.0
+    if shipped_energy:
+        g1 = cfg.gamma - 1.0
+        q = _round_bits(_round_bits(Fraction(1, 10 ** 9), 113) / Fraction(g1), 113)
+        u[:, 2] = float(q)
+    else:
+        u[:, 2] = e_amb
+    u[n // 2 + 1 + 2, 2] = 3200000.0 / cfg.dx
+    return u

@@ -49,18 +49,47 @@ void run_block(std::function<void()> &body) {
     f.state = 0;
     f.tid = dim3(t % bDim.x, (t / bDim.x) % bDim.y, t / (bDim.x * bDim.y));
   }
-  int alive = n;
-  while (alive > 0) {
-    alive = 0;
+  // run every runnable fiber to its next yield; then release the waves whose
+  // live fibers all wait at a wave-level exchange, else (all live fibers at
+  // __syncthreads) the block barrier
+  const int nw = (n + 63) / 64;
+  while (true) {
     for (int t = 0; t < n; ++t) {
       Fiber &f = fibers[t];
-      if (f.state == 2) continue;
-      f.state = 0;
+      if (f.state != 0) continue;
       cur = t;
       tIdx = f.tid;
       swapcontext(&sched, &f.ctx);
-      if (f.state != 2) ++alive;
     }
+    int live = 0, at_block = 0, at_wave = 0;
+    for (int t = 0; t < n; ++t) {
+      live += fibers[t].state != 2;
+      at_block += fibers[t].state == 1;
+      at_wave += fibers[t].state == 3;
+    }
+    if (live == 0) break;
+    bool released = false;
+    for (int w = 0; w < nw && at_wave; ++w) {
+      int wl = 0, ww = 0;
+      for (int t = w * 64; t < n && t < (w + 1) * 64; ++t) {
+        wl += fibers[t].state != 2;
+        ww += fibers[t].state == 3;
+      }
+      if (ww > 0 && ww == wl) {
+        for (int t = w * 64; t < n && t < (w + 1) * 64; ++t)
+          if (fibers[t].state == 3) fibers[t].state = 0;
+        released = true;
+      }
+    }
+    if (released) continue;
+    if (at_block == live) {
+      for (int t = 0; t < n; ++t)
+        if (fibers[t].state == 1) fibers[t].state = 0;
+      continue;
+    }
+    fprintf(stderr, "emu: deadlock in block (%u,%u): %d live, %d at __syncthreads, %d at wave exchanges\n",
+            bIdx.x, bIdx.y, live, at_block, at_wave);
+    abort();
   }
   cur = -1;
 }
@@ -90,6 +119,19 @@ void barrier() {
   swapcontext(&fibers[cur].ctx, &sched);
 }
 
+void wave_sync() {
+  if (cur < 0) return;
+  fibers[cur].state = 3;
+  swapcontext(&fibers[cur].ctx, &sched);
+}
+
+unsigned char *dyn_lds = nullptr;
+static std::vector<unsigned char> dyn_lds_buf;
+void set_dynamic_lds(size_t bytes) {
+  dyn_lds_buf.assign(bytes ? bytes : 8, 0xA5);  // like uninitialised LDS
+  dyn_lds = dyn_lds_buf.data();
+}
+
 void launch(dim3 g, dim3 b, std::function<void()> body) {
   if (capturing) {
     capture_list->push_back([=]() { launch(g, b, body); });
@@ -115,6 +157,11 @@ struct emu_event {
 };
 
 hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int) {
+  *v = a == hipDeviceAttributeMaxSharedMemoryPerBlock ? 160 * 1024 : 0;
+  return hipSuccess;
+}
+hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int) { return hipSuccess; }
 hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (hipStream_t)0x1; return hipSuccess; }
 hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }

@@ -2,10 +2,12 @@
 // tests/emu/Makefile builds the library's own sources against this header with
 // g++).  Kernels run block by block on one OS thread; each thread of a block is
 // a fiber (ucontext) that runs until its next __syncthreads or its end, so the
-// threads of a block interleave deterministically.  __shfl_xor exchanges through
-// a buffer between two barriers (every thread of a block reaches the shuffles of
-// these kernels together); atomics are plain read-modify-writes; streams are
-// synchronous; a captured graph records the launches and replays them.
+// threads of a block interleave deterministically.  __shfl_xor, __shfl_up and
+// __ballot exchange through a buffer between two wave-level synchronisations
+// (every thread of a wave reaches them together); __syncthreads waits for every
+// live thread of the block; atomics are plain read-modify-writes; streams are
+// synchronous; a captured graph records the launches and replays them; dynamic
+// LDS (HIP_DYNAMIC_SHARED) is one buffer of the launch's size, filled with 0xA5.
 // Device memory is host memory filled with 0xA5 bytes at allocation.
 #pragma once
 #include <stddef.h>
@@ -35,7 +37,10 @@ struct dim3 {
 namespace emu {
 extern dim3 tIdx, bIdx, bDim, gDim;
 void barrier();
+void wave_sync();
 int lin_tid();
+extern unsigned char *dyn_lds;
+void set_dynamic_lds(size_t bytes);
 extern unsigned long long shfl_buf[1024];
 void launch(dim3 g, dim3 b, std::function<void()> body);
 }  // namespace emu
@@ -45,6 +50,19 @@ void launch(dim3 g, dim3 b, std::function<void()> body);
 #define gridDim (emu::gDim)
 
 inline void __syncthreads() { emu::barrier(); }
+#define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_lds;
+
+inline unsigned long long __ballot(int pred) {
+  const int t = emu::lin_tid();
+  emu::shfl_buf[t] = pred != 0;
+  emu::wave_sync();
+  unsigned long long m = 0;
+  const int w0 = t & ~63;
+  for (int l = 0; l < 64 && w0 + l < (int)(emu::bDim.x * emu::bDim.y * emu::bDim.z); ++l)
+    if (emu::shfl_buf[w0 + l]) m |= 1ull << l;
+  emu::wave_sync();
+  return m;
+}
 
 template <class T>
 T __shfl_xor(T v, int off, int width = 64) {
@@ -53,10 +71,25 @@ T __shfl_xor(T v, int off, int width = 64) {
   unsigned long long w = 0;
   memcpy(&w, &v, sizeof(T));
   emu::shfl_buf[t] = w;
-  emu::barrier();
+  emu::wave_sync();
   const int src = (t & ~(width - 1)) | ((t & (width - 1)) ^ off);
   unsigned long long r = emu::shfl_buf[src];
-  emu::barrier();
+  emu::wave_sync();
+  T out;
+  memcpy(&out, &r, sizeof(T));
+  return out;
+}
+
+template <class T>
+T __shfl(T v, int src, int width = 64) {
+  static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+  const int t = emu::lin_tid();
+  unsigned long long w = 0;
+  memcpy(&w, &v, sizeof(T));
+  emu::shfl_buf[t] = w;
+  emu::wave_sync();
+  unsigned long long r = emu::shfl_buf[(t & ~(width - 1)) | (src & (width - 1))];
+  emu::wave_sync();
   T out;
   memcpy(&out, &r, sizeof(T));
   return out;
@@ -73,11 +106,11 @@ T __shfl_up(T v, unsigned d, int width = 64) {
   unsigned long long w = 0;
   memcpy(&w, &v, sizeof(T));
   emu::shfl_buf[t] = w;
-  emu::barrier();
+  emu::wave_sync();
   const int lane = t & (width - 1);
   const int src = lane >= (int)d ? t - (int)d : t;
   unsigned long long r = emu::shfl_buf[src];
-  emu::barrier();
+  emu::wave_sync();
   T out;
   memcpy(&out, &r, sizeof(T));
   return out;
@@ -104,8 +137,12 @@ typedef struct emu_node *hipGraphNode_t;
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
 enum { hipStreamNonBlocking = 1 };
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal };
+enum hipDeviceAttribute_t { hipDeviceAttributeMaxSharedMemoryPerBlock = 1 };
+enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize = 1 };
 
 hipError_t hipSetDevice(int);
+hipError_t hipDeviceGetAttribute(int *, hipDeviceAttribute_t, int);
+hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int);
 hipError_t hipGetDeviceCount(int *);
 hipError_t hipStreamCreateWithFlags(hipStream_t *, unsigned);
 hipError_t hipStreamDestroy(hipStream_t);
@@ -133,9 +170,12 @@ hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
 namespace emu {
 void trace_launch(const void *fn, dim3 g, dim3 b);
 template <class K, class... A>
-void launch_k(K k, dim3 g, dim3 b, size_t, hipStream_t, A... a) {
+void launch_k(K k, dim3 g, dim3 b, size_t lds, hipStream_t, A... a) {
   trace_launch((const void *)k, g, b);
-  launch(g, b, [=]() { k(a...); });
+  launch(g, b, [=]() {
+    if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) set_dynamic_lds(lds);
+    k(a...);
+  });
 }
 }  // namespace emu
 #define hipLaunchKernelGGL(K, ...) emu::launch_k(K, __VA_ARGS__)

@@ -141,6 +141,27 @@ def batch_vs_single(prec, nsteps):
     return out
 
 
+def line_run(prec, n, cfl, swp, shipped, nsteps):
+    """The 1D program: oracle (exact) and the emulated library from the same state."""
+    cfg = S.sedov1d_config(n, prec, cfl=cfl, swp_num=swp)
+    u = S.init_sedov1d(cfg, shipped_energy=shipped)
+    o = O.Oracle1D(O.sedov1d_config(n, cfl=cfl, swp_num=swp))
+    o.set_state(u)
+    to, no = o.run(0.0, 0.001, nsteps)
+    s = S.Solver1D(cfg)
+    s.upload(u)
+    ts, ns = s.advance(0.0, 0.001, nsteps)
+    a, b = s.download(), o.state()
+    so, se = o.stats(), s.stats()
+    q, ng, ps = s.track()
+    qo, ngo, pso = o.track()
+    return {"steps": [no, ns], "time": [to, ts],
+            "differ": int((a.view(np.int64) != b.view(np.int64)).sum()),
+            "l1_rel": float(np.abs(a[:, 0] - b[:, 0]).sum() / np.abs(b[:, 0]).sum()),
+            "stats": {k: [so[k], se[k]] for k in STATS},
+            "track_equal": bool(np.array_equal(q, qo) and np.array_equal(ng, ngo) and np.array_equal(ps, pso))}
+
+
 def main(name):
     if name == "sedov_sweeps":
         cfg = S.sedov_config(32, "fp64_exact")
@@ -164,6 +185,12 @@ def main(name):
         out = batch_vs_single(name[len("batch_"):], 25)
     elif name.startswith("first_dt_"):
         out = first_dt(S.jet_config(40, 20, name[len("first_dt_"):], cfl=0.5, half=False))
+    elif name.startswith("line_stress_"):
+        out = line_run(name[len("line_stress_"):], 800, 1.5, 50, True, 40)
+    elif name == "line_cfg1":
+        out = line_run("fp64_exact", 800, 0.6, 2, False, 30)
+    elif name == "line_small":
+        out = line_run("fp64_exact", 60, 1.5, 50, True, 12)
     else:
         raise SystemExit("unknown scenario " + name)
     print(json.dumps(out), flush=True)

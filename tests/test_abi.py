@@ -13,14 +13,15 @@ HEADER = os.path.join(ROOT, "include", "swf.h")
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(swf_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(swf(?:1d)?_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_header_declares_the_boundary():
     names = declared()
     for n in ("swf_create", "swf_destroy", "swf_upload", "swf_download", "swf_step", "swf_advance", "swf_rhs",
               "swf_alpha_max", "swf_check_positivity", "swf_sweep", "swf_bc", "swf_stats", "swf_track",
-              "swf_weno_points", "swf_time_continue", "swf_strerror"):
+              "swf_weno_points", "swf_time_continue", "swf_strerror", "swf1d_create", "swf1d_destroy",
+              "swf1d_upload", "swf1d_download", "swf1d_advance", "swf1d_stats", "swf1d_track"):
         assert n in names
 
 
@@ -39,6 +40,15 @@ def test_config_struct_layout_matches_header():
     names = [f[0] for f in S.SwfConfig._fields_]
     text = open(HEADER).read()
     body = text[text.index("typedef struct swf_config"):text.index("} swf_config;")]
+    decl = re.findall(r"(?:int32_t|double)\s+([^;]+);", body)
+    flat = [v.strip() for d in decl for v in d.split(",")]
+    assert names == flat
+
+
+def test_line_config_struct_layout_matches_header():
+    names = [f[0] for f in S.Swf1dConfig._fields_]
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct swf1d_config"):text.index("} swf1d_config;")]
     decl = re.findall(r"(?:int32_t|double)\s+([^;]+);", body)
     flat = [v.strip() for d in decl for v in d.split(",")]
     assert names == flat

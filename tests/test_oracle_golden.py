@@ -16,6 +16,7 @@
 import hashlib
 import json
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -139,3 +140,25 @@ def test_ex2_1d_run_matches_reference_bitwise(case):
         assert k // 50 * 50 == g["last_progress_step"]  # progress printed every 50 steps
     q, neg, pos = o.track()
     assert int(q.sum()) == g["stats"]["total sweep"] and int(pos.sum()) == g["stats"]["total positive neighbors"]
+
+
+def test_flang_dot_product_sums_left_to_right():
+    """ex2's sweep mean state is dot_product(rho_t, weights) (ex2_sedov_1d/main.f90:577-579);
+    the 1D oracle sums u*w left to right from 0.  oracle/ref_harness/dot_probe.f90, built
+    with ex2's flags, prints dot_product on data where the order matters."""
+    exe = os.path.join(os.path.dirname(O.__file__), "_ref", "dot_probe")
+    if not os.path.exists(exe):
+        if not os.path.exists("/opt/rocm/lib/llvm/bin/flang"):
+            pytest.skip("flang not available")
+        subprocess.run(["make", "-s", "-C", os.path.dirname(O.__file__), exe], check=True)
+    dot_hex, loop_hex = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    x = np.array([1.0 + (i % 7) * 0.37 for i in range(1, 807)])
+    x[0], x[399] = 1e17, -1e17
+    w = 1 / (800.0 + 6)
+    s = 0.0
+    for v in x:
+        s = s + v * w
+    exact = float(sum(__import__("fractions").Fraction(float(v * w)) for v in x))
+    assert exact != s  # the data distinguish the orders
+    want = "%016X" % np.float64(s).view(np.uint64)
+    assert dot_hex == want and loop_hex == want
