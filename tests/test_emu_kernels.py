@@ -19,6 +19,9 @@ arithmetic and control flow, not races; the GPU tests check the real device.
 - A batch of three instances against each run alone: bitwise.
 - After upload of the ambient jet and swf_bc, the first dt sees the Mach 2000
   inflow ghosts (alpha ~ 800) in every precision mode.
+- The 1D program (one workgroup, whole time loop): 60 cells for 12 steps and 800
+  cells for 40 steps at cfl 1.5 (16 sweeps, 5273 limited pairs) and 30 steps of the
+  1D benchmark case: the whole state bitwise, statistics and tracks equal.
 """
 import json
 import os
@@ -96,3 +99,14 @@ def test_emulated_first_dt_after_bc(emu_lib, prec):
     dto, dte = r["dt"]
     assert r["rc"] == 0 and dto < 0.05 * 0.001
     assert abs(dte - dto) <= 1e-12 * dto
+
+
+@pytest.mark.parametrize("name", ["line_small", "line_stress_fp64_exact", "line_cfg1"])
+def test_emulated_line_matches_oracle(emu_lib, name):
+    r = scenario(name)
+    assert r["steps"][0] == r["steps"][1] and r["time"][0] == r["time"][1]
+    assert r["differ"] == 0 and r["track_equal"]
+    for key, (o, e) in r["stats"].items():
+        assert o == e, key
+    if name == "line_stress_fp64_exact":
+        assert r["stats"]["sweep_count"][0] == 16 and r["stats"]["total_pos"][0] == 5273
