@@ -66,7 +66,7 @@ def build_library(verbose: bool = False) -> str:
 
 def build_driver() -> list:
     """The Fortran drivers (reference parameter surfaces) linked against libswf.so:
-    swf_driver (ex4 Sedov) and swf_vortex (ex3 vortex accuracy)."""
+    swf_driver (ex4 Sedov), swf_vortex (ex3 vortex accuracy) and swf_sedov1d (ex2 1D Sedov)."""
     drv_dir = os.path.join(PKG, "fortran")
     mod_src = os.path.join(drv_dir, "swf_mod.f90")
     if not os.path.exists(FLANG) or not os.path.exists(mod_src):
@@ -74,7 +74,7 @@ def build_driver() -> list:
     mod = os.path.join(OBJ, "fmod")
     os.makedirs(mod, exist_ok=True)
     built = []
-    for prog in ("swf_driver", "swf_vortex"):
+    for prog in ("swf_driver", "swf_vortex", "swf_sedov1d"):
         src = os.path.join(drv_dir, prog + ".f90")
         if not os.path.exists(src):
             continue

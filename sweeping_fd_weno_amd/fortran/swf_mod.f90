@@ -4,7 +4,8 @@
 ! PARAMETER surface and time loop and calls these entry points in place of the
 ! internal procedures `rk`, `wenofd`, `findalpha`, `checkpositivity`, `sweep_lim`
 ! and `bc` (ex4_sedov_2d/main.f90).  The state array keeps the reference layout
-! u(4, -2:nx+3, -2:ny+3).
+! u(4, -2:nx+3, -2:ny+3); the 1D program's (swf1d_*, ex2_sedov_1d/main.f90) keeps
+! u(3, -2:n+3).
 module swf
    use iso_c_binding
    implicit none
@@ -24,6 +25,12 @@ module swf
       real(c_double) :: jet_rho, jet_u, amb_rho, amb_p, jet_halfwidth
       integer(c_int32_t) :: precision, device
    end type swf_config
+
+   type, bind(C) :: swf1d_config
+      integer(c_int32_t) :: n
+      real(c_double) :: dx, gamma, cfl, eps
+      integer(c_int32_t) :: use_limiter, swp_num, precision, device
+   end type swf1d_config
 
    type, bind(C) :: swf_sweep_stats
       integer(c_int64_t) :: sweep_count, den_count, pre_count
@@ -88,6 +95,51 @@ module swf
          integer(c_int64_t), intent(out) :: q(*), neg(*), pos(*)
          integer(c_int64_t), value :: cap
       end function swf_track
+
+      integer(c_int) function swf1d_create(cfg, ctx) bind(C, name='swf1d_create')
+         import :: swf1d_config, c_ptr, c_int
+         type(swf1d_config), intent(in) :: cfg
+         type(c_ptr), intent(out) :: ctx
+      end function swf1d_create
+
+      subroutine swf1d_destroy(ctx) bind(C, name='swf1d_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end subroutine swf1d_destroy
+
+      integer(c_int) function swf1d_upload(ctx, u) bind(C, name='swf1d_upload')
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+      end function swf1d_upload
+
+      integer(c_int) function swf1d_download(ctx, u) bind(C, name='swf1d_download')
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: u(*)
+      end function swf1d_download
+
+      integer(c_int) function swf1d_advance(ctx, time, tfinal, max_steps, steps) bind(C, name='swf1d_advance')
+         import :: c_ptr, c_int, c_double, c_int64_t
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: time
+         real(c_double), value :: tfinal
+         integer(c_int64_t), value :: max_steps
+         integer(c_int64_t), intent(out) :: steps
+      end function swf1d_advance
+
+      integer(c_int) function swf1d_stats(ctx, s) bind(C, name='swf1d_stats')
+         import :: c_ptr, c_int, swf_sweep_stats
+         type(c_ptr), value :: ctx
+         type(swf_sweep_stats), intent(out) :: s
+      end function swf1d_stats
+
+      integer(c_int64_t) function swf1d_track(ctx, q, neg, pos, cap) bind(C, name='swf1d_track')
+         import :: c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), intent(out) :: q(*), neg(*), pos(*)
+         integer(c_int64_t), value :: cap
+      end function swf1d_track
 
       type(c_ptr) function swf_strerror(code) bind(C, name='swf_strerror')
          import :: c_ptr, c_int

@@ -162,7 +162,8 @@ def make_ex3_l1() -> None:
 def make_ex2() -> None:
     """The three ex2 builds of oracle/Makefile (ex2, ex2_cfg1, ex2_stress)."""
     cases = {
-        "shipped": ("ex2", "as shipped: n=800, cfl=1.2, swp_num=50, E_amb=1d-9/gamma1, T=0.001"),
+        "shipped": ("ex2", "source as shipped (n=800, cfl=1.2, swp_num=50, E_amb=1d-9/gamma1, T=0.001), "
+                           "built with -O2 -fdefault-real-8 (FP64)"),
         "cfg1": ("ex2_cfg1", "cfl=0.6, swp_num=2, E_amb=1d-12, otherwise as shipped"),
         "stress": ("ex2_stress", "cfl=1.5, max_iter=40, otherwise as shipped (stops after 40 steps)"),
     }
@@ -185,6 +186,7 @@ def make_ex2() -> None:
             "last_progress_step": steps[-1] if steps else 0,
             "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
                             for fn in ("den.dat", "pre.dat", "vel.dat")},
+            "stdout_sha256": hashlib.sha256(text.encode()).hexdigest(),
         }
         mr = [l for l in lines if l.strip().startswith("min_rho:")]
         if mr:

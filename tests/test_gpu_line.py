@@ -7,11 +7,15 @@
   the ghost cells bitwise and the per-sweep tracks.
 - The fast modes on the 1D benchmark case (cfl 0.6, ambient E 1e-12, swp_num 2, T = 1e-3)
   against FP64-exact: relative L1 of the density, tolerance in the test.
+- fortran/swf_sedov1d, the ex2 program over the C-ABI, writes den.dat, pre.dat and
+  vel.dat byte for byte as the reference builds do, and prints the same norms and
+  statistics.
 - Argument checks of swf1d_create.
 """
 import hashlib
 import json
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -22,6 +26,9 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "sweeping_fd_weno_amd",
+                      "fortran", "swf_sedov1d")
+DRIVER_ARGS = {"shipped": [], "cfg1": ["0.6", "2", "1"], "stress": ["1.5", "50", "0", "40"]}
 
 # key: (cfl, swp_num, shipped ambient energy, max steps)
 CASES = {
@@ -103,3 +110,29 @@ def test_line_create_rejects_bad_arguments():
         S.Solver1D(S.sedov1d_config(800, use_limiter=1))
     with pytest.raises(S.SwfError):
         S.Solver1D(S.sedov1d_config(800, precision=3))
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_line_driver_reproduces_reference_output_files(case, tmp_path):
+    g = json.load(open(os.path.join(GOLD, "ref_ex2_sedov1d.json")))[case]
+    print("case", case, flush=True)
+    res = subprocess.run([DRIVER] + DRIVER_ARGS[case], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    for fn, digest in g["file_sha256"].items():
+        assert hashlib.sha256((tmp_path / fn).read_bytes()).hexdigest() == digest, fn
+    assert hashlib.sha256(res.stdout.encode()).hexdigest() == g["stdout_sha256"]  # the whole printout
+    lines = res.stdout.splitlines()
+    at = [i for i, l in enumerate(lines) if l.strip().startswith("L1 =")][0]
+    assert [float(v) for v in lines[at + 1].split()] == [g["L1"], g["Linf"]]
+    stats = {}
+    for l in lines:
+        t = l.strip()
+        if "=" in t and not t.startswith("L1") and not t.startswith("avg"):
+            k, v = t.split("=", 1)
+            stats[k.strip()] = int(v)
+        if t.startswith("# of limiter"):
+            stats["den_count"], stats["pre_count"] = (int(v) for v in t.split()[-2:])
+    stats.pop("L1order", None)
+    assert stats == g["stats"]
+    progress = [int(l.split()[0]) for l in lines if l.strip().endswith("%")]
+    assert (progress[-1] if progress else 0) == g["last_progress_step"]
