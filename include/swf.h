@@ -14,7 +14,7 @@
  *   swf_rhs                    `wenofd` + `scheme` (:179-650)
  *   swf_check_positivity       `checkpositivity` (:779-801)
  *   swf_sweep                  `sweep_lim` incl. `scalar_sweep` (:1401-1833)
- *   swf_bc                     `bc` (:803-996, physical part)
+ *   swf_bc                     `bc` (:803-996, physical part; ex6: :888-942)
  *   swf_stats / swf_track      the statistics printed by `output` (:1326-1351)
  *   swf_weno_points            weno_right / weno_left (ex4_sedov_2d/weno5.f90:86-146)
  *   swf_time_continue          the binary128 loop test of :112
@@ -49,11 +49,15 @@ extern "C" {
 #define SWF_BC_PERIODIC 1 /* ex3 */
 #define SWF_BC_JET_HALF 2 /* ex5: inflow at x=0 for y<=w, reflective y=0, zero-gradient elsewhere */
 #define SWF_BC_JET_FULL 3 /* full-domain jet: inflow at x=0 for |y|<w, zero-gradient elsewhere */
+#define SWF_BC_STEP 4     /* ex6: gas at rest below and right, zero-gradient above, post-shock
+                             inflow left, column 1 reset below 6*step_total (the program's bc) */
 
 /* return codes */
 #define SWF_OK 0
 #define SWF_E_PBAR 1        /* "ERROR: P(u_bar) <= 0." (ex4_sedov_2d/main.f90:1447-1454) */
 #define SWF_E_SWEEP_FAIL 2  /* "SWEEP FAIL: swp_num =" (ex4_sedov_2d/main.f90:1747-1752) */
+#define SWF_E_PSUM 3        /* "Sweep check failed: total pressure increased after adjustment"
+                               (ex6_shock_diffraction/main.f90:1418-1426) */
 #define SWF_E_INVALID (-1)
 #define SWF_E_HIP (-2)
 #define SWF_E_NOMEM (-3)
@@ -83,6 +87,16 @@ typedef struct swf_config {
   int32_t pbar_check;        /* ex4: stop on P(u_bar) <= 0 */
   int32_t fail_stop;         /* ex4: SWEEP FAIL stops (else print and continue) */
   double jet_rho, jet_u, amb_rho, amb_p, jet_halfwidth; /* jet boundary data */
+  /* the step diffraction program (ex6_shock_diffraction/main.f90), on one domain: */
+  int32_t step_total;        /* > 0: the step of step_total x 6*step_total cells, its walls
+                                reflected inside wenofd (:188-197, :417-431); bc SWF_BC_STEP */
+  int32_t alpha_lines;       /* wenofd splits each row (x) / column (y) with its own alpha max
+                                (:217-219, :449-451) */
+  int32_t weno_eps12;        /* the program's weno5.f90: 1d-12 in the weights (:104-106);
+                                alpha_lines and weno_eps12 are set together */
+  int32_t psum_check;        /* sweep stops (SWF_E_PSUM) when an order-I pair adjustment raises
+                                the pair's total pressure (:1418-1426) */
+  double in_rho, in_u, in_p; /* post-shock inflow state (dle, ule, ple of :1104-1109) */
   int32_t precision;         /* SWF_FP64_EXACT / SWF_FP64 / SWF_FP32_BULK */
   int32_t device;            /* HIP device ordinal */
 } swf_config;

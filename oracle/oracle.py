@@ -19,7 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-BC_SEDOV, BC_PERIODIC, BC_JET_HALF, BC_JET_FULL = 0, 1, 2, 3
+BC_SEDOV, BC_PERIODIC, BC_JET_HALF, BC_JET_FULL, BC_STEP = 0, 1, 2, 3, 4
 
 
 class SwoConfig(C.Structure):
@@ -35,6 +35,8 @@ class SwoConfig(C.Structure):
         ("fail_stop", C.c_int32),
         ("jet_rho", C.c_double), ("jet_u", C.c_double), ("amb_rho", C.c_double), ("amb_p", C.c_double),
         ("jet_halfwidth", C.c_double),
+        ("step_total", C.c_int32), ("alpha_lines", C.c_int32), ("weno_eps12", C.c_int32), ("psum_check", C.c_int32),
+        ("in_rho", C.c_double), ("in_u", C.c_double), ("in_p", C.c_double),
     ]
 
 
@@ -98,6 +100,8 @@ def lib():
         L.swo_init_vortex.argtypes = [P(SwoConfig), C.c_double, C.c_double, dp]
         L.swo_vortex_exact.argtypes = [P(SwoConfig), C.c_double, C.c_double, dp]
         L.swo_init_jet.argtypes = [P(SwoConfig), dp]
+        L.swo_init_step.argtypes = [P(SwoConfig), C.c_double, dp]
+        L.swo_weno_eps12_n.argtypes = [dp, dp, dp, C.c_int64]
         L.swo1d_create.restype = C.c_void_p
         L.swo1d_create.argtypes = [P(Swo1dConfig)]
         L.swo1d_destroy.argtypes = [C.c_void_p]
@@ -256,6 +260,30 @@ def jet_config(nx: int = 800, ny: int = 200, cfl: float = 0.25, half: bool = Tru
                        alpha_in_scheme=1, roe_eps_double=1, ydens_plain=1, eps2_quad_literal=1,
                        success_ge=1, jet_rho=5.0, jet_u=800.0, amb_rho=0.5, amb_p=0.4127,
                        jet_halfwidth=0.05)
+
+
+def step_config(total: int = 80, cfl: float = 0.9, swp_num: int = 20000) -> SwoConfig:
+    """ex6_shock_diffraction/main.f90:16-20,1089-1101,695 on one domain (cut_x = cut_y = 1):
+    [0, 13] x [0, 11] with 13*total x 11*total cells, a step of total x 6*total cells."""
+    nx, ny = 13 * total, 11 * total
+    return make_config(nx=nx, ny=ny, xlo=0.0, ylo=0.0, dx=13.0 / nx, dy=11.0 / ny, gamma=1.4, cfl=cfl, eps=1e-13,
+                       eps2=1e-8, delta=1e-12, bc=BC_STEP, use_limiter=2, swp_num=swp_num, poscheck_le=1,
+                       pbar_check=1, fail_stop=1, step_total=total, alpha_lines=1, weno_eps12=1, psum_check=1)
+
+
+def init_step(cfg: SwoConfig, mach: float = 5.09) -> np.ndarray:
+    """ex6 initdata; also sets cfg.in_rho/in_u/in_p (the inflow state of the step bc)."""
+    u = np.zeros(state_shape(cfg))
+    lib().swo_init_step(C.byref(cfg), mach, _dp(u))
+    return u
+
+
+def weno_eps12(a: np.ndarray):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    r, l = np.empty(n), np.empty(n)
+    lib().swo_weno_eps12_n(_dp(a), _dp(r), _dp(l), n)
+    return r, l
 
 
 def init_sedov(cfg: SwoConfig, e_corner_num: float = 0.244816, e_amb: float = 1e-12) -> np.ndarray:

@@ -69,14 +69,15 @@ static void weno_beta(const double *a, double beta[3]) {
     beta[2] = D(q13_12 * (x * x) + q025 * (y * y));
 }
 
-/* alpha = d/(1d-6+beta)**2 with d a REAL(8) literal, evaluated in REAL(16). */
-static inline double weno_alpha(double d, double beta) {
-    quad e = 1e-6Q + Q(beta);
+/* alpha = d/(1d-6+beta)**2 with d a REAL(8) literal, evaluated in REAL(16); the
+ * weno5.f90 of ex6_shock_diffraction has 1d-12 in its place (weps). */
+static inline double weno_alpha(double d, double beta, quad weps) {
+    quad e = weps + Q(beta);
     return D(Q(d) / (e * e));
 }
 
 /* weno_right (weno5.f90:120-146): value at x_{i+1/2}; `a` points at average(0) */
-static double weno_right(const double *a) {
+static double weno_right(const double *a, quad weps) {
     double um[3], beta[3], al[3], ww[3];
     for (int k = 1; k <= 3; ++k) {
         int is = k - 3;
@@ -85,9 +86,9 @@ static double weno_right(const double *a) {
         um[k - 1] = s;
     }
     weno_beta(a, beta);
-    al[0] = weno_alpha(0.3, beta[0]);
-    al[1] = weno_alpha(0.6, beta[1]);
-    al[2] = weno_alpha(0.1, beta[2]);
+    al[0] = weno_alpha(0.3, beta[0], weps);
+    al[1] = weno_alpha(0.6, beta[1], weps);
+    al[2] = weno_alpha(0.1, beta[2], weps);
     double sum = (al[0] + al[1]) + al[2];
     ww[0] = al[0] / sum;
     ww[1] = al[1] / sum;
@@ -96,7 +97,7 @@ static double weno_right(const double *a) {
 }
 
 /* weno_left (weno5.f90:86-113): value at x_{i-1/2} */
-static double weno_left(const double *a) {
+static double weno_left(const double *a, quad weps) {
     double up[3], beta[3], al[3], ww[3];
     for (int k = 1; k <= 3; ++k) {
         int is = k - 3;
@@ -105,9 +106,9 @@ static double weno_left(const double *a) {
         up[k - 1] = s;
     }
     weno_beta(a, beta);
-    al[0] = weno_alpha(0.1, beta[0]);
-    al[1] = weno_alpha(0.6, beta[1]);
-    al[2] = weno_alpha(0.3, beta[2]);
+    al[0] = weno_alpha(0.1, beta[0], weps);
+    al[1] = weno_alpha(0.6, beta[1], weps);
+    al[2] = weno_alpha(0.3, beta[2], weps);
     double sum = (al[0] + al[1]) + al[2];
     ww[0] = al[0] / sum;
     ww[1] = al[1] / sum;
@@ -116,10 +117,17 @@ static double weno_left(const double *a) {
 }
 
 void swo_weno_right_n(const double *a, double *out, int64_t n) {
-    for (int64_t k = 0; k < n; ++k) out[k] = weno_right(a + 5 * k + 2);
+    for (int64_t k = 0; k < n; ++k) out[k] = weno_right(a + 5 * k + 2, 1e-6Q);
 }
 void swo_weno_left_n(const double *a, double *out, int64_t n) {
-    for (int64_t k = 0; k < n; ++k) out[k] = weno_left(a + 5 * k + 2);
+    for (int64_t k = 0; k < n; ++k) out[k] = weno_left(a + 5 * k + 2, 1e-6Q);
+}
+/* ex6_shock_diffraction/weno5.f90:104-106,138-140: 1d-12 */
+void swo_weno_eps12_n(const double *a, double *right, double *left, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) {
+        right[k] = weno_right(a + 5 * k + 2, 1e-12Q);
+        left[k] = weno_left(a + 5 * k + 2, 1e-12Q);
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -141,6 +149,9 @@ struct swo_ctx {
     int64_t *trk_q, *trk_neg, *trk_pos;
     int64_t trk_cap;
     int64_t t_pos_count, t_neg_count;
+    quad weps;            /* the WENO weights' 1d-6 (1d-12 in ex6) */
+    double *alpha;        /* findalpha's alpha(i,j) (per-line alphas) */
+    double *aline;        /* alpha max of one row / column */
 };
 
 #define U4(arr, c, i, j) (arr)[(c) + 4 * (((i) + 2) + C->nxg * ((j) + 2))]
@@ -194,6 +205,9 @@ swo_ctx *swo_create(const swo_config *cfg) {
     C->ordII_i = (int32_t *)malloc(N * sizeof(int32_t));
     C->ordII_j = (int32_t *)malloc(N * sizeof(int32_t));
     C->rho_t = (double *)malloc(N * sizeof(double));
+    C->weps = cfg->weno_eps12 ? 1e-12Q : 1e-6Q;
+    C->alpha = (double *)calloc((size_t)C->nxg * C->nyg, sizeof(double));
+    C->aline = (double *)calloc((size_t)(C->nxg > C->nyg ? C->nxg : C->nyg), sizeof(double));
     C->trk_cap = 1024;
     C->trk_q = (int64_t *)calloc(C->trk_cap, sizeof(int64_t));
     C->trk_neg = (int64_t *)calloc(C->trk_cap, sizeof(int64_t));
@@ -210,6 +224,7 @@ void swo_destroy(swo_ctx *C) {
     free(C->ordI_i); free(C->ordI_j); free(C->ordII_i); free(C->ordII_j);
     free(C->rho_t);
     free(C->trk_q); free(C->trk_neg); free(C->trk_pos);
+    free(C->alpha); free(C->aline);
     free(C);
 }
 
@@ -314,6 +329,51 @@ void swo_bc(swo_ctx *C) {
                 for (int c = 0; c < 4; ++c) U4(u, c, i, ny + ig) = U4(u, c, i, ny);
         break;
     }
+    case SWO_BC_STEP: {
+        /* ex6_shock_diffraction/main.f90:888-942 on one domain (all four faces are
+         * physical; the self exchange before them is overwritten in every ghost
+         * cell), in this order: fixed quiescent gas (gamma, 0, 0, 1/gamma1) below and
+         * to the right, zero gradient above, the post-shock inflow on the left; then
+         * the loop that was meant for the step's face reads i after its loop has
+         * ended (i = 1) and resets column 1 to (1, 0, 0, 1/gamma1) in rows below
+         * 6*total. */
+        const double g1 = C->gamma1, g = C->cfg.gamma;
+        for (int jg = -2; jg <= 0; ++jg)
+            for (int i = -2; i <= nx + 3; ++i) {
+                U4(u, 0, i, jg) = g;
+                U4(u, 1, i, jg) = 0.;
+                U4(u, 2, i, jg) = 0.;
+                U4(u, 3, i, jg) = 1. / g1;
+            }
+        for (int ig = 1; ig <= 3; ++ig)
+            for (int j = -2; j <= ny + 3; ++j) {
+                U4(u, 0, nx + ig, j) = g;
+                U4(u, 1, nx + ig, j) = 0.;
+                U4(u, 2, nx + ig, j) = 0.;
+                U4(u, 3, nx + ig, j) = 1. / g1;
+            }
+        for (int ig = 1; ig <= 3; ++ig)
+            for (int i = -2; i <= nx + 3; ++i)
+                for (int c = 0; c < 4; ++c) U4(u, c, i, ny + ig) = U4(u, c, i, ny);
+        const double dle = C->cfg.in_rho, ule = C->cfg.in_u, ple = C->cfg.in_p;
+        for (int ig = -2; ig <= 0; ++ig)
+            for (int j = -2; j <= ny + 3; ++j) {
+                U4(u, 0, ig, j) = dle;
+                U4(u, 1, ig, j) = ule * dle;
+                U4(u, 2, ig, j) = 0.;
+                U4(u, 3, ig, j) = ple / g1 + (0.5 * dle) * (ule * ule);
+            }
+        const int T = C->cfg.step_total;
+        if (1 < T)
+            for (int j = -2; j <= ny + 3; ++j)
+                if (j < 6 * T) {
+                    U4(u, 0, 1, j) = 1.;
+                    U4(u, 1, 1, j) = 0.;
+                    U4(u, 2, 1, j) = 0.;
+                    U4(u, 3, 1, j) = 1. / g1;
+                }
+        break;
+    }
     default:
         break;
     }
@@ -349,6 +409,7 @@ double swo_alpha_max(swo_ctx *C) {
                 double pres = g1 * ((u4 - (0.5 * den) * (vx * vx)) - (0.5 * den) * (vy * vy));
                 double sound = sqrt((g * pres) / den);
                 double a = fmax2(fabs(u2 / u1) + sound, fabs(u3 / u1) + sound);
+                C->alpha[(i + 2) + (size_t)C->nxg * (j + 2)] = a;
                 amax = fmax2(amax, a);
             }
         }
@@ -471,8 +532,8 @@ static void interface_flux(const swo_ctx *C, int dir, double wa, double wb, doub
             cun[k][ig + 2] = dot4(evl[k], fns + 4 * ig);
         }
     for (int k = 0; k < 4; ++k) {
-        ur[k] = weno_right(&cup[k][2]);
-        ul[k] = weno_left(&cun[k][2]);
+        ur[k] = weno_right(&cup[k][2], C->weps);
+        ul[k] = weno_left(&cun[k][2], C->weps);
     }
     for (int k = 0; k < 4; ++k) {
         fpo[k] = dot4(evr[k], ur);
@@ -533,14 +594,53 @@ static void zs_count(swo_ctx *C, int changed) {
  * and columns that `scheme` reads; with it (use_limiter == 1) the reference's
  * full ranges (x: rows 0..ny+1, y: columns 0..nx+1, cells 0..n+1), whose every
  * limited half counts in den_count / pre_count.  Fills fpf/fnf and gpf/gnf. */
+/* alpha max of row j (x-pass) or column i (y-pass) over every cell of the line,
+ * ghosts included, from 0 (ex6_shock_diffraction/main.f90:217-219, :449-451) */
+static double line_alpha(const swo_ctx *C, int dir, int line) {
+    double a = 0.;
+    if (dir == 0)
+        for (int i = -2; i <= C->nx + 3; ++i) a = fmax2(a, C->alpha[(i + 2) + (size_t)C->nxg * (line + 2)]);
+    else
+        for (int j = -2; j <= C->ny + 3; ++j) a = fmax2(a, C->alpha[(line + 2) + (size_t)C->nxg * (j + 2)]);
+    return a;
+}
+
+/* The step's walls inside wenofd (ex6_shock_diffraction/main.f90:188-197 before the
+ * x-pass, :417-431 before the y-pass): the three cells inside the vertical wall
+ * x = total mirror the three outside it (m1 negated) on rows 0..ny+1 up to 6*total;
+ * the three rows inside the top y = 6*total mirror the three above it (m2 negated)
+ * in columns 1..total.  The state keeps these values (they enter the RK update). */
+static void step_reflect(swo_ctx *C, int dir) {
+    double *u = C->u;
+    const int T = C->cfg.step_total;
+    if (dir == 0) {
+        for (int j = 0; j <= C->ny + 1 && j <= 6 * T; ++j)
+            for (int i = 0; i <= 2; ++i) {
+                for (int c = 0; c < 4; ++c) U4(u, c, T - i, j) = U4(u, c, T + 1 + i, j);
+                U4(u, 1, T - i, j) = -U4(u, 1, T + 1 + i, j);
+            }
+    } else {
+        const int j = 6 * T;
+        if (j < -2 || j > C->ny + 3) return;
+        for (int i = 0; i <= 2; ++i)
+            for (int k = 1; k <= T; ++k) {
+                for (int c = 0; c < 4; ++c) U4(u, c, k, j - i) = U4(u, c, k, j + 1 + i);
+                U4(u, 2, k, j - i) = -U4(u, 2, k, j + 1 + i);
+            }
+    }
+}
+
 static void wenofd(swo_ctx *C, double alphamax) {
     const double *u = C->u;
     const int nx = C->nx, ny = C->ny;
     const int scale = !C->cfg.alpha_in_scheme;
     double f[4];
     const int zs = C->cfg.use_limiter == 1;
+    const double amax_all = alphamax;
+    if (C->cfg.step_total > 0) step_reflect(C, 0);
     /* x sweep over rows j = 1..ny (0..ny+1 with the limiter) */
     for (int j = zs ? 0 : 1; j <= (zs ? ny + 1 : ny); ++j) {
+        alphamax = C->cfg.alpha_lines ? line_alpha(C, 0, j) : amax_all;
         for (int i = 0; i <= nx + 2; ++i)
             roe_prims(C, &U4(u, 0, i, j), 0, &C->pw[i], &C->pvx[i], &C->pvy[i], &C->ph[i]);
         for (int i = -2; i <= nx + 3; ++i) {
@@ -565,8 +665,10 @@ static void wenofd(swo_ctx *C, double alphamax) {
             }
         }
     }
+    if (C->cfg.step_total > 0) step_reflect(C, 1);
     /* y sweep over columns i = 1..nx (0..nx+1 with the limiter) */
     for (int i = zs ? 0 : 1; i <= (zs ? nx + 1 : nx); ++i) {
+        alphamax = C->cfg.alpha_lines ? line_alpha(C, 1, i) : amax_all;
         for (int j = 0; j <= ny + 2; ++j)
             roe_prims(C, &U4(u, 0, i, j), 1, &C->pw[j], &C->pvx[j], &C->pvy[j], &C->ph[j]);
         for (int j = -2; j <= ny + 3; ++j) {
@@ -736,8 +838,11 @@ static void track_push(swo_ctx *C, int64_t q, int64_t neg, int64_t pos) {
     C->trk_pos[C->sweep_count - 1] = pos;
 }
 
-/* one directional pass of sweep_lim over an order (:1468-1586) */
-static void sweep_pass(swo_ctx *C, const int32_t *oi, const int32_t *oj, int forward, const double ub[4]) {
+/* one directional pass of sweep_lim over an order (:1468-1586).  check: ex6's test
+ * of each pair adjustment (ex6_shock_diffraction/main.f90:1418-1426,1477-1485 in the
+ * order-I passes): P(v_i) + P(v_p) > P(v_i~) + P(v_p~) + eps stops the program. */
+static int sweep_pass(swo_ctx *C, const int32_t *oi, const int32_t *oj, int forward, const double ub[4],
+                      int check) {
     double *u = C->u;
     const double g1 = C->gamma1, eps = C->cfg.eps, w = C->weight;
     const int64_t N = (int64_t)C->nx * C->ny;
@@ -756,8 +861,11 @@ static void sweep_pass(swo_ctx *C, const int32_t *oi, const int32_t *oj, int for
             for (int c = 0; c < 4; ++c) vpt[c] = vp[c] + (w / w) * (vi[c] - vit[c]);
             store4(C, u, oi[k], oj[k], vit);
             store4(C, u, oi[p], oj[p], vpt);
+            if (check && pres + sweep_pressure(g1, vp) > (sweep_pressure(g1, vit) + sweep_pressure(g1, vpt)) + eps)
+                return SWO_PSUM_INCREASE;
         }
     }
+    return SWO_OK;
 }
 
 /* sweep_lim (ex4_sedov_2d/main.f90:1401-1761) */
@@ -785,11 +893,13 @@ int swo_sweep(swo_ctx *C) {
     track_push(C, 0, 0, 0);
     for (int q = 1; q <= C->cfg.swp_num; ++q) {
         if (q % 2 == 1) {
-            sweep_pass(C, C->ordI_i, C->ordI_j, 1, ub);
-            sweep_pass(C, C->ordI_i, C->ordI_j, 0, ub);
+            int rc = sweep_pass(C, C->ordI_i, C->ordI_j, 1, ub, C->cfg.psum_check);
+            if (rc) return rc;
+            rc = sweep_pass(C, C->ordI_i, C->ordI_j, 0, ub, C->cfg.psum_check);
+            if (rc) return rc;
         } else {
-            sweep_pass(C, C->ordII_i, C->ordII_j, 1, ub);
-            sweep_pass(C, C->ordII_i, C->ordII_j, 0, ub);
+            sweep_pass(C, C->ordII_i, C->ordII_j, 1, ub, 0);
+            sweep_pass(C, C->ordII_i, C->ordII_j, 0, ub, 0);
         }
         double rho_min = 1e20, p_min = 1e20;
         for (int j = 1; j <= ny; ++j)
@@ -954,6 +1064,37 @@ void swo_init_jet(const swo_config *cfg, double *u) {
         }
 }
 
+/* ex6_shock_diffraction/main.f90:1068-1151 at cut_x = cut_y = 1: the Rankine-Hugoniot
+ * state behind a shock of Mach xm1 = mach into gas at rest (dri = gamma, pri = 1),
+ * every expression in the program's order */
+void swo_init_step(swo_config *cfg, double mach, double *u) {
+    const int nx = cfg->nx, ny = cfg->ny, nxg = nx + 6, T = cfg->step_total;
+    const double gamma = cfg->gamma, g1 = gamma - 1.;
+    const double ss = mach, dri = gamma, pri = 1.;
+    const double ach2 = (((0.0 - ss) * (0.0 - ss)) / gamma / pri) * dri;
+    const double ple = pri * (((gamma * ach2) / (gamma + 1.)) * 2. - (gamma - 1.) / (gamma + 1.));
+    const double dle = (((dri * (gamma + 1.)) / 2.) * ach2) / (((gamma - 1.) / 2.) * ach2 + 1.);
+    const double ule = ((0.0 - ss) / dle) * dri + ss;
+    cfg->in_rho = dle;
+    cfg->in_u = ule;
+    cfg->in_p = ple;
+    for (int j = -2; j <= ny + 3; ++j)
+        for (int i = -2; i <= nx + 3; ++i) {
+            double *a = &u[4 * ((i + 2) + (size_t)nxg * (j + 2))];
+            if (i <= T / 2 && j >= T * 6) {
+                a[0] = dle;
+                a[1] = ule * dle;
+                a[2] = 0.;
+                a[3] = ple / g1 + (0.5 * dle) * (ule * ule);
+            } else {
+                a[0] = dri;
+                a[1] = 0.;
+                a[2] = 0.;
+                a[3] = pri / g1;
+            }
+        }
+}
+
 /* ========================================================================= */
 /* The 1D program (ex2_sedov_1d/main.f90), built with -fdefault-real-8: the   */
 /* state u(3, -2:n+3), local (cell) eigenvectors, no boundary conditions (the */
@@ -1096,8 +1237,8 @@ static void wenofd1d(swo1d_ctx *C) {
             }
         double ur[3], ul[3];
         for (int k = 0; k < 3; ++k) {
-            ur[k] = weno_right(&cup[k][2]);
-            ul[k] = weno_left(&cun[k][2]);
+            ur[k] = weno_right(&cup[k][2], 1e-6Q);
+            ul[k] = weno_left(&cun[k][2], 1e-6Q);
         }
         for (int k = 0; k < 3; ++k) {
             U1(C->hpf, k, m) = (evr[k][0] * ur[0] + evr[k][1] * ur[1]) + evr[k][2] * ur[2];

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-enum swo_bc { SWO_BC_SEDOV = 0, SWO_BC_PERIODIC = 1, SWO_BC_JET_HALF = 2, SWO_BC_JET_FULL = 3 };
+enum swo_bc { SWO_BC_SEDOV = 0, SWO_BC_PERIODIC = 1, SWO_BC_JET_HALF = 2, SWO_BC_JET_FULL = 3, SWO_BC_STEP = 4 };
 
 typedef struct swo_config {
     int32_t nx, ny;
@@ -50,6 +50,13 @@ typedef struct swo_config {
     int32_t fail_stop;         /* ex4: SWEEP FAIL stops the run */
     /* jet boundary data (ex5 bc) */
     double jet_rho, jet_u, amb_rho, amb_p, jet_halfwidth;
+    /* the step diffraction program (ex6_shock_diffraction/main.f90) */
+    int32_t step_total;  /* > 0: a step of step_total x 6*step_total cells at the lower left,
+                            its walls reflected inside wenofd (SWO_BC_STEP) */
+    int32_t alpha_lines; /* wenofd splits with the alpha max of each row (x) / column (y) */
+    int32_t weno_eps12;  /* weno5.f90 of ex6: 1d-12 in place of 1d-6 in the weights */
+    int32_t psum_check;  /* sweep stops when an order-I pair adjustment raises P_i + P_p */
+    double in_rho, in_u, in_p; /* post-shock inflow state (ex6 initdata) */
 } swo_config;
 
 typedef struct swo_stats {
@@ -64,6 +71,7 @@ typedef struct swo_ctx swo_ctx;
 #define SWO_OK 0
 #define SWO_PBAR_NONPOSITIVE 1
 #define SWO_SWEEP_FAIL 2
+#define SWO_PSUM_INCREASE 3
 
 swo_ctx *swo_create(const swo_config *cfg);
 void swo_destroy(swo_ctx *c);
@@ -89,6 +97,8 @@ int swo_time_continue(double time, double tfinal);
 /* weno5.f90 point reconstructions (n stencils of 5 values a(-2..2)) */
 void swo_weno_right_n(const double *a, double *out, int64_t n);
 void swo_weno_left_n(const double *a, double *out, int64_t n);
+/* the same with weno5.f90 of ex6 (1d-12 in the weights) */
+void swo_weno_eps12_n(const double *a, double *right, double *left, int64_t n);
 /* nod_lim (ex4_sedov_2d/main.f90:1359-1396); returns 1 for the positive branch, 0 negative */
 int swo_nod_lim(const double *v1, const double *v2, const double *vbar, double gamma1,
                 double eps, double *out);
@@ -98,6 +108,10 @@ void swo_init_sedov(const swo_config *cfg, double e_corner_num, double e_amb, do
 void swo_init_vortex(const swo_config *cfg, double mag, double t, double *u);
 void swo_vortex_exact(const swo_config *cfg, double mag, double t, double *u);
 void swo_init_jet(const swo_config *cfg, double *u);
+/* ex6 initdata (:1068-1151) on a single domain: pre-shock gas (gamma, 0, 0, 1/gamma1)
+ * and the post-shock state of a Mach `mach` shock for i <= total/2, j >= 6*total.
+ * Writes in_rho/in_u/in_p into *cfg (the inflow state of SWO_BC_STEP). */
+void swo_init_step(swo_config *cfg, double mach, double *u);
 
 /* The 1D program ex2_sedov_1d/main.f90 (built with -fdefault-real-8): u(3, -2:n+3). */
 typedef struct swo1d_config {

@@ -12,13 +12,13 @@ without a HIP device, raises.
 """
 from ._lib import (SwfConfig, SwfError, Solver, Batch, lib, library_path, sedov_config, vortex_config,
                    jet_config, weno_points, time_continue, state_shape, PRECISIONS, BC_SEDOV,
-                   BC_PERIODIC, BC_JET_HALF, BC_JET_FULL, Swf1dConfig, Solver1D, sedov1d_config,
+                   BC_PERIODIC, BC_JET_HALF, BC_JET_FULL, BC_STEP, step_config, step_inflow, Swf1dConfig, Solver1D, sedov1d_config,
                    state1d_shape)
-from .problems import init_sedov, init_vortex, init_jet, vortex_exact, init_sedov1d
+from .problems import init_sedov, init_vortex, init_jet, vortex_exact, init_sedov1d, init_step
 
 __all__ = [
     "SwfConfig", "SwfError", "Solver", "Batch", "lib", "library_path", "sedov_config", "vortex_config", "jet_config",
     "weno_points", "time_continue", "state_shape", "PRECISIONS", "BC_SEDOV", "BC_PERIODIC", "BC_JET_HALF",
     "BC_JET_FULL", "init_sedov", "init_vortex", "init_jet", "vortex_exact", "Swf1dConfig", "Solver1D",
-    "sedov1d_config", "state1d_shape", "init_sedov1d",
+    "sedov1d_config", "state1d_shape", "init_sedov1d", "BC_STEP", "step_config", "step_inflow", "init_step",
 ]

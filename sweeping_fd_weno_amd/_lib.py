@@ -4,6 +4,7 @@ The parameter constructors restate the PARAMETER blocks of the reference program
   sedov_config  ex4_sedov_2d/main.f90:15-20,50,687 at cut_x = cut_y = 1
   vortex_config ex3_vortex_accuracy_2d/main.f90:5-7,570-572,671-697
   jet_config    ex5_mach2000_jet/main.f90:4-6,597,686-768
+  step_config   ex6_shock_diffraction/main.f90:16-20,695,1089-1109 on one domain
   sedov1d_config ex2_sedov_1d/main.f90:4-7,398-405 (the 1D program)
 """
 from __future__ import annotations
@@ -17,9 +18,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_HERE, "libswf.so")
 
 PRECISIONS = {"fp64_exact": 0, "fp64": 1, "fp32_bulk": 2}
-BC_SEDOV, BC_PERIODIC, BC_JET_HALF, BC_JET_FULL = 0, 1, 2, 3
+BC_SEDOV, BC_PERIODIC, BC_JET_HALF, BC_JET_FULL, BC_STEP = 0, 1, 2, 3, 4
 
-SWF_OK, SWF_E_PBAR, SWF_E_SWEEP_FAIL = 0, 1, 2
+SWF_OK, SWF_E_PBAR, SWF_E_SWEEP_FAIL, SWF_E_PSUM = 0, 1, 2, 3
 
 
 class SwfConfig(C.Structure):
@@ -35,6 +36,8 @@ class SwfConfig(C.Structure):
         ("fail_stop", C.c_int32),
         ("jet_rho", C.c_double), ("jet_u", C.c_double), ("amb_rho", C.c_double), ("amb_p", C.c_double),
         ("jet_halfwidth", C.c_double),
+        ("step_total", C.c_int32), ("alpha_lines", C.c_int32), ("weno_eps12", C.c_int32), ("psum_check", C.c_int32),
+        ("in_rho", C.c_double), ("in_u", C.c_double), ("in_p", C.c_double),
         ("precision", C.c_int32), ("device", C.c_int32),
     ]
 
@@ -185,6 +188,33 @@ def jet_config(nx: int = 800, ny: int = 200, precision="fp64_exact", cfl: float 
                    alpha_in_scheme=1, roe_eps_double=1, ydens_plain=1, eps2_quad_literal=1, success_ge=1,
                    jet_rho=5.0, jet_u=800.0, amb_rho=0.5, amb_p=0.4127, jet_halfwidth=0.05,
                    precision=_prec(precision), device=device)
+
+
+def step_config(total: int = 80, precision="fp64_exact", mach: float = 5.09, cfl: float = 0.9,
+                swp_num: int = 20000) -> SwfConfig:
+    """ex6_shock_diffraction/main.f90 on one domain (cut_x = cut_y = 1): [0, 13] x [0, 11] with
+    13*total x 11*total cells, a step of total x 6*total cells at the lower left, a Mach
+    `mach` shock (5.09 in the program) entering from the left; per-line alphas, 1d-12 in the
+    WENO weights, the sweep's pressure-sum check, cfl 0.9.  The inflow state is the
+    Rankine-Hugoniot state of `step_inflow`."""
+    nx, ny = 13 * total, 11 * total
+    dle, ule, ple = step_inflow(mach, 1.4)
+    return _config(nx=nx, ny=ny, xlo=0.0, ylo=0.0, dx=13.0 / nx, dy=11.0 / ny, gamma=1.4, cfl=cfl, eps=1e-13,
+                   eps2=1e-8, delta=1e-12, bc=BC_STEP, use_limiter=2, swp_num=swp_num, poscheck_le=1, pbar_check=1,
+                   fail_stop=1, step_total=total, alpha_lines=1, weno_eps12=1, psum_check=1, in_rho=dle, in_u=ule,
+                   in_p=ple, precision=_prec(precision), device=0)
+
+
+def step_inflow(mach: float, gamma: float = 1.4):
+    """The post-shock state (dle, ule, ple) of ex6_shock_diffraction/main.f90:1098-1109:
+    a shock of Mach `mach` into gas with rho = gamma, u = 0, p = 1, each expression in the
+    program's order (binary64)."""
+    ss, dri, pri = mach, gamma, 1.0
+    ach2 = (((0.0 - ss) * (0.0 - ss)) / gamma / pri) * dri
+    ple = pri * (((gamma * ach2) / (gamma + 1.0)) * 2.0 - (gamma - 1.0) / (gamma + 1.0))
+    dle = (((dri * (gamma + 1.0)) / 2.0) * ach2) / (((gamma - 1.0) / 2.0) * ach2 + 1.0)
+    ule = ((0.0 - ss) / dle) * dri + ss
+    return dle, ule, ple
 
 
 def time_continue(t: float, tfinal: float) -> bool:

@@ -110,7 +110,7 @@ SWF_D void weno1d(const K1Cfg &k, const double *um, const double *sh, int N, dou
       cp[j] = (l0 * sh[j - 2] + l1 * sh[N + j - 2]) + l2 * sh[2 * N + j - 2];
       cn[j] = (l0 * sh[3 * N + j - 2] + l1 * sh[4 * N + j - 2]) + l2 * sh[5 * N + j - 2];
     }
-    const double r = weno_right<M>(&cp[2]), l = weno_left<M>(&cn[2]);
+    const double r = weno_right<M>(&cp[2], kWeps6), l = weno_left<M>(&cn[2], kWeps6);
     ur0 = c == 0 ? r : ur0;
     ur1 = c == 1 ? r : ur1;
     ur2 = c == 2 ? r : ur2;

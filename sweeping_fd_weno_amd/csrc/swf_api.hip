@@ -70,6 +70,7 @@ const char *swf_strerror(int code) {
     case SWF_OK: return "ok";
     case SWF_E_PBAR: return "ERROR: P(u_bar) <= 0.";
     case SWF_E_SWEEP_FAIL: return "SWEEP FAIL: swp_num reached";
+    case SWF_E_PSUM: return "Sweep check failed: total pressure increased after adjustment";
     case SWF_E_INVALID: return "invalid argument";
     case SWF_E_HIP: return "HIP runtime error";
     case SWF_E_NOMEM: return "out of device memory";
@@ -84,9 +85,21 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   if (!cfg || !out) return SWF_E_INVALID;
   *out = nullptr;
   if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->dx > 0) || !(cfg->dy > 0) || cfg->precision < 0 || cfg->precision > 2 ||
-      cfg->bc < 0 || cfg->bc > 3 || cfg->use_limiter < 0 || cfg->use_limiter > 2 ||
+      cfg->bc < 0 || cfg->bc > 4 || cfg->use_limiter < 0 || cfg->use_limiter > 2 ||
       (cfg->use_limiter == 2 && cfg->swp_num < 1))
     return SWF_E_INVALID;
+  // the step program: its bc and walls need the step inside the grid; the per-line
+  // alphas are not combined with the flux limiter's edge kernels
+  if ((cfg->bc == SWF_BC_STEP) != (cfg->step_total > 0) ||
+      (cfg->step_total > 0 && (cfg->step_total < 2 || cfg->nx < cfg->step_total + 3 ||
+                               cfg->ny < 6 * cfg->step_total + 3)) ||
+      (cfg->alpha_lines && cfg->use_limiter == 1) || (cfg->alpha_lines != 0) != (cfg->weno_eps12 != 0)) {
+    snprintf(g_last_error, sizeof(g_last_error),
+             "swf_create: step_total > 0 exactly with bc SWF_BC_STEP, 2 <= step_total, nx >= step_total + 3, "
+             "ny >= 6*step_total + 3; alpha_lines and weno_eps12 together (the step program's wenofd), not with "
+             "use_limiter 1");
+    return SWF_E_INVALID;
+  }
   if ((long long)cfg->nx * cfg->ny >= (1LL << 31) - 64) return SWF_E_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
@@ -138,6 +151,14 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
                           cell_alpha(k, k.amb_rho, 0.0, 0.0, k.amb_e));
   }
   k.bc = cfg->bc;
+  k.step_total = cfg->step_total;
+  k.alpha_lines = cfg->alpha_lines;
+  k.weno_eps12 = cfg->weno_eps12;
+  k.psum_check = cfg->psum_check;
+  k.in_rho = cfg->in_rho;
+  // ex6_shock_diffraction/main.f90:934-937: ule*dle, ple/gamma1 + 0.5*dle*ule**2
+  k.in_mx = cfg->in_u * cfg->in_rho;
+  k.in_e = cfg->in_p / (cfg->gamma - 1.) + (0.5 * cfg->in_rho) * (cfg->in_u * cfg->in_u);
   k.use_limiter = cfg->use_limiter;
   k.swp_num = cfg->swp_num;
   k.track_cap = kTrackCap;
@@ -171,8 +192,12 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   // limiter's (den, pre) count pairs
   const size_t nblk = (size_t)ub_gx(k) * ub_gy(k);
   const size_t part_bytes = sizeof(double) * 5 * nblk + sizeof(int) * 2 * (size_t)zs_pairs(k);
+  // then the per-line alphas and their tile partials (alpha_lines)
+  const size_t nline = (size_t)(k.nx + 6) + (k.ny + 6);
+  const size_t nlpart = (size_t)la_gx(k) * (k.ny + 6) + (size_t)la_gy(k) * (k.nx + 6);
+  const size_t line_bytes = cfg->alpha_lines ? sizeof(double) * (nline + nlpart) : 0;
   double *apart = nullptr;
-  if (e == hipSuccess) e = hipMalloc(&apart, part_bytes);
+  if (e == hipSuccess) e = hipMalloc(&apart, part_bytes + line_bytes);
   if (e == hipSuccess) e = hipMemsetAsync(apart, 0, part_bytes, c->stream);
   // the sweep's walk records (use_limiter = 2)
   double *wrec = nullptr;
@@ -215,6 +240,12 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   c->b.wrec = wrec;
   c->b.wpos = wrec ? reinterpret_cast<unsigned *>(wrec + 4 * SW_REC * SW_T) : nullptr;
   c->b.zs = reinterpret_cast<int *>(apart + 5 * nblk);
+  if (cfg->alpha_lines) {
+    double *lines = reinterpret_cast<double *>(reinterpret_cast<char *>(apart) + part_bytes);
+    c->b.arow = lines;
+    c->b.acol = lines + (k.ny + 6);
+    c->b.lpart = lines + nline;
+  }
   c->d_fortran = stage;
   *out = c;
   return SWF_OK;
@@ -279,7 +310,12 @@ static Bufs stage_bufs(const swf_ctx *c, int s) {
 static void enqueue_stage(swf_ctx *c, int s, int p) {
   const int slot_in = s - 1, slot_out = s % 3;
   const Bufs b = stage_bufs(c, s);
+  // the step program: findalpha's per-line maxima of the stage input, then the
+  // wall images written into it before each pass
+  if (c->k.alpha_lines) launch_line_alpha(c->k, b, c->stream, slot_in, p);
+  if (c->k.step_total > 0) launch_step_reflect(c->k, b, c->stream, 0, p);
   launch_xflux(c->mode, c->k, b, c->stream, s, slot_in, p);
+  if (c->k.step_total > 0) launch_step_reflect(c->k, b, c->stream, 1, p);
   launch_ystage(c->mode, c->k, b, c->stream, s, slot_in, p, 0);
   if (c->k.use_limiter == 1) {
     launch_zs_edges(c->mode, c->k, b, c->stream, slot_in, p);

@@ -119,6 +119,7 @@ SWF_HD double dd_round(dd a) { return a.hi + a.lo; }
 struct qconst {
   static constexpr double c13_12_hi = 0x1.1555555555555p+0, c13_12_lo = 0x1.5555555555555p-54;
   static constexpr double e6_hi = 0x1.0c6f7a0b5ed8dp-20, e6_lo = 0x1.b5a63f9a49c2cp-75;
+  static constexpr double e12_hi = 0x1.19799812dea11p-40, e12_lo = 0x1.97f27f0f6e886p-96;
   static constexpr double e15_hi = 0x1.203af9ee75616p-50, e15_lo = -0x1.937831647f5ap-104;
   static constexpr double e10_hi = 0x1.b7cdfd9d7bdbbp-34, e10_lo = -0x1.20a5465df8d2cp-88;
   // 1d-8 rounds to 0x1.5798ee2308c3ap-27 with a negative residual: for binary64 x,
@@ -126,9 +127,10 @@ struct qconst {
   static constexpr double e8_hi = 0x1.5798ee2308c3ap-27;
 };
 
-// alpha = d/(1d-6 + beta)**2 with d the binary64 value of the REAL(8) literal
-SWF_HD double alpha_q(double d, double beta) {
-  dd e = dd_add_d(dd{qconst::e6_hi, qconst::e6_lo}, beta);
+// alpha = d/(1d-6 + beta)**2 with d the binary64 value of the REAL(8) literal;
+// (ehi, elo) is the binary128 1d-6 (1d-12 in the weno5.f90 of the step program)
+SWF_HD double alpha_q(double d, double beta, double ehi, double elo) {
+  dd e = dd_add_d(dd{ehi, elo}, beta);
   return dd_round(d_div_dd(d, dd_sqr(e)));
 }
 

@@ -25,8 +25,8 @@
 namespace swf {
 
 enum { MODE_EXACT = 0, MODE_FP64 = 1, MODE_FP32 = 2 };
-enum { BC_SEDOV = 0, BC_PERIODIC = 1, BC_JET_HALF = 2, BC_JET_FULL = 3 };
-enum { ST_OK = 0, ST_PBAR = 1, ST_SWEEP_FAIL = 2 };
+enum { BC_SEDOV = 0, BC_PERIODIC = 1, BC_JET_HALF = 2, BC_JET_FULL = 3, BC_STEP = 4 };
+enum { ST_OK = 0, ST_PBAR = 1, ST_SWEEP_FAIL = 2, ST_PSUM = 3 };
 enum { YS_RHS_ONLY = 1, YS_NO_SCALARS = 2 };  // flags of the stage kernel
 
 // Kernel-side configuration (plain values, passed by value to every kernel).
@@ -39,9 +39,26 @@ struct KCfg {
   double xlo, ylo, jet_rho, jet_mx, jet_e, amb_rho, amb_e, jet_halfwidth;
   double alpha_const;  // alpha of the prescribed boundary states (jet inflow/ambient), else 0
   int bc, use_limiter, swp_num, track_cap;
+  int step_total, alpha_lines, weno_eps12, psum_check;  // the step diffraction program
+  double in_rho, in_mx, in_e;                           // its post-shock inflow state
   int alpha_in_scheme, roe_eps_double, ydens_plain, eps2_quad_literal, findalpha_ex3;
   int dt_rule_dx53, poscheck_le, success_ge, pbar_check, fail_stop;
 };
+
+// the cells the step program's bc sets: the ghost frame (3 full rows below and
+// above, 3 columns left and right of the interior rows), then column 1 of rows
+// 1 .. 6*step_total-1
+SWF_HD long long step_bc_cells(const KCfg &k) {
+  const long long W = k.nx + 6;
+  const int lim = 6 * k.step_total - 1;
+  const int nres = k.ny < lim ? k.ny : lim;
+  return 6 * W + 6LL * k.ny + (nres > 0 ? nres : 0);
+}
+
+// per-line alpha tiles (the step program): 64 columns x 8 rows of the frame
+constexpr int LA_W = 64, LA_H = 8;
+SWF_HD int la_gx(const KCfg &k) { return (k.nx + 6 + LA_W - 1) / LA_W; }
+SWF_HD int la_gy(const KCfg &k) { return (k.ny + 6 + LA_H - 1) / LA_H; }
 
 // Device-resident scalars of one solver context.
 struct DevScalars {
@@ -80,6 +97,9 @@ struct Bufs {
   unsigned *wpos;                   // their branch bits: SW_REC/32 words x SW_T slots
   int *zs;                          // flux limiter: (den, pre) counts per workgroup of the
                                     // x-flux, stage and edge kernels (zs_pairs)
+  double *arow, *acol;              // per-line alphas: max of row j at arow[j+2], of column
+                                    // i at acol[i+2] (alpha_lines)
+  double *lpart;                    // their partials per 64 x 8 cell tile
 };
 
 // One instance of a batch: its configuration, its buffers outside a stage (Uin ==
@@ -277,22 +297,33 @@ SWF_HD void stage_point_fast(const KCfg &k, const double u[4], double ia, double
 #define SWF_C_2_1 (-1.16666666666666666666666666666667)
 #define SWF_C_2_2 1.83333333333333333333333333333333
 
+// The constant of the WENO weights alpha_k = d_k/(eps + beta_k)**2: eps = 1d-6 as a
+// binary128 double-double (exact mode) and 4*eps in FP64 / FP32 (fast modes).
+struct WenoEps {
+  double hi, lo, q4;
+  float q4f;
+};
+constexpr WenoEps kWeps6 = {qconst::e6_hi, qconst::e6_lo, 4e-6, 4e-6f};
+// ex6_shock_diffraction/weno5.f90:104-106,138-140
+constexpr WenoEps kWeps12 = {qconst::e12_hi, qconst::e12_lo, 4e-12, 4e-12f};
+
 // weno_right: value at x_{i+1/2}; `a` points at average(0).  The exact mode
 // follows weno5.f90:120-146 operation by operation (beta and alpha at binary128);
 // the fast modes use the first-difference forms of swf_weno.h.
 template <int M>
-SWF_HD double weno_right(const double *a) {
+SWF_HD double weno_right(const double *a, const WenoEps &we) {
   if (M != MODE_EXACT) {
     double e[4];
     first_differences(a, e);
-    return a[0] + (M == MODE_FP64 ? weno_d64<true>(e) : weno_d32_or_64<true>(e));
+    return a[0] + (M == MODE_FP64 ? weno_d64<true>(e, we.q4) : weno_d32_or_64<true>(e, we.q4, we.q4f));
   }
   const double um1 = ((0.0 + SWF_C_2_0 * a[-2]) + SWF_C_2_1 * a[-1]) + SWF_C_2_2 * a[0];
   const double um2 = ((0.0 + SWF_C_1_0 * a[-1]) + SWF_C_1_1 * a[0]) + SWF_C_1_2 * a[1];
   const double um3 = ((0.0 + SWF_C_0_0 * a[0]) + SWF_C_0_1 * a[1]) + SWF_C_0_2 * a[2];
   double b[3];
   beta_q(a, b);
-  const double al1 = alpha_q(0.3, b[0]), al2 = alpha_q(0.6, b[1]), al3 = alpha_q(0.1, b[2]);
+  const double al1 = alpha_q(0.3, b[0], we.hi, we.lo), al2 = alpha_q(0.6, b[1], we.hi, we.lo),
+               al3 = alpha_q(0.1, b[2], we.hi, we.lo);
   const double s = (al1 + al2) + al3;
   const double w1 = al1 / s, w2 = al2 / s, w3 = al3 / s;
   return (w1 * um3 + w2 * um2) + w3 * um1;
@@ -300,18 +331,19 @@ SWF_HD double weno_right(const double *a) {
 
 // weno_left: value at x_{i-1/2}; `a` points at average(0) (weno5.f90:86-113)
 template <int M>
-SWF_HD double weno_left(const double *a) {
+SWF_HD double weno_left(const double *a, const WenoEps &we) {
   if (M != MODE_EXACT) {
     double e[4];
     first_differences(a, e);
-    return a[0] + (M == MODE_FP64 ? weno_d64<false>(e) : weno_d32_or_64<false>(e));
+    return a[0] + (M == MODE_FP64 ? weno_d64<false>(e, we.q4) : weno_d32_or_64<false>(e, we.q4, we.q4f));
   }
   const double up1 = ((0.0 + SWF_C_1_0 * a[-2]) + SWF_C_1_1 * a[-1]) + SWF_C_1_2 * a[0];
   const double up2 = ((0.0 + SWF_C_0_0 * a[-1]) + SWF_C_0_1 * a[0]) + SWF_C_0_2 * a[1];
   const double up3 = ((0.0 + SWF_C_M1_0 * a[0]) + SWF_C_M1_1 * a[1]) + SWF_C_M1_2 * a[2];
   double b[3];
   beta_q(a, b);
-  const double al1 = alpha_q(0.1, b[0]), al2 = alpha_q(0.6, b[1]), al3 = alpha_q(0.3, b[2]);
+  const double al1 = alpha_q(0.1, b[0], we.hi, we.lo), al2 = alpha_q(0.6, b[1], we.hi, we.lo),
+               al3 = alpha_q(0.3, b[2], we.hi, we.lo);
   const double s = (al1 + al2) + al3;
   const double w1 = al1 / s, w2 = al2 / s, w3 = al3 / s;
   return (w1 * up3 + w2 * up2) + w3 * up1;
@@ -326,9 +358,11 @@ SWF_HD double weno_left(const double *a) {
 // ---------------------------------------------------------------------------
 // FP64-exact: the reference's operations in its order, structural zeros kept
 // (they decide signs of zero), binary128 sites emulated.
-template <int DIR, bool RIGHT, int SS, int CS>
+// E12: the weights' constant is 1d-12 (the step program's weno5.f90), else 1d-6
+template <int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
 SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   const double g1 = k.gamma1;
+  const WenoEps we = E12 ? kWeps12 : kWeps6;
   const double t0 = k.roe_eps_double ? pa.w / ((pa.w + pb.w) + 1e-15) : roe_weight_q(pa.w, pb.w);
   const double t1 = 1. - t0;
   const double vxm = t0 * pa.vx + t1 * pb.vx;
@@ -377,7 +411,7 @@ SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const
       const double *f = st + ig * SS;
       cu[ig + 2] = ((e[0] * f[0] + e[1] * f[CS]) + e[2] * f[2 * CS]) + e[3] * f[3 * CS];
     }
-    r[m] = RIGHT ? weno_right<MODE_EXACT>(&cu[2]) : weno_left<MODE_EXACT>(&cu[2]);
+    r[m] = RIGHT ? weno_right<MODE_EXACT>(&cu[2], we) : weno_left<MODE_EXACT>(&cu[2], we);
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -391,9 +425,10 @@ SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const
 //     out = f(0) + R * weno_d(L * (first differences of f))
 // which is R * weno(L * f) in exact arithmetic (R = L^-1, L linear).  Only the
 // correction passes through the eigenvectors and the nonlinear weights.
-template <int M, int DIR, bool RIGHT, int SS, int CS>
+template <int M, int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
 SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   const double g1 = k.gamma1;
+  const WenoEps we = E12 ? kWeps12 : kWeps6;
   const double t0 = pa.w * rcp64((pa.w + pb.w) + 1e-15);
   const double t1 = 1. - t0;
   const double vxm = t0 * pa.vx + t1 * pb.vx;
@@ -423,7 +458,8 @@ SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const 
   }
   double r[4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) r[m] = (M == MODE_FP64) ? weno_d64<RIGHT>(e[m]) : weno_d32_or_64<RIGHT>(e[m]);
+  for (int m = 0; m < 4; ++m)
+    r[m] = (M == MODE_FP64) ? weno_d64<RIGHT>(e[m], we.q4) : weno_d32_or_64<RIGHT>(e[m], we.q4, we.q4f);
   // R * r with the right eigenvector columns (1, vn-c, vt, H-vn c), (0, 0, 1, vt),
   // (1, vn, vt, q), (1, vn+c, vt, H+vn c)
   const double s = (r[0] + r[2]) + r[3], dr = r[3] - r[0];
@@ -442,12 +478,12 @@ SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const 
   for (int c = 0; c < 4; ++c) out[c] = st[c * CS] + corr[c];
 }
 
-template <int M, int DIR, bool RIGHT, int SS, int CS>
+template <int M, int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
 SWF_HD void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   if (M == MODE_EXACT)
-    half_flux_exact<DIR, RIGHT, SS, CS>(k, pa, pb, st, out);
+    half_flux_exact<DIR, RIGHT, SS, CS, E12>(k, pa, pb, st, out);
   else
-    half_flux_fast<M, DIR, RIGHT, SS, CS>(k, pa, pb, st, out);
+    half_flux_fast<M, DIR, RIGHT, SS, CS, E12>(k, pa, pb, st, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -3,6 +3,8 @@
 // -ffp-contract=off: every product and sum is rounded separately, as in the
 // reference build (x86-64 SSE2 without FMA).
 #define SWF_COMMON_KERNELS 1
+#include <algorithm>
+
 #include "swf_kernels.h"
 #include "swf_launch.h"
 
@@ -30,6 +32,9 @@ hipError_t launch_run1d(int mode, const K1Cfg &k, const B1 &b, hipStream_t s, lo
 void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
   if (k.use_limiter == 1)
     hipLaunchKernelGGL((k_xflux<MODE_EXACT, true>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
+  else if (k.alpha_lines)
+    hipLaunchKernelGGL((k_xflux<MODE_EXACT, false, true>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in,
+                       parity);
   else
     hipLaunchKernelGGL((k_xflux<MODE_EXACT, false>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
 }
@@ -37,6 +42,9 @@ void launch_ystage_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, in
   if (k.use_limiter == 1)
     hipLaunchKernelGGL((k_ystage<MODE_EXACT, true>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity,
                        flags);
+  else if (k.alpha_lines)
+    hipLaunchKernelGGL((k_ystage<MODE_EXACT, false, true>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in,
+                       parity, flags);
   else
     hipLaunchKernelGGL((k_ystage<MODE_EXACT, false>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity,
                        flags);
@@ -176,8 +184,16 @@ void launch_alpha_all(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out)
   hipLaunchKernelGGL(k_alpha_all, dim3((k.nx + 6 + 63) / 64, (k.ny + 6 + 3) / 4), dim3(64, 4), 0, s, k, b, slot_out);
 }
 void launch_bc(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out, int parity) {
-  const long long total = 2LL * 3 * (k.nx + 6) + 2LL * 3 * k.ny;
+  const long long total = k.bc == BC_STEP ? step_bc_cells(k) : 2LL * 3 * (k.nx + 6) + 2LL * 3 * k.ny;
   hipLaunchKernelGGL(k_bc, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, k, b, slot_out, parity);
+}
+void launch_line_alpha(const KCfg &k, const Bufs &b, hipStream_t s, int slot, int parity) {
+  hipLaunchKernelGGL(k_line_alpha, dim3(la_gx(k), la_gy(k)), dim3(256), 0, s, k, b, parity);
+  hipLaunchKernelGGL(k_line_alpha_reduce, dim3(1), dim3(1024), 0, s, k, b, slot, parity);
+}
+void launch_step_reflect(const KCfg &k, const Bufs &b, hipStream_t s, int dir, int parity) {
+  const int n = dir == 0 ? 3 * (std::min(k.ny + 1, 6 * k.step_total) + 1) : 3 * k.step_total;
+  hipLaunchKernelGGL(k_step_reflect, dim3((n + 255) / 256), dim3(256), 0, s, k, b, dir, parity);
 }
 void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s) {
   hipLaunchKernelGGL(k_from_fortran, dim3(1024), dim3(256), 0, s, k, src, dst);

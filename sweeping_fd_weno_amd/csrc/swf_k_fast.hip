@@ -25,13 +25,21 @@ hipError_t launch_run1d_m12(int mode, const K1Cfg &k, const B1 &b, hipStream_t s
 
 // the flux limiter (use_limiter = 1) has kernels of its own, so that the others
 // carry none of its registers
+// so has the step program's wenofd (alpha_lines: per-line alphas, 1d-12 in the weights)
 template <int M, bool LIM>
 static void xflux(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
-  hipLaunchKernelGGL((k_xflux<M, LIM>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
+  if (!LIM && k.alpha_lines)
+    hipLaunchKernelGGL((k_xflux<M, false, true>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
+  else
+    hipLaunchKernelGGL((k_xflux<M, LIM>), grid_xflux(k), dim3(256), 0, s, k, b, stage, slot_in, parity);
 }
 template <int M, bool LIM>
 static void ystage(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags) {
-  hipLaunchKernelGGL((k_ystage<M, LIM>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity, flags);
+  if (!LIM && k.alpha_lines)
+    hipLaunchKernelGGL((k_ystage<M, false, true>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity,
+                       flags);
+  else
+    hipLaunchKernelGGL((k_ystage<M, LIM>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity, flags);
 }
 
 void launch_xflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {

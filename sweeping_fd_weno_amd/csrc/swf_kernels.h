@@ -1,10 +1,10 @@
 // swf_kernels.h - grid kernels of one SSP-RK3 stage, templated on precision mode.
 //
 // One stage (ex4_sedov_2d/main.f90:685-777 `rk`) is three launches:
-//   k_xflux<M,LIM> wenofd x-pass: net x-interface flux F(i+1/2) = fp_flux(i) - fn_flux(i+1)
+//   k_xflux<M,LIM,STEP> wenofd x-pass: net x-interface flux F(i+1/2) = fp_flux(i) - fn_flux(i+1)
 //                (scaled by alpha as :414-415) into Fx; also clears the stage's
 //                sweep bitmaps and flags
-//   k_ystage<M,LIM> wenofd y-pass for a tile, then per cell `scheme` (RHS from Fx and
+//   k_ystage<M,LIM,STEP> wenofd y-pass for a tile, then per cell `scheme` (RHS from Fx and
 //                the tile's y fluxes), the RK combination, checkpositivity, sweep
 //                flags, ghost images (`bc`) and the tile's alpha max (findalpha)
 //   k_stage_end  (mode-independent, one workgroup) sweep_lim when flagged, alpha
@@ -32,6 +32,7 @@ constexpr int XT_W = 64, XT_H = 4, XT_PTS = XT_W + 5;
 constexpr int YT_W = UB_W, YT_H = UB_H + 1, YT_PTS = YT_H + 5;
 
 SWF_D double alpha_from_bits(unsigned long long b) { return __longlong_as_double((long long)b); }
+
 
 // LDS of a flux tile, component-major so that consecutive lanes read
 // consecutive doubles: split fluxes f+ (fp) and f- (fn) of every stencil point,
@@ -75,17 +76,21 @@ SWF_D void stage_point(const KCfg &k, const double u[4], double alpha, double ia
 // both loads are issued before either point is processed, so the tile waits for
 // one memory round trip.  point(p, i, j) maps a point to its cell and returns
 // false outside the grid; store(p, fp, fn, pr) writes the staged values.
-template <int M, int DIR, int NPTS, class PointF, class StoreF>
-SWF_D void stage_tile(const KCfg &k, const Bufs &b, double alpha, double ia, int tid, PointF point, StoreF store) {
+// STEP (the step program, alpha_lines): the split of a point uses the alpha of its
+// line, row j for the x-pass (aline[j+2]) and column i for the y-pass (aline[i+2]).
+template <int M, int DIR, int NPTS, bool STEP, class PointF, class StoreF>
+SWF_D void stage_tile(const KCfg &k, const Bufs &b, double alpha, double ia, const double *aline, int tid,
+                      PointF point, StoreF store) {
   static_assert(NPTS <= 512, "at most two points per thread");
   double u[2][4];
   bool ok[2];
-  int pp[2];
+  int pp[2], line[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     pp[s] = tid + 256 * s;
     int i = 0, j = 0;
     ok[s] = pp[s] < NPTS && point(pp[s], i, j);
+    line[s] = DIR == 0 ? j : i;
     if (ok[s]) {
       const long long o = cidx(k, i, j);
 #pragma unroll
@@ -96,7 +101,12 @@ SWF_D void stage_tile(const KCfg &k, const Bufs &b, double alpha, double ia, int
   for (int s = 0; s < 2; ++s) {
     if (ok[s]) {
       double fp[4], fn[4], pr[4];
-      stage_point<M, DIR>(k, u[s], alpha, ia, fp, fn, pr);
+      double al = alpha, ial = ia;
+      if (STEP) {
+        al = aline[line[s] + 2];
+        ial = 1.0 / al;
+      }
+      stage_point<M, DIR>(k, u[s], al, ial, fp, fn, pr);
       store(pp[s], fp, fn, pr);
     }
   }
@@ -117,13 +127,13 @@ SWF_D Prim load_prim(const double *pr) {
 // point p, Roe(p, p+1)) minus f- of the next point (centre p+1, Roe(p+1, p+2)),
 // scaled by alpha inside wenofd (ex4) or later in `scheme` (ex3/ex5).  Pointers
 // are at component 0 of point p; points SS doubles apart, components CS.
-template <int M, int DIR, int SS, int CS, bool LIM>
+template <int M, int DIR, int SS, int CS, bool LIM, bool E12 = false>
 SWF_D int iface_flux(const KCfg &k, const double *pr, const double *fp, const double *fn, double alpha,
                      double F[4]) {
   const Prim p0 = load_prim<CS>(pr), p1 = load_prim<CS>(pr + SS), p2 = load_prim<CS>(pr + 2 * SS);
   double a[4], c[4];
-  half_flux<M, DIR, true, SS, CS>(k, p0, p1, fp, a);
-  half_flux<M, DIR, false, SS, CS>(k, p1, p2, fn + SS, c);
+  half_flux<M, DIR, true, SS, CS, E12>(k, p0, p1, fp, a);
+  half_flux<M, DIR, false, SS, CS, E12>(k, p1, p2, fn + SS, c);
   int lim = 0;
   if (LIM) lim = zs_limit<CS>(k.gamma1, fp, a) | (zs_limit<CS>(k.gamma1, fn + SS, c) << 2);
 #pragma unroll
@@ -163,7 +173,10 @@ SWF_D void zs_store(int den, int pre, int *pair, int *red, int tid) {
 // stage's sweep bitmaps and flags (the stage kernel sets them), and at stage 1
 // of an inactive step keeps the next step inactive.
 // Tile (bx, by) of a gx x gy grid of x-flux tiles.
-template <int M, bool LIM>
+// STEP: the step diffraction program's wenofd (ex6_shock_diffraction/main.f90:
+// 170-647): each row (x-pass) / column (y-pass) split and scaled with its own
+// alpha max, and 1d-12 in the WENO weights.
+template <int M, bool LIM, bool STEP = false>
 SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int parity, int bx, int by, int gx,
                       int gy, XSmem &sm) {
   constexpr int CS = XSmem::CS;
@@ -188,8 +201,9 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
   const double ia = 1.0 / alpha;
   const int k0 = bx * XT_W - XT_SHIFT, j0 = 1 + by * XT_H;
-  stage_tile<M, 0, XT_H * XT_PTS>(
-      k, b, alpha, ia, tid,
+  const double *aline = STEP ? b.arow : nullptr;
+  stage_tile<M, 0, XT_H * XT_PTS, STEP>(
+      k, b, alpha, ia, aline, tid,
       [&](int p, int &i, int &j) {
         const int r = p / XT_PTS;
         i = k0 - 2 + (p - r * XT_PTS);
@@ -211,7 +225,8 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
   int lim = 0;
   if (kk >= 0 && kk <= k.nx && j <= k.ny) {
     double F[4];
-    lim = iface_flux<M, 0, 1, CS, LIM>(k, &sm.pr[0][ty][tx + 2], &sm.fp[0][ty][tx + 2], &sm.fn[0][ty][tx + 2], alpha, F);
+    lim = iface_flux<M, 0, 1, CS, LIM, STEP>(k, &sm.pr[0][ty][tx + 2], &sm.fp[0][ty][tx + 2], &sm.fn[0][ty][tx + 2],
+                                       STEP ? aline[j + 2] : alpha, F);
     const long long o = cidx(k, kk, j);
 #pragma unroll
     for (int c = 0; c < 4; ++c) b.Fx[c * k.plane + o] = F[c];
@@ -219,10 +234,10 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
   if (LIM && stage > 0) zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (bx + by * gx), sm.zred, tid);
 }
 
-template <int M, bool LIM>
+template <int M, bool LIM, bool STEP = false>
 __global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_in, int parity) {
   __shared__ XSmem sm;
-  xflux_tile<M, LIM>(k, b, stage, slot_in, parity, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, sm);
+  xflux_tile<M, LIM, STEP>(k, b, stage, slot_in, parity, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, sm);
 }
 
 // dt of the step (ex4_sedov_2d/main.f90:696-700; ex3 :570-572)
@@ -269,6 +284,8 @@ SWF_D void write_ghosts(const KCfg &k, const Bufs &b, int i, int j, const double
       if (i <= 3) store_ij(k, b, i + k.nx, j, v);
       if (j >= k.ny - 2) store_ij(k, b, i, j - k.ny, v);
       if (j <= 3) store_ij(k, b, i, j + k.ny, v);
+      break;
+    case BC_STEP:  // the whole frame and column 1 after the stage (step_frame)
       break;
     default:  // jets
       if (i == 1) {
@@ -389,7 +406,7 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
 // flags: YS_RHS_ONLY stores RHS into b.R instead (parity tests of the spatial
 // operator); YS_NO_SCALARS leaves the device scalars alone (kernel timing).
 // Stage tile (bx, by) of a grid gx tiles wide.
-template <int M, bool LIM>
+template <int M, bool LIM, bool STEP = false>
 SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int parity, int flags, int bx, int by,
                        int gx, YSmem &sm) {
   DevScalars *g = b.g;
@@ -406,8 +423,9 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
   const int i = i0 + tx, jf = jf0 + ty;
   const int j = jf;
   const bool in = ty >= 1 && i <= k.nx && j <= k.ny;
-  stage_tile<M, 1, YT_PTS * YT_W>(
-      k, b, alpha, ia, tid,
+  const double *aline = STEP ? b.acol : nullptr;
+  stage_tile<M, 1, YT_PTS * YT_W, STEP>(
+      k, b, alpha, ia, aline, tid,
       [&](int p, int &i, int &j) {
         const int r = p / YT_W;
         i = i0 + (p - r * YT_W);
@@ -427,8 +445,8 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
   double G[4] = {0.0, 0.0, 0.0, 0.0};
   int lim = 0;
   if (i <= k.nx && jf <= k.ny)
-    lim = iface_flux<M, 1, YT_W, YSmem::CS, LIM>(k, &sm.pr[0][ty + 2][tx], &sm.fp[0][ty + 2][tx], &sm.fn[0][ty + 2][tx],
-                                            alpha, G);
+    lim = iface_flux<M, 1, YT_W, YSmem::CS, LIM, STEP>(k, &sm.pr[0][ty + 2][tx], &sm.fp[0][ty + 2][tx], &sm.fn[0][ty + 2][tx],
+                                            STEP ? aline[i + 2] : alpha, G);
   // a tile's first interface is the previous tile's last: counted there
   if (ty == 0 && by > 0) lim = 0;
 #pragma unroll
@@ -525,10 +543,10 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
     zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (xf_gx(k) * xf_gy(k) + bx + by * gx), sm.zred, tid);
 }
 
-template <int M, bool LIM>
+template <int M, bool LIM, bool STEP = false>
 __global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_in, int parity, int flags) {
   __shared__ YSmem sm;
-  ystage_tile<M, LIM>(k, b, stage, slot_in, parity, flags, blockIdx.x, blockIdx.y, gridDim.x, sm);
+  ystage_tile<M, LIM, STEP>(k, b, stage, slot_in, parity, flags, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
 
 // One limited half flux outside the flux tiles (use_limiter == 1): the half of
@@ -645,6 +663,66 @@ __global__ SWF_FLUX_BOUNDS(M) void k_ystage_batch(const BatchInst *inst, const i
   ystage_tile<M, false>(I.k, batch_stage_bufs(I, stage), stage, slot_in, parity, flags, e.y % gx, e.y / gx, gx, sm);
 }
 
+// ---------------------------------------------------------------------------
+// The step diffraction program (ex6_shock_diffraction/main.f90)
+// ---------------------------------------------------------------------------
+
+// Its bc (:888-942) on one domain, in the program's order: fixed gas at rest
+// (gamma, 0, 0, 1/gamma1) below and to the right, zero gradient above, the
+// post-shock inflow on the left, then the loop meant for the step's face, which
+// reads i after its loop has ended (i = 1) and resets column 1 to (1, 0, 0,
+// 1/gamma1) in every row below 6*total.  Value of cell (i, j) after these passes;
+// false for a cell they leave alone.
+SWF_D bool step_bc_value(const KCfg &k, const Bufs &b, int i, int j, double v[4]) {
+  const int T = k.step_total;
+  if (i <= 0) {
+    v[0] = k.in_rho; v[1] = k.in_mx; v[2] = 0.; v[3] = k.in_e;
+  } else if (i == 1 && j < 6 * T && 1 < T) {
+    v[0] = 1.; v[1] = 0.; v[2] = 0.; v[3] = 1. / k.gamma1;
+  } else if (j <= 0 || i > k.nx) {
+    v[0] = k.gamma; v[1] = 0.; v[2] = 0.; v[3] = 1. / k.gamma1;
+  } else if (j > k.ny) {
+    load_ij(k, b, i, k.ny, v);
+  } else {
+    return false;
+  }
+  return true;
+}
+
+SWF_D void step_bc_cell(const KCfg &k, long long t, int &i, int &j) {
+  const long long W = k.nx + 6, nrow = 3 * W, nside = 3LL * k.ny;
+  if (t < nrow) {
+    j = -2 + (int)(t / W);
+    i = -2 + (int)(t % W);
+  } else if (t < 2 * nrow) {
+    j = k.ny + 1 + (int)((t - nrow) / W);
+    i = -2 + (int)((t - nrow) % W);
+  } else if (t < 2 * nrow + nside) {
+    const long long s = t - 2 * nrow;
+    j = 1 + (int)(s / 3);
+    i = -2 + (int)(s % 3);
+  } else if (t < 2 * nrow + 2 * nside) {
+    const long long s = t - 2 * nrow - nside;
+    j = 1 + (int)(s / 3);
+    i = k.nx + 1 + (int)(s % 3);
+  } else {
+    i = 1;
+    j = 1 + (int)(t - 2 * nrow - 2 * nside);
+  }
+}
+
+// the bc by nthreads threads from thread t0 (the top rows copy interior row ny,
+// which the bc itself does not write)
+SWF_D void step_frame(const KCfg &k, const Bufs &b, long long t0, long long nthreads) {
+  const long long n = step_bc_cells(k);
+  for (long long t = t0; t < n; t += nthreads) {
+    int i, j;
+    step_bc_cell(k, t, i, j);
+    double v[4];
+    if (step_bc_value(k, b, i, j, v)) store_ij(k, b, i, j, v);
+  }
+}
+
 #ifdef SWF_COMMON_KERNELS
 // Mode-independent grid kernels, compiled once (swf_k_exact.hip).
 
@@ -702,6 +780,8 @@ SWF_D void stage_end_body(const KCfg &k, const Bufs &b, int stage, int slot_out,
     a = 0.0;
     for (int q = tid; q < nblk; q += SW_T) a = fmax2(a, b.apart[q]);
   }
+  // the step program's bc follows checkpositivity and the sweep (:736-769)
+  if (k.bc == BC_STEP) step_frame(k, b, tid, SW_T);
   a = block_max<SW_T>(a, red, tid);
   if (tid == 0) {
     g->alpha_bits[slot_out] = (unsigned long long)__double_as_longlong(fmax2(a, k.alpha_const));
@@ -769,6 +849,21 @@ __global__ __launch_bounds__(256) void k_alpha_all(KCfg k, Bufs b, int slot_out)
 __global__ __launch_bounds__(256) void k_bc(KCfg k, Bufs b, int slot_out, int parity) {
   DevScalars *g = b.g;
   if (parity >= 0 && !g->active[parity]) return;
+  if (k.bc == BC_STEP) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    double a = 0.0;
+    if (t < step_bc_cells(k)) {
+      int i, j;
+      step_bc_cell(k, t, i, j);
+      double v[4];
+      if (step_bc_value(k, b, i, j, v)) {
+        store_ij(k, b, i, j, v);
+        a = cell_alpha(k, v[0], v[1], v[2], v[3]);
+      }
+    }
+    if (slot_out >= 0) alpha_atomic_max(&g->alpha_bits[slot_out], a);
+    return;
+  }
   const int W = k.nx + 6;
   const long long nrow = 3LL * W;                 // cells in 3 full ghost rows
   const long long nside = 3LL * k.ny;             // cells in 3 ghost columns of interior rows
@@ -845,6 +940,106 @@ __global__ __launch_bounds__(256) void k_bc(KCfg k, Bufs b, int slot_out, int pa
   if (slot_out >= 0) alpha_atomic_max(&g->alpha_bits[slot_out], a);
 }
 
+// Per-line alphas (ex6_shock_diffraction/main.f90:217-219, :449-451): findalpha's
+// alpha(i,j) of the stage input over every cell, the max of each row and each
+// column of a 64 x 8 tile into lpart (rows first: la_gx x (ny+6), then columns:
+// la_gy x (nx+6)), then k_line_alpha_reduce.
+__global__ __launch_bounds__(256) void k_line_alpha(KCfg k, Bufs b, int parity) {
+  __shared__ double cred[4][LA_W];
+  const DevScalars *g = b.g;
+  if (parity >= 0 && (!g->active[parity] || g->status != ST_OK)) return;
+  const int tx = threadIdx.x & (LA_W - 1), ty = threadIdx.x / LA_W;
+  const int i = -2 + blockIdx.x * LA_W + tx;
+  double colmax = 0.0;
+  for (int r = ty; r < LA_H; r += 4) {
+    const int j = -2 + blockIdx.y * LA_H + r;
+    double a = 0.0;
+    if (i <= k.nx + 3 && j <= k.ny + 3) {
+      const long long o = cidx(k, i, j);
+      a = cell_alpha(k, b.Uin[o], b.Uin[k.plane + o], b.Uin[2 * k.plane + o], b.Uin[3 * k.plane + o]);
+    }
+    colmax = fmax2(colmax, a);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double o = __shfl_xor(a, off, 64);
+      a = (o > a) ? o : a;
+    }
+    if (tx == 0 && j <= k.ny + 3) b.lpart[(long long)blockIdx.x * (k.ny + 6) + (j + 2)] = a;
+  }
+  cred[ty][tx] = colmax;
+  __syncthreads();
+  if (ty == 0 && i <= k.nx + 3) {
+    const double m = fmax2(fmax2(cred[0][tx], cred[1][tx]), fmax2(cred[2][tx], cred[3][tx]));
+    b.lpart[(long long)la_gx(k) * (k.ny + 6) + (long long)blockIdx.y * (k.nx + 6) + (i + 2)] = m;
+  }
+}
+
+// arow, acol and alphamax (into alpha_bits[slot]) from the tile partials; one workgroup
+__global__ __launch_bounds__(1024) void k_line_alpha_reduce(KCfg k, Bufs b, int slot, int parity) {
+  __shared__ double red[16];
+  DevScalars *g = b.g;
+  if (parity >= 0 && (!g->active[parity] || g->status != ST_OK)) return;
+  const int nyg = k.ny + 6, nxg = k.nx + 6, gxa = la_gx(k), gya = la_gy(k);
+  double gm = 0.0;
+  for (int t = threadIdx.x; t < nyg + nxg; t += 1024) {
+    double m = 0.0;
+    if (t < nyg) {
+      for (int q = 0; q < gxa; ++q) m = fmax2(m, b.lpart[(long long)q * nyg + t]);
+      b.arow[t] = m;
+    } else {
+      const int c = t - nyg;
+      for (int q = 0; q < gya; ++q) m = fmax2(m, b.lpart[(long long)gxa * nyg + (long long)q * nxg + c]);
+      b.acol[c] = m;
+    }
+    gm = fmax2(gm, m);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(gm, off, 64);
+    gm = (o > gm) ? o : gm;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = gm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < 16; ++w) m = fmax2(m, red[w]);
+    g->alpha_bits[slot] = (unsigned long long)__double_as_longlong(m);
+  }
+}
+
+// The step's walls inside wenofd (ex6_shock_diffraction/main.f90:188-197, :417-431),
+// on the stage input: dir 0, before the x-pass, the cells (T-2..T, j) mirror
+// (T+3..T+1, j) with m1 negated in rows 0 .. min(ny+1, 6T); dir 1, before the
+// y-pass, the cells (i, 6T-2..6T) mirror (i, 6T+3..6T+1) with m2 negated in
+// columns 1..T.  Sources and targets are disjoint.
+__global__ __launch_bounds__(256) void k_step_reflect(KCfg k, Bufs b, int dir, int parity) {
+  const DevScalars *g = b.g;
+  if (parity >= 0 && (!g->active[parity] || g->status != ST_OK)) return;
+  const int T = k.step_total;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  int di, dj, si, sj;
+  if (dir == 0) {
+    const int nr = min(k.ny + 1, 6 * T) + 1;  // rows 0 .. min(ny+1, 6T)
+    if (t >= 3 * nr) return;
+    const int r = t / 3, q = t - 3 * r;
+    dj = sj = r;
+    di = T - q;
+    si = T + 1 + q;
+  } else {
+    if (t >= 3 * T) return;
+    const int r = t / T, c = t - r * T;
+    di = si = 1 + c;
+    dj = 6 * T - r;
+    sj = 6 * T + 1 + r;
+  }
+  const long long so = cidx(k, si, sj), d = cidx(k, di, dj);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double v = b.Uin[c * k.plane + so];
+    b.Uin[c * k.plane + d] = (c == 1 + dir) ? -v : v;
+  }
+}
+
 // Fortran-layout u(4,-2:nx+3,-2:ny+3) <-> SoA planes
 __global__ void k_from_fortran(KCfg k, const double *__restrict__ src, double *__restrict__ dst) {
   const long long n = (long long)(k.nx + 6) * (k.ny + 6);
@@ -878,8 +1073,8 @@ __global__ void k_weno_points(const double *__restrict__ a, double *__restrict__
   double s[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) s[q] = a[5 * t + q];
-  right[t] = weno_right<M>(&s[2]);
-  left[t] = weno_left<M>(&s[2]);
+  right[t] = weno_right<M>(&s[2], kWeps6);
+  left[t] = weno_left<M>(&s[2], kWeps6);
 }
 
 }  // namespace swf

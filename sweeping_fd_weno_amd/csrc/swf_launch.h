@@ -54,6 +54,9 @@ void launch_flags(const KCfg &k, const Bufs &b, hipStream_t s);
 void launch_begin_run(const Bufs &b, hipStream_t s, double time, double tfinal, int active);
 void launch_set_alpha(const Bufs &b, hipStream_t s, int slot, unsigned long long bits, int p, int active);
 void launch_upload_reset(const Bufs &b, hipStream_t s);
+// the step program: per-line alphas of the stage input (two kernels) and the walls
+void launch_line_alpha(const KCfg &k, const Bufs &b, hipStream_t s, int slot, int parity);
+void launch_step_reflect(const KCfg &k, const Bufs &b, hipStream_t s, int dir, int parity);
 void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 

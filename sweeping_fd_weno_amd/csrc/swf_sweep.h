@@ -209,6 +209,11 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
     double vn2[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) vn2[c] = vn[c] + wr * (v[c] - vt[c]);
+    // the step program's check of an order-I adjustment (ex6_shock_diffraction/
+    // main.f90:1418-1426, :1477-1485): total pressure of the pair must not rise
+    if (order == 0 && k.psum_check &&
+        pv + pn > (sweep_pres(g1, vt[0], vt[1], vt[2], vt[3]) + sweep_pres(g1, vn2[0], vn2[1], vn2[2], vn2[3])) + eps)
+      b.g->status = ST_PSUM;
     if (WRITE) {
       store_ij(k, b, i, j, vt);
       mark_dirty(k, *sh, i, j);

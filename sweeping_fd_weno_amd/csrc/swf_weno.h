@@ -20,8 +20,8 @@
 // correction, never a(0).
 //
 // Weights: alpha_k = d_k/(1e-6+beta_k)^2 normalised.  Both forms scale beta by 4
-// (4 beta_k = 13/3 x^2 + y^2, with 4e-6 in place of 1e-6), which leaves the
-// weights unchanged.
+// (4 beta_k = 13/3 x^2 + y^2, with q4 = 4e-6 in place of 1e-6), which leaves the
+// weights unchanged.  The step diffraction program's weno5.f90 has 1e-12 (q4 = 4e-12).
 //   FP64: a common denominator, alpha_k ~ d_k * s_j * s_l with s = (4e-6+4beta)^2,
 //         so one reciprocal per reconstruction.
 //   FP32: alpha_k = d_k (min(q)/q_k)^2 with q = 4e-6+4beta (no overflow for any
@@ -59,15 +59,15 @@ SWF_HD double rcp64(double x) {
 
 // weno_right(a) - a(0) (RIGHT) or weno_left(a) - a(0), FP64
 template <bool RIGHT>
-SWF_HD double weno_d64(const double e[4]) {
+SWF_HD double weno_d64(const double e[4], double q4) {
   const double em2 = e[0], em1 = e[1], e0 = e[2], e1 = e[3];
   const double x1 = e1 - e0, y1 = e1 - 3.0 * e0;
   const double x2 = e0 - em1, y2 = em1 + e0;
   const double x3 = em1 - em2, y3 = 3.0 * em1 - em2;
   const double c = 13.0 / 3.0;
-  const double q1 = 4e-6 + ((c * x1) * x1 + y1 * y1);
-  const double q2 = 4e-6 + ((c * x2) * x2 + y2 * y2);
-  const double q3 = 4e-6 + ((c * x3) * x3 + y3 * y3);
+  const double q1 = q4 + ((c * x1) * x1 + y1 * y1);
+  const double q2 = q4 + ((c * x2) * x2 + y2 * y2);
+  const double q3 = q4 + ((c * x3) * x3 + y3 * y3);
   const double d1 = RIGHT ? 0.3 : 0.1, d2 = 0.6, d3 = RIGHT ? 0.1 : 0.3;
   const double c3 = RIGHT ? 4.0 * e0 - e1 : 2.0 * e1 - 5.0 * e0;
   const double c2 = RIGHT ? em1 + 2.0 * e0 : -(2.0 * em1 + e0);
@@ -90,11 +90,11 @@ SWF_HD double weno_d64(const double e[4]) {
 // be negligible.  The smallest-q stencil has alpha = d_k >= 0.1, so the sum never
 // vanishes.  Requires max |x|, |y| <= 1e18 (then 4 beta < 1e38).
 template <bool RIGHT>
-SWF_HD void alphas_d32(const float x[3], const float y[3], float a[3]) {
+SWF_HD void alphas_d32(const float x[3], const float y[3], float a[3], float q4) {
   const float c = 13.0f / 3.0f;
-  const float q1 = 4e-6f + ((c * x[0]) * x[0] + y[0] * y[0]);
-  const float q2 = 4e-6f + ((c * x[1]) * x[1] + y[1] * y[1]);
-  const float q3 = 4e-6f + ((c * x[2]) * x[2] + y[2] * y[2]);
+  const float q1 = q4 + ((c * x[0]) * x[0] + y[0] * y[0]);
+  const float q2 = q4 + ((c * x[1]) * x[1] + y[1] * y[1]);
+  const float q3 = q4 + ((c * x[2]) * x[2] + y[2] * y[2]);
   const float m = fminf(q1, fminf(q2, q3));
   const float r1 = m * rcp32(q1), r2 = m * rcp32(q2), r3 = m * rcp32(q3);
   a[0] = (RIGHT ? 0.3f : 0.1f) * (r1 * r1);
@@ -109,7 +109,7 @@ SWF_HD void alphas_d32(const float x[3], const float y[3], float a[3]) {
 // candidates with each weight perturbed by ~1e-7 of itself.  Falls back to
 // weno_d64 beyond FP32 range.
 template <bool RIGHT>
-SWF_HD double weno_d32_or_64(const double e[4]) {
+SWF_HD double weno_d32_or_64(const double e[4], double q4, float q4f) {
   const double em2 = e[0], em1 = e[1], e0 = e[2], e1 = e[3];
   const double x1 = e1 - e0, y1 = e1 - 3.0 * e0;
   const double x2 = e0 - em1, y2 = em1 + e0;
@@ -117,9 +117,9 @@ SWF_HD double weno_d32_or_64(const double e[4]) {
   const float x[3] = {(float)x1, (float)x2, (float)x3}, y[3] = {(float)y1, (float)y2, (float)y3};
   const float big = fmaxf(fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(y[0]))),
                           fmaxf(fabsf(y[1]), fabsf(y[2])));
-  if (!(big <= 1e18f)) return weno_d64<RIGHT>(e);
+  if (!(big <= 1e18f)) return weno_d64<RIGHT>(e, q4);
   float al[3];
-  alphas_d32<RIGHT>(x, y, al);
+  alphas_d32<RIGHT>(x, y, al, q4f);
   const double a1 = al[0], a2 = al[1], a3 = al[2];
   const double c3 = RIGHT ? 4.0 * e0 - e1 : 2.0 * e1 - 5.0 * e0;
   const double c2 = RIGHT ? em1 + 2.0 * e0 : -(2.0 * em1 + e0);

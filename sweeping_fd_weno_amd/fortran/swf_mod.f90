@@ -12,8 +12,8 @@ module swf
 
    integer(c_int32_t), parameter :: SWF_FP64_EXACT = 0, SWF_FP64 = 1, SWF_FP32_BULK = 2
    integer(c_int32_t), parameter :: SWF_BC_SEDOV = 0, SWF_BC_PERIODIC = 1, SWF_BC_JET_HALF = 2, &
-                                    SWF_BC_JET_FULL = 3
-   integer(c_int), parameter :: SWF_OK = 0, SWF_E_PBAR = 1, SWF_E_SWEEP_FAIL = 2
+                                    SWF_BC_JET_FULL = 3, SWF_BC_STEP = 4
+   integer(c_int), parameter :: SWF_OK = 0, SWF_E_PBAR = 1, SWF_E_SWEEP_FAIL = 2, SWF_E_PSUM = 3
 
    type, bind(C) :: swf_config
       integer(c_int32_t) :: nx, ny
@@ -23,6 +23,8 @@ module swf
       integer(c_int32_t) :: alpha_in_scheme, roe_eps_double, ydens_plain, eps2_quad_literal, findalpha_ex3
       integer(c_int32_t) :: dt_rule_dx53, poscheck_le, success_ge, pbar_check, fail_stop
       real(c_double) :: jet_rho, jet_u, amb_rho, amb_p, jet_halfwidth
+      integer(c_int32_t) :: step_total, alpha_lines, weno_eps12, psum_check
+      real(c_double) :: in_rho, in_u, in_p
       integer(c_int32_t) :: precision, device
    end type swf_config
 

@@ -111,6 +111,13 @@ program swf_vortex
    cfg%amb_rho = 0.
    cfg%amb_p = 0.
    cfg%jet_halfwidth = 0.
+   cfg%step_total = 0
+   cfg%alpha_lines = 0
+   cfg%weno_eps12 = 0
+   cfg%psum_check = 0
+   cfg%in_rho = 0.
+   cfg%in_u = 0.
+   cfg%in_p = 0.
    cfg%precision = prec
    cfg%device = 0
    rc = swf_create(cfg, ctx)

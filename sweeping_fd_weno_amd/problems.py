@@ -95,6 +95,27 @@ def init_jet(cfg) -> np.ndarray:
     return u
 
 
+def init_step(cfg) -> np.ndarray:
+    """initdata of the step program (ex6_shock_diffraction/main.f90:1111-1131) on one
+    domain: gas at rest (gamma, 0, 0, 1/gamma1) and the inflow state (cfg.in_rho, in_u,
+    in_p) where i <= total/2 and j >= 6*total, ghosts included.  The program applies no
+    bc before its first step."""
+    T = cfg.step_total
+    g1 = cfg.gamma - 1.0
+    u = np.zeros(state_shape(cfg))
+    u[:, :, 0] = cfg.gamma
+    u[:, :, 3] = 1.0 / g1
+    dle, ule, ple = cfg.in_rho, cfg.in_u, cfg.in_p
+    jj = np.arange(-2, cfg.ny + 4)[:, None]
+    ii = np.arange(-2, cfg.nx + 4)[None, :]
+    shocked = (ii <= T // 2) & (jj >= 6 * T)
+    u[shocked, 0] = dle
+    u[shocked, 1] = ule * dle
+    u[shocked, 2] = 0.0
+    u[shocked, 3] = ple / g1 + (0.5 * dle) * (ule * ule)
+    return u
+
+
 def _round_bits(x, p: int):
     """x (a positive Fraction) rounded to p significant bits, ties to even."""
     from fractions import Fraction

@@ -141,6 +141,29 @@ def batch_vs_single(prec, nsteps):
     return out
 
 
+def step_run(prec, total, nsteps, mach=5.09):
+    """The step program (no bc before the first step, as there): oracle (exact) and
+    the emulated library from the same initial state."""
+    cfg = S.step_config(total, prec, mach=mach)
+    u = S.init_step(cfg)
+    o = O.Oracle(ocfg(cfg))
+    o.set_state(u)
+    rc, to, no = o.run(0.0, 2.3, max_steps=nsteps)
+    s = S.Solver(cfg)
+    s.upload(u)
+    try:
+        ts, ns = s.advance(0.0, 2.3, max_steps=nsteps)
+        rce = 0
+    except S.SwfError as e:
+        ts, ns, rce = None, None, e.code
+    a, b = s.download(), o.state()
+    so, se = o.stats(), s.stats()
+    return {"rc": [rc, rce], "steps": [no, ns], "time": [to, ts],
+            "differ": int((a.view(np.int64) != b.view(np.int64)).sum()),
+            "l1": float(np.abs(a[3:-3, 3:-3, 0] - b[3:-3, 3:-3, 0]).sum() / np.abs(b[3:-3, 3:-3, 0]).sum()),
+            "stats": {k: [so[k], se[k]] for k in STATS}}
+
+
 def line_run(prec, n, cfl, swp, shipped, nsteps):
     """The 1D program: oracle (exact) and the emulated library from the same state."""
     cfg = S.sedov1d_config(n, prec, cfl=cfl, swp_num=swp)
@@ -185,6 +208,9 @@ def main(name):
         out = batch_vs_single(name[len("batch_"):], 25)
     elif name.startswith("first_dt_"):
         out = first_dt(S.jet_config(40, 20, name[len("first_dt_"):], cfl=0.5, half=False))
+    elif name.startswith("step_"):
+        prec, total, nsteps = name[len("step_"):].rsplit("_", 2)
+        out = step_run(prec, int(total), int(nsteps))
     elif name.startswith("line_stress_"):
         out = line_run(name[len("line_stress_"):], 800, 1.5, 50, True, 40)
     elif name == "line_cfg1":
