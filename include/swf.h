@@ -135,7 +135,9 @@ int swf_stats(swf_ctx *ctx, swf_sweep_stats *out);
 /* per-trigger (q, negative pairs, positive neighbours); returns sweep_count */
 int64_t swf_track(swf_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap);
 
-/* weno_right / weno_left of n host stencils a(-2..2) on the device */
+/* weno_right / weno_left of n host stencils a(-2..2) on the device; precision may
+ * carry SWF_WENO_EPS12: the weights of ex6_shock_diffraction/weno5.f90 (1d-12) */
+#define SWF_WENO_EPS12 0x10
 int swf_weno_points(const double *a, double *right, double *left, int64_t n, int32_t precision,
                     int32_t device);
 

@@ -222,12 +222,14 @@ def time_continue(t: float, tfinal: float) -> bool:
     return bool(lib().swf_time_continue(t, tfinal))
 
 
-def weno_points(a: np.ndarray, precision="fp64_exact", device: int = 0):
-    """weno_right / weno_left of n stencils a(-2..2) computed on the device."""
+def weno_points(a: np.ndarray, precision="fp64_exact", device: int = 0, eps12: bool = False):
+    """weno_right / weno_left of n stencils a(-2..2) computed on the device; eps12: the
+    weights of the step program's weno5.f90 (1d-12 in place of 1d-6)."""
     a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 5)
     r = np.empty(a.shape[0])
     l = np.empty(a.shape[0])
-    _check(lib().swf_weno_points(_dp(a), _dp(r), _dp(l), a.shape[0], _prec(precision), device), "swf_weno_points")
+    flags = _prec(precision) | (0x10 if eps12 else 0)
+    _check(lib().swf_weno_points(_dp(a), _dp(r), _dp(l), a.shape[0], flags, device), "swf_weno_points")
     return r, l
 
 

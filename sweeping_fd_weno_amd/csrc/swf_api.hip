@@ -561,6 +561,8 @@ int swf_sync(swf_ctx *c) {
 }
 
 int swf_weno_points(const double *a, double *right, double *left, int64_t n, int32_t precision, int32_t device) {
+  const int e12 = (precision & SWF_WENO_EPS12) != 0;
+  precision &= ~SWF_WENO_EPS12;
   if (!a || !right || !left || n < 0 || precision < 0 || precision > 2) return SWF_E_INVALID;
   if (n == 0) return SWF_OK;
   int ndev = 0;
@@ -570,7 +572,7 @@ int swf_weno_points(const double *a, double *right, double *left, int64_t n, int
   HIPC(hipMalloc(&d, sizeof(double) * 7 * (size_t)n));
   hipError_t e = hipMemcpy(d, a, sizeof(double) * 5 * (size_t)n, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    launch_weno_points(precision, d, d + 5 * n, d + 6 * n, n, 0);
+    launch_weno_points(precision, d, d + 5 * n, d + 6 * n, n, e12, 0);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(right, d + 5 * n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);

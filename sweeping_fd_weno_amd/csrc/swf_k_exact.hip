@@ -52,8 +52,8 @@ void launch_ystage_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, in
 void launch_zs_edges_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
   hipLaunchKernelGGL(k_zs_edges<MODE_EXACT>, dim3(grid_zs_edges(k)), dim3(256), 0, s, k, b, slot_in, parity);
 }
-void launch_weno_points_m0(const double *a, double *r, double *l, long long n, hipStream_t s) {
-  hipLaunchKernelGGL(k_weno_points<MODE_EXACT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, r, l, n);
+void launch_weno_points_m0(const double *a, double *r, double *l, long long n, int e12, hipStream_t s) {
+  hipLaunchKernelGGL(k_weno_points<MODE_EXACT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, r, l, n, e12);
 }
 
 void launch_xflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
@@ -108,11 +108,11 @@ void launch_zs_edges(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int 
   else
     launch_zs_edges_m12(mode, k, b, s, slot_in, parity);
 }
-void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, hipStream_t s) {
+void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, int e12, hipStream_t s) {
   if (mode == MODE_EXACT)
-    launch_weno_points_m0(a, r, l, n, s);
+    launch_weno_points_m0(a, r, l, n, e12, s);
   else
-    launch_weno_points_m12(mode, a, r, l, n, s);
+    launch_weno_points_m12(mode, a, r, l, n, e12, s);
 }
 
 // checkpositivity flag and sweep bitmaps of the current interior (no RK update)

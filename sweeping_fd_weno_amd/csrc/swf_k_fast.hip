@@ -79,12 +79,12 @@ void launch_ystage_batch_m12(int mode, const BatchInst *inst, const int2 *map, i
     hipLaunchKernelGGL(k_ystage_batch<MODE_FP32>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
                        flags);
 }
-void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, hipStream_t s) {
+void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, int e12, hipStream_t s) {
   const int blocks = (int)((n + 255) / 256);
   if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_weno_points<MODE_FP64>, dim3(blocks), dim3(256), 0, s, a, r, l, n);
+    hipLaunchKernelGGL(k_weno_points<MODE_FP64>, dim3(blocks), dim3(256), 0, s, a, r, l, n, e12);
   else
-    hipLaunchKernelGGL(k_weno_points<MODE_FP32>, dim3(blocks), dim3(256), 0, s, a, r, l, n);
+    hipLaunchKernelGGL(k_weno_points<MODE_FP32>, dim3(blocks), dim3(256), 0, s, a, r, l, n, e12);
 }
 
 }  // namespace swf

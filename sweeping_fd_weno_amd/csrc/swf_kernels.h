@@ -1067,14 +1067,15 @@ __global__ void k_to_fortran(KCfg k, const double *__restrict__ src, double *__r
 // weno_right / weno_left of n 5-point stencils (parity tests of the reconstruction)
 template <int M>
 __global__ void k_weno_points(const double *__restrict__ a, double *__restrict__ right,
-                              double *__restrict__ left, long long n) {
+                              double *__restrict__ left, long long n, int e12) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   double s[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) s[q] = a[5 * t + q];
-  right[t] = weno_right<M>(&s[2], kWeps6);
-  left[t] = weno_left<M>(&s[2], kWeps6);
+  const WenoEps we = e12 ? kWeps12 : kWeps6;
+  right[t] = weno_right<M>(&s[2], we);
+  left[t] = weno_left<M>(&s[2], we);
 }
 
 }  // namespace swf

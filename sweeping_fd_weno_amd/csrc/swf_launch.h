@@ -14,17 +14,17 @@ void launch_xflux(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int sta
 void launch_ystage(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
                    int flags);
 void launch_zs_edges(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity);
-void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, hipStream_t s);
+void launch_weno_points(int mode, const double *a, double *r, double *l, long long n, int e12, hipStream_t s);
 
 void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity);
 void launch_ystage_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags);
 void launch_zs_edges_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity);
-void launch_weno_points_m0(const double *a, double *r, double *l, long long n, hipStream_t s);
+void launch_weno_points_m0(const double *a, double *r, double *l, long long n, int e12, hipStream_t s);
 void launch_xflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity);
 void launch_ystage_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
                        int flags);
 void launch_zs_edges_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity);
-void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, hipStream_t s);
+void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, int e12, hipStream_t s);
 
 // batch kernels: blocks of the x-flux / stage launches from the schedule maps
 void launch_xflux_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,

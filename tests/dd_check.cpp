@@ -52,6 +52,7 @@ static int neq(double a, double b) { return memcmp(&a, &b, 8) != 0; }
 int main(int argc, char **argv) {
   long n = argc > 1 ? atol(argv[1]) : 1000000;
   long kb[4] = {0, 0, 0, 0}, kr[4] = {0, 0, 0, 0}, bad_q128 = 0;
+  long bad_alpha12 = 0;
   long bad_beta = 0, bad_alpha = 0, bad_roe = 0, bad_cm = 0, bad_yr = 0, bad_rk2 = 0, bad_rk3 = 0, bad_t = 0;
   for (long it = 0; it < n; ++it) {
     const int kind = (int)(it % 4);
@@ -69,7 +70,12 @@ int main(int argc, char **argv) {
     const double dlin[3] = {0.1, 0.6, 0.3};
     for (int m = 0; m < 3; ++m) {
       const q e = 1e-6Q + Q(b2[m]);
-      bad_alpha += neq(swf::alpha_q(dlin[m], b2[m]), D(Q(dlin[m]) / (e * e)));
+      bad_alpha += neq(swf::alpha_q(dlin[m], b2[m], swf::qconst::e6_hi, swf::qconst::e6_lo), D(Q(dlin[m]) / (e * e)));
+      // the step program's weno5.f90: 1d-12 (its betas are near 1d-12 for small data)
+      const double bs = b2[m] * 1e-6;
+      const q e12 = 1e-12Q + Q(bs);
+      bad_alpha12 +=
+          neq(swf::alpha_q(dlin[m], bs, swf::qconst::e12_hi, swf::qconst::e12_lo), D(Q(dlin[m]) / (e12 * e12)));
     }
     const double wa = fabs(value(kind, fabs(base) + 1e-300)), wb = fabs(value(kind, fabs(base) + 1e-300));
     bad_roe += neq(swf::roe_weight_q(wa, wb), D(Q(wa) / (Q(wa + wb) + 1e-15Q)));
@@ -83,6 +89,7 @@ int main(int argc, char **argv) {
     const double tf = uni() * 2.0, t = tf - 1e-10 * (uni() * 2.0);
     bad_t += (swf::time_continue_q(t, tf) != (Q(t) <= Q(tf) - 1e-10Q));
   }
+  printf("alpha12 %ld %ld\n", bad_alpha12, 3 * n);
   printf("beta %ld %ld\nalpha %ld %ld\nroe_weight %ld %ld\ncm %ld %ld\nyrecip %ld %ld\nrk2 %ld %ld\nrk3 %ld %ld\n"
          "time_continue %ld %ld\nq128_path %ld %ld\n",
          bad_beta, 3 * n, bad_alpha, 3 * n, bad_roe, n, bad_cm, n, bad_yr, n, bad_rk2, n, bad_rk3, n, bad_t, n, bad_q128, 4 * n);

@@ -6,6 +6,9 @@ oracle/_ref/, run there, and only its OUTPUTS are stored here:
 
   ref_weno.npz            weno_right / weno_left (ex4_sedov_2d/weno5.f90:86-146) on
                           seeded adversarial 5-point stencils, via oracle/ref_harness.
+  ref_weno6.npz           the same reconstructions from ex6_shock_diffraction/weno5.f90
+                          (1d-12 in the weights) via oracle/_ref/weno_probe6, on those
+                          stencils plus small-amplitude ones where beta ~ 1d-12.
   ref_ex3_vortex180.json  ex3_vortex_accuracy_2d as shipped (180^2, T = 0.01, sweep on):
                           SHA-256 of the final rho/m1/m2/p fields, a strided sample,
                           the printed L1/Linf errors and the sweep statistics.
@@ -17,7 +20,7 @@ oracle/_ref/, run there, and only its OUTPUTS are stored here:
                           swp_num 2) and a 40-step cfl 1.5 run stopped by the program's
                           max_iter branch: SHA-256 of den/pre/vel, samples, statistics.
 
-Usage:  python tests/golden/make_golden.py [weno] [ex3] [ex3_l1] [ex2] [ex5]
+Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5]
 """
 from __future__ import annotations
 
@@ -59,6 +62,20 @@ def make_weno() -> None:
     subprocess.run([os.path.join(REF, "weno_probe"), fin, fout], check=True)
     r = np.fromfile(fout).reshape(-1, 2)
     np.savez_compressed(os.path.join(GOLD, "ref_weno.npz"), a=a, right=r[:, 0], left=r[:, 1])
+
+
+def make_weno6() -> None:
+    rng = np.random.default_rng(20261204)
+    small = [rng.standard_normal((1000, 5)) * 10.0 ** rng.integers(-9, -3, (1000, 1))]
+    x = np.arange(-2, 3)
+    c = rng.standard_normal((1000, 3)) * 10.0 ** rng.integers(-8, -4, (1000, 1))
+    small.append(c[:, :1] + c[:, 1:2] * x + c[:, 2:3] * x ** 2)
+    a = np.ascontiguousarray(np.concatenate([weno_stencils()] + small))
+    fin, fout = "/tmp/swf_wp6_in.bin", "/tmp/swf_wp6_out.bin"
+    a.tofile(fin)
+    subprocess.run([os.path.join(REF, "weno_probe6"), fin, fout], check=True)
+    r = np.fromfile(fout).reshape(-1, 2)
+    np.savez_compressed(os.path.join(GOLD, "ref_weno6.npz"), a=a, right=r[:, 0], left=r[:, 1])
 
 
 def _tokens(fn: str, ncol: int) -> np.ndarray:
@@ -222,10 +239,12 @@ def make_ex5() -> None:
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["weno", "ex3", "ex3_l1", "ex2", "ex5"]
+    what = sys.argv[1:] or ["weno", "weno6", "ex3", "ex3_l1", "ex2", "ex5"]
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     if "weno" in what:
         make_weno()
+    if "weno6" in what:
+        make_weno6()
     if "ex3" in what:
         make_ex3()
     if "ex3_l1" in what:

@@ -19,6 +19,8 @@ arithmetic and control flow, not races; the GPU tests check the real device.
 - A batch of three instances against each run alone: bitwise.
 - After upload of the ambient jet and swf_bc, the first dt sees the Mach 2000
   inflow ghosts (alpha ~ 800) in every precision mode.
+- The step diffraction program, 52 x 44 cells for 40 steps from its initial data
+  (50 sweeps, 15 density sweeps, 13 negative-neighbour adjustments): bitwise.
 - The 1D program (one workgroup, whole time loop): 60 cells for 12 steps and 800
   cells for 40 steps at cfl 1.5 (16 sweeps, 5273 limited pairs) and 30 steps of the
   1D benchmark case: the whole state bitwise, statistics and tracks equal.
@@ -110,3 +112,12 @@ def test_emulated_line_matches_oracle(emu_lib, name):
         assert o == e, key
     if name == "line_stress_fp64_exact":
         assert r["stats"]["sweep_count"][0] == 16 and r["stats"]["total_pos"][0] == 5273
+
+
+def test_emulated_step_program_matches_oracle(emu_lib):
+    r = scenario("step_fp64_exact_4_40")
+    assert r["rc"] == [0, 0] and r["steps"] == [40, 40] and r["time"][0] == r["time"][1]
+    assert r["differ"] == 0
+    for key, (o, e) in r["stats"].items():
+        assert o == e, key
+    assert r["stats"]["den_count"][0] > 0 and r["stats"]["total_neg"][0] > 0

@@ -4,6 +4,8 @@ The program needs MPI, which this image cannot build, so its runs are not availa
 as fixtures: the oracle restates it (parity pinned by reading, by its weno5.f90
 built unmodified, and by the invariants of tests/test_oracle_step.py), and these
 tests hold the device to the oracle.
+- The exact mode's reconstructions with the program's weno5.f90 weights (1d-12)
+  bitwise against that module built unmodified (tests/golden/ref_weno6.npz).
 - FP64-exact against the oracle, 104 x 88 cells (total = 8), 40 steps from the
   program's initial data with the sweep triggered at most stages (density sweeps and
   negative-neighbour adjustments among them): the whole state bitwise, every
@@ -16,6 +18,8 @@ tests hold the device to the oracle.
   FAIL or the pressure-sum check), positive density and pressure at the end.
 - swf_create rejects inconsistent step configurations.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -27,6 +31,16 @@ pytestmark = pytest.mark.gpu
 
 STATS = ("sweep_count", "den_count", "pre_count", "total_sweep", "max_sweep", "total_neg", "max_neg", "total_pos",
          "max_pos")
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_weno_eps12_exact_matches_reference_module_bitwise():
+    g = np.load(os.path.join(GOLD, "ref_weno6.npz"))
+    r, l = S.weno_points(g["a"], "fp64_exact", eps12=True)
+    assert np.array_equal(r.view(np.int64), g["right"].view(np.int64))
+    assert np.array_equal(l.view(np.int64), g["left"].view(np.int64))
 
 
 def _ocfg(cfg):

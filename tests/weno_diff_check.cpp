@@ -121,8 +121,9 @@ int main(int argc, char **argv) {
       for (int side = 0; side < 2; ++side) {
         const bool right = side == 0;
         const q ref = weno_ref(a, right) - (q)a[0];
-        const double r64 = right ? swf::weno_d64<true>(e) : swf::weno_d64<false>(e);
-        const double r32 = right ? swf::weno_d32_or_64<true>(e) : swf::weno_d32_or_64<false>(e);
+        const double r64 = right ? swf::weno_d64<true>(e, 4e-6) : swf::weno_d64<false>(e, 4e-6);
+        const double r32 =
+            right ? swf::weno_d32_or_64<true>(e, 4e-6, 4e-6f) : swf::weno_d32_or_64<false>(e, 4e-6, 4e-6f);
         const double d64 = (double)fabsq((q)r64 - ref) / scale;
         const double d32 = (double)fabsq((q)r32 - ref) / scale;
         m64[side] = fmax(m64[side], d64);
