@@ -594,6 +594,10 @@ struct swf_batch {
   BatchInst *d_inst = nullptr;
   int2 *d_xmap = nullptr, *d_ymap = nullptr;  // block -> (instance, tile)
   int nxb = 0, nyb = 0;
+  // the same for the step-program instances (their own wenofd variant), and their indices
+  int2 *d_xmap_s = nullptr, *d_ymap_s = nullptr;
+  int nxb_s = 0, nyb_s = 0;
+  std::vector<int> steps;
   double *d_time = nullptr, *d_tfinal = nullptr;
   BatchStatus *d_status = nullptr;
   hipStream_t stream = nullptr;
@@ -606,12 +610,13 @@ namespace {
 // The order of the tiles of all instances in a batch launch.  0: instance by
 // instance; 1: round robin over the instances (tile t of every instance before
 // tile t+1); 2: round robin with the largest instances first.
-void batch_map(const swf_batch *bt, bool ytiles, int schedule, std::vector<int2> &map) {
+// step: the map of the step-program instances, else of the others.
+void batch_map(const swf_batch *bt, bool ytiles, int schedule, bool step, std::vector<int2> &map) {
   map.clear();
   std::vector<int> cnt(bt->n), order(bt->n);
   for (int i = 0; i < bt->n; ++i) {
     const KCfg &k = bt->ctx[i]->k;
-    cnt[i] = ytiles ? ub_gx(k) * ub_gy(k) : xf_gx(k) * xf_gy(k);
+    cnt[i] = ((k.alpha_lines != 0) != step) ? 0 : ytiles ? ub_gx(k) * ub_gy(k) : xf_gx(k) * xf_gy(k);
     order[i] = i;
   }
   if (schedule == 2)
@@ -630,19 +635,30 @@ void batch_map(const swf_batch *bt, bool ytiles, int schedule, std::vector<int2>
     }
 }
 
+int upload_map(const std::vector<int2> &m, int2 **d, int *n) {
+  if (*d) (void)hipFree(*d);
+  *d = nullptr;
+  *n = (int)m.size();
+  if (m.empty()) return SWF_OK;
+  HIPC(hipMalloc(d, sizeof(int2) * m.size()));
+  HIPC(hipMemcpy(*d, m.data(), sizeof(int2) * m.size(), hipMemcpyHostToDevice));
+  return SWF_OK;
+}
+
 int batch_set_maps(swf_batch *bt, int schedule) {
-  std::vector<int2> xm, ym;
-  batch_map(bt, false, schedule, xm);
-  batch_map(bt, true, schedule, ym);
-  if (bt->d_xmap) (void)hipFree(bt->d_xmap);
-  if (bt->d_ymap) (void)hipFree(bt->d_ymap);
-  bt->d_xmap = bt->d_ymap = nullptr;
-  HIPC(hipMalloc(&bt->d_xmap, sizeof(int2) * xm.size()));
-  HIPC(hipMalloc(&bt->d_ymap, sizeof(int2) * ym.size()));
-  HIPC(hipMemcpy(bt->d_xmap, xm.data(), sizeof(int2) * xm.size(), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(bt->d_ymap, ym.data(), sizeof(int2) * ym.size(), hipMemcpyHostToDevice));
-  bt->nxb = (int)xm.size();
-  bt->nyb = (int)ym.size();
+  std::vector<int2> xm, ym, xs, ys;
+  batch_map(bt, false, schedule, false, xm);
+  batch_map(bt, true, schedule, false, ym);
+  batch_map(bt, false, schedule, true, xs);
+  batch_map(bt, true, schedule, true, ys);
+  int rc = upload_map(xm, &bt->d_xmap, &bt->nxb);
+  if (!rc) rc = upload_map(ym, &bt->d_ymap, &bt->nyb);
+  if (!rc) rc = upload_map(xs, &bt->d_xmap_s, &bt->nxb_s);
+  if (!rc) rc = upload_map(ys, &bt->d_ymap_s, &bt->nyb_s);
+  if (rc) return rc;
+  bt->steps.clear();
+  for (int i = 0; i < bt->n; ++i)
+    if (bt->ctx[i]->k.alpha_lines || bt->ctx[i]->k.step_total > 0) bt->steps.push_back(i);
   bt->schedule = schedule;
   if (bt->gexec) {
     (void)hipGraphExecDestroy(bt->gexec);
@@ -651,11 +667,35 @@ int batch_set_maps(swf_batch *bt, int schedule) {
   return SWF_OK;
 }
 
-// one RK stage of every instance: three launches
+// the flux launches of one stage over every instance: the step-program instances
+// (per-line alphas, 1d-12) have kernels of their own
+void batch_flux_launches(swf_batch *bt, bool y, int s, int slot_in, int p, int flags) {
+  if (y) {
+    if (bt->nyb) launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap, bt->nyb, bt->stream, s, slot_in, p, flags, 0);
+    if (bt->nyb_s)
+      launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap_s, bt->nyb_s, bt->stream, s, slot_in, p, flags, 1);
+  } else {
+    if (bt->nxb) launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap, bt->nxb, bt->stream, s, slot_in, p, 0);
+    if (bt->nxb_s) launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap_s, bt->nxb_s, bt->stream, s, slot_in, p, 1);
+  }
+}
+
+// one RK stage of every instance: the flux and stage-end launches over all of
+// them, and for each step-program instance its per-line alphas and wall images
 void batch_enqueue_stage(swf_batch *bt, int s, int p) {
   const int slot_in = s - 1, slot_out = s % 3;
-  launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap, bt->nxb, bt->stream, s, slot_in, p);
-  launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap, bt->nyb, bt->stream, s, slot_in, p, 0);
+  for (int i : bt->steps) {
+    const swf_ctx *c = bt->ctx[i];
+    const Bufs b = stage_bufs(c, s);
+    if (c->k.alpha_lines) launch_line_alpha(c->k, b, bt->stream, slot_in, p);
+    if (c->k.step_total > 0) launch_step_reflect(c->k, b, bt->stream, 0, p);
+  }
+  batch_flux_launches(bt, false, s, slot_in, p, 0);
+  for (int i : bt->steps) {
+    const swf_ctx *c = bt->ctx[i];
+    if (c->k.step_total > 0) launch_step_reflect(c->k, stage_bufs(c, s), bt->stream, 1, p);
+  }
+  batch_flux_launches(bt, true, s, slot_in, p, 0);
   launch_stage_end_batch(bt->d_inst, bt->n, bt->stream, s, slot_out, p, bt->mode == MODE_EXACT ? 1 : 0);
 }
 
@@ -729,6 +769,8 @@ void swf_batch_destroy(swf_batch *bt) {
   (void)hipFree(bt->d_inst);
   (void)hipFree(bt->d_xmap);
   (void)hipFree(bt->d_ymap);
+  if (bt->d_xmap_s) (void)hipFree(bt->d_xmap_s);
+  if (bt->d_ymap_s) (void)hipFree(bt->d_ymap_s);
   (void)hipFree(bt->d_time);
   (void)hipFree(bt->d_status);
   if (bt->stream) (void)hipStreamDestroy(bt->stream);
@@ -815,14 +857,17 @@ int swf_batch_tune(swf_batch *bt, int32_t reps, int32_t *schedule_out, float *ms
   for (int sched = 0; sched < 3; ++sched) {
     int rc = batch_set_maps(bt, sched);
     if (rc) return rc;
+    // the step-program instances' per-line alphas of their current state
+    for (int i : bt->steps)
+      if (bt->ctx[i]->k.alpha_lines) launch_line_alpha(bt->ctx[i]->k, bt->ctx[i]->b, bt->stream, 3, -1);
     for (int w = 0; w < 2; ++w) {
-      launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap, bt->nxb, bt->stream, 0, 0, 0);
-      launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap, bt->nyb, bt->stream, 1, 0, 0, YS_NO_SCALARS);
+      batch_flux_launches(bt, false, 0, 0, 0, 0);
+      batch_flux_launches(bt, true, 1, 0, 0, YS_NO_SCALARS);
     }
     HIPC(hipEventRecord(e0, bt->stream));
     for (int r = 0; r < reps; ++r) {
-      launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap, bt->nxb, bt->stream, 0, 0, 0);
-      launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap, bt->nyb, bt->stream, 1, 0, 0, YS_NO_SCALARS);
+      batch_flux_launches(bt, false, 0, 0, 0, 0);
+      batch_flux_launches(bt, true, 1, 0, 0, YS_NO_SCALARS);
     }
     HIPC(hipEventRecord(e1, bt->stream));
     HIPC(hipEventSynchronize(e1));

@@ -70,27 +70,36 @@ void launch_ystage(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int st
     launch_ystage_m12(mode, k, b, s, stage, slot_in, parity, flags);
 }
 void launch_xflux_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                           int slot_in, int parity) {
-  hipLaunchKernelGGL(k_xflux_batch<MODE_EXACT>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity);
+                           int slot_in, int parity, int step) {
+  if (step)
+    hipLaunchKernelGGL((k_xflux_batch<MODE_EXACT, true>), dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in,
+                       parity);
+  else
+    hipLaunchKernelGGL(k_xflux_batch<MODE_EXACT>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity);
 }
 void launch_ystage_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                            int slot_in, int parity, int flags) {
-  hipLaunchKernelGGL(k_ystage_batch<MODE_EXACT>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
-                     flags);
-}
-void launch_xflux_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                        int slot_in, int parity) {
-  if (mode == MODE_EXACT)
-    launch_xflux_batch_m0(inst, map, nblocks, s, stage, slot_in, parity);
+                            int slot_in, int parity, int flags, int step) {
+  if (step)
+    hipLaunchKernelGGL((k_ystage_batch<MODE_EXACT, true>), dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in,
+                       parity, flags);
   else
-    launch_xflux_batch_m12(mode, inst, map, nblocks, s, stage, slot_in, parity);
+    hipLaunchKernelGGL(k_ystage_batch<MODE_EXACT>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
+                       flags);
+}
+// step: the blocks of step-program instances (their wenofd variant)
+void launch_xflux_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
+                        int slot_in, int parity, int step) {
+  if (mode == MODE_EXACT)
+    launch_xflux_batch_m0(inst, map, nblocks, s, stage, slot_in, parity, step);
+  else
+    launch_xflux_batch_m12(mode, inst, map, nblocks, s, stage, slot_in, parity, step);
 }
 void launch_ystage_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                         int slot_in, int parity, int flags) {
+                         int slot_in, int parity, int flags, int step) {
   if (mode == MODE_EXACT)
-    launch_ystage_batch_m0(inst, map, nblocks, s, stage, slot_in, parity, flags);
+    launch_ystage_batch_m0(inst, map, nblocks, s, stage, slot_in, parity, flags, step);
   else
-    launch_ystage_batch_m12(mode, inst, map, nblocks, s, stage, slot_in, parity, flags);
+    launch_ystage_batch_m12(mode, inst, map, nblocks, s, stage, slot_in, parity, flags, step);
 }
 void launch_stage_end_batch(const BatchInst *inst, int n, hipStream_t s, int stage, int slot_out, int parity,
                             int serial_ubar) {

@@ -63,21 +63,34 @@ void launch_zs_edges_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, 
   else
     hipLaunchKernelGGL(k_zs_edges<MODE_FP32>, dim3(grid_zs_edges(k)), dim3(256), 0, s, k, b, slot_in, parity);
 }
+template <int M, bool STEP>
+static void xflux_batch(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage, int slot_in,
+                        int parity) {
+  hipLaunchKernelGGL((k_xflux_batch<M, STEP>), dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity);
+}
+template <int M, bool STEP>
+static void ystage_batch(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage, int slot_in,
+                         int parity, int flags) {
+  hipLaunchKernelGGL((k_ystage_batch<M, STEP>), dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
+                     flags);
+}
 void launch_xflux_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                            int slot_in, int parity) {
+                            int slot_in, int parity, int step) {
   if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_xflux_batch<MODE_FP64>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity);
+    (step ? xflux_batch<MODE_FP64, true> : xflux_batch<MODE_FP64, false>)(inst, map, nblocks, s, stage, slot_in,
+                                                                          parity);
   else
-    hipLaunchKernelGGL(k_xflux_batch<MODE_FP32>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity);
+    (step ? xflux_batch<MODE_FP32, true> : xflux_batch<MODE_FP32, false>)(inst, map, nblocks, s, stage, slot_in,
+                                                                          parity);
 }
 void launch_ystage_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                             int slot_in, int parity, int flags) {
+                             int slot_in, int parity, int flags, int step) {
   if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_ystage_batch<MODE_FP64>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
-                       flags);
+    (step ? ystage_batch<MODE_FP64, true> : ystage_batch<MODE_FP64, false>)(inst, map, nblocks, s, stage, slot_in,
+                                                                            parity, flags);
   else
-    hipLaunchKernelGGL(k_ystage_batch<MODE_FP32>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
-                       flags);
+    (step ? ystage_batch<MODE_FP32, true> : ystage_batch<MODE_FP32, false>)(inst, map, nblocks, s, stage, slot_in,
+                                                                            parity, flags);
 }
 void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, int e12, hipStream_t s) {
   const int blocks = (int)((n + 255) / 256);

@@ -643,24 +643,26 @@ SWF_D Bufs batch_stage_bufs(const BatchInst &I, int s) {
 }
 
 // map[block] = (instance, tile of that instance)
-template <int M>
+template <int M, bool STEP = false>
 __global__ SWF_FLUX_BOUNDS(M) void k_xflux_batch(const BatchInst *inst, const int2 *map, int stage, int slot_in,
                                                  int parity) {
   __shared__ XSmem sm;
   const int2 e = map[blockIdx.x];
   const BatchInst &I = inst[e.x];
   const int gx = xf_gx(I.k), gy = xf_gy(I.k);
-  xflux_tile<M, false>(I.k, batch_stage_bufs(I, stage), stage, slot_in, parity, e.y % gx, e.y / gx, gx, gy, sm);
+  xflux_tile<M, false, STEP>(I.k, batch_stage_bufs(I, stage), stage, slot_in, parity, e.y % gx, e.y / gx, gx, gy,
+                             sm);
 }
 
-template <int M>
+template <int M, bool STEP = false>
 __global__ SWF_FLUX_BOUNDS(M) void k_ystage_batch(const BatchInst *inst, const int2 *map, int stage, int slot_in,
                                                   int parity, int flags) {
   __shared__ YSmem sm;
   const int2 e = map[blockIdx.x];
   const BatchInst &I = inst[e.x];
   const int gx = ub_gx(I.k);
-  ystage_tile<M, false>(I.k, batch_stage_bufs(I, stage), stage, slot_in, parity, flags, e.y % gx, e.y / gx, gx, sm);
+  ystage_tile<M, false, STEP>(I.k, batch_stage_bufs(I, stage), stage, slot_in, parity, flags, e.y % gx, e.y / gx,
+                              gx, sm);
 }
 
 // ---------------------------------------------------------------------------

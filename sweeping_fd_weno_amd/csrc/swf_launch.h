@@ -28,17 +28,17 @@ void launch_weno_points_m12(int mode, const double *a, double *r, double *l, lon
 
 // batch kernels: blocks of the x-flux / stage launches from the schedule maps
 void launch_xflux_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                        int slot_in, int parity);
+                        int slot_in, int parity, int step = 0);
 void launch_ystage_batch(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                         int slot_in, int parity, int flags);
+                         int slot_in, int parity, int flags, int step = 0);
 void launch_xflux_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                           int slot_in, int parity);
+                           int slot_in, int parity, int step = 0);
 void launch_ystage_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                            int slot_in, int parity, int flags);
+                            int slot_in, int parity, int flags, int step = 0);
 void launch_xflux_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                            int slot_in, int parity);
+                            int slot_in, int parity, int step = 0);
 void launch_ystage_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
-                             int slot_in, int parity, int flags);
+                             int slot_in, int parity, int flags, int step = 0);
 void launch_stage_end_batch(const BatchInst *inst, int n, hipStream_t s, int stage, int slot_out, int parity,
                             int serial_ubar);
 void launch_begin_run_batch(const BatchInst *inst, int n, const double *time, const double *tfinal, hipStream_t s);

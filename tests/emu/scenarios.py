@@ -116,21 +116,26 @@ def crafted_sweep(n, nbad, seed, chain=0):
 
 
 def batch_vs_single(prec, nsteps):
-    """A batch of a Sedov, a vortex and a full jet instance against each run alone
-    (same precision): every field bitwise, the same time and steps."""
-    cfgs = [S.sedov_config(40, prec), S.vortex_config(36, prec), S.jet_config(48, 24, prec, cfl=0.5, half=False)]
-    inits = [S.init_sedov(cfgs[0]), S.init_vortex(cfgs[1]), S.init_jet(cfgs[2])]
-    tf = np.array([1.0, 0.01, 0.001])
+    """A batch of a Sedov, a vortex, a full jet and a step diffraction instance against
+    each run alone (same precision): every field bitwise, the same time and steps.  The
+    step instance starts from its initial data without bc, as its program does."""
+    cfgs = [S.sedov_config(40, prec), S.vortex_config(36, prec), S.jet_config(48, 24, prec, cfl=0.5, half=False),
+            S.step_config(4, prec, mach=4.0)]
+    inits = [S.init_sedov(cfgs[0]), S.init_vortex(cfgs[1]), S.init_jet(cfgs[2]), S.init_step(cfgs[3])]
+    tf = np.array([1.0, 0.01, 0.001, 2.3])
+    n = len(cfgs)
     bt = S.Batch(cfgs)
-    for i in range(3):
+    for i in range(n):
         bt.instance(i).upload(inits[i])
-        bt.instance(i).bc()
-    tb, nb, stb = bt.advance(np.zeros(3), tf, max_steps=nsteps)
+        if i < 3:
+            bt.instance(i).bc()
+    tb, nb, stb = bt.advance(np.zeros(n), tf, max_steps=nsteps)
     out = {"rc": int(stb.max()), "steps": nb.tolist(), "differ": 0, "time_equal": True}
-    for i in range(3):
+    for i in range(n):
         s = S.Solver(cfgs[i])
         s.upload(inits[i])
-        s.bc()
+        if i < 3:
+            s.bc()
         ts, ns = s.advance(0.0, tf[i], max_steps=nsteps)
         a, b = bt.instance(i).download(), s.download()
         out["differ"] += int((a.view(np.int64) != b.view(np.int64)).sum())

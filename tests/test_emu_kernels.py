@@ -84,11 +84,11 @@ def test_emulated_fast_modes_jet_sweeps(emu_lib, prec, tol):
 
 @pytest.mark.parametrize("prec", ["fp64_exact", "fp32_bulk"])
 def test_emulated_batch_matches_single_runs(emu_lib, prec):
-    """A batch of Sedov 40^2, vortex 36^2 and full jet 48x24 (the vortex ends after
-    one step while the others go on): every instance bitwise equal to its run alone,
-    with the same steps, time and sweeps."""
+    """A batch of Sedov 40^2, vortex 36^2, full jet 48x24 and a Mach 4 step
+    diffraction 52x44 (the vortex ends after one step while the others go on): every
+    instance bitwise equal to its run alone, with the same steps, time and sweeps."""
     r = scenario("batch_" + prec)
-    assert r["rc"] == 0 and r["steps"] == [25, 1, 25]
+    assert r["rc"] == 0 and r["steps"] == [25, 1, 25, 25]
     assert r["differ"] == 0 and r["time_equal"]
     for o, e in r["sweeps"]:
         assert o == e
