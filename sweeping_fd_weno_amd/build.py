@@ -66,7 +66,8 @@ def build_library(verbose: bool = False) -> str:
 
 def build_driver() -> list:
     """The Fortran drivers (reference parameter surfaces) linked against libswf.so:
-    swf_driver (ex4 Sedov), swf_vortex (ex3 vortex accuracy) and swf_sedov1d (ex2 1D Sedov)."""
+    swf_driver (ex4 Sedov), swf_vortex (ex3 vortex accuracy), swf_sedov1d (ex2 1D Sedov) and swf_jet
+    (ex5 Mach 2000 jet)."""
     drv_dir = os.path.join(PKG, "fortran")
     mod_src = os.path.join(drv_dir, "swf_mod.f90")
     if not os.path.exists(FLANG) or not os.path.exists(mod_src):
@@ -74,7 +75,7 @@ def build_driver() -> list:
     mod = os.path.join(OBJ, "fmod")
     os.makedirs(mod, exist_ok=True)
     built = []
-    for prog in ("swf_driver", "swf_vortex", "swf_sedov1d"):
+    for prog in ("swf_driver", "swf_vortex", "swf_sedov1d", "swf_jet"):
         src = os.path.join(drv_dir, prog + ".f90")
         if not os.path.exists(src):
             continue

@@ -12,6 +12,10 @@ oracle/_ref/, run there, and only its OUTPUTS are stored here:
   ref_ex3_vortex180.json  ex3_vortex_accuracy_2d as shipped (180^2, T = 0.01, sweep on):
                           SHA-256 of the final rho/m1/m2/p fields, a strided sample,
                           the printed L1/Linf errors and the sweep statistics.
+  ref_ex5_jet160x40.json  ex5_mach2000_jet with its grid PARAMETERs set to 160 x 40, to
+                          T = 0.001 (612 steps, 1 687 sweeps) and to T = 1.5e-4: SHA-256
+                          of den.dat / pre.dat / m1.dat and of the printed text, the log
+                          fields of the unmirrored half, the statistics.
   ref_ex5_jet.json        ex5_mach2000_jet as shipped (800x200, T = 0.001), when its
                           (hour-long) run has finished: log-field hashes + statistics.
 
@@ -20,7 +24,7 @@ oracle/_ref/, run there, and only its OUTPUTS are stored here:
                           swp_num 2) and a 40-step cfl 1.5 run stopped by the program's
                           max_iter branch: SHA-256 of den/pre/vel, samples, statistics.
 
-Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5]
+Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5s] [ex5]
 """
 from __future__ import annotations
 
@@ -213,6 +217,34 @@ def make_ex2() -> None:
         json.dump(out, f, indent=1)
 
 
+def make_ex5_small() -> None:
+    """oracle/Makefile targets ex5_small (T = 0.001) and ex5_short (T = 1.5e-4), 160 x 40."""
+    nx, ny = 160, 40
+    out = {"layout": "log fields of the unmirrored half (the file's second half), (j outer, i inner), "
+                     "160 x 40 interior"}
+    for key, build, desc in (("full", "ex5_small", "T = 0.001"), ("short", "ex5_short", "T = 1.5e-4")):
+        rundir = os.path.join(REF, "run_" + build)
+        text = _run(os.path.join(REF, build, "a"), rundir)
+        fields = {}
+        for name, fn, skip in (("log_rho", "den.dat", 2), ("log_p", "pre.dat", 1), ("log_p_over_rho", "m1.dat", 1)):
+            body = open(os.path.join(rundir, fn)).read().split("\n", skip)[skip]
+            t = np.array(body.split(), dtype=np.float64).reshape(-1, 3)
+            fields[name] = t[ny * nx:, 2]
+        steps = [int(l.split()[0]) for l in text.splitlines() if l.strip().endswith("%")]
+        out[key] = {
+            "program": f"ex5_mach2000_jet with nx = 160, ny = 40 (PARAMETER line), {desc}, otherwise as shipped",
+            "sha256": {k: field_digest(v) for k, v in fields.items()},
+            "sample": {k: v[::97].tolist() for k, v in fields.items()},
+            "stats": _parse_stats(text),
+            "last_progress_step": steps[-1] if steps else 0,
+            "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
+                            for fn in ("den.dat", "pre.dat", "m1.dat")},
+            "stdout_sha256": hashlib.sha256(text.encode()).hexdigest(),
+        }
+    with open(os.path.join(GOLD, "ref_ex5_jet160x40.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def make_ex5() -> None:
     rundir = os.path.join(REF, "run_ex5")
     if not os.path.exists(os.path.join(rundir, "DONE")):
@@ -239,7 +271,7 @@ def make_ex5() -> None:
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["weno", "weno6", "ex3", "ex3_l1", "ex2", "ex5"]
+    what = sys.argv[1:] or ["weno", "weno6", "ex3", "ex3_l1", "ex2", "ex5s", "ex5"]
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     if "weno" in what:
         make_weno()
@@ -251,5 +283,7 @@ if __name__ == "__main__":
         make_ex3_l1()
     if "ex2" in what:
         make_ex2()
+    if "ex5s" in what:
+        make_ex5_small()
     if "ex5" in what:
         make_ex5()

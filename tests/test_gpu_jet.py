@@ -1,0 +1,59 @@
+"""The Mach 2000 jet program (ex5_mach2000_jet/main.f90) on the GPU against the
+reference itself, built with its grid PARAMETERs set to 160 x 40 and run to T = 0.001
+(612 steps; the sweep runs at 1 687 stages with 120 653 negative-neighbour pairs):
+tests/golden/ref_ex5_jet160x40.json.
+
+- FP64-exact: the log density / pressure / p-over-rho fields the program writes,
+  bitwise (logarithms by the C library, as the program's), and every printed statistic.
+- fortran/swf_jet, the ex5 program over the C-ABI, writes den.dat / pre.dat / m1.dat
+  byte for byte and prints the same text (progress percentages, statistics).
+"""
+import hashlib
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import sweeping_fd_weno_amd as S
+from test_oracle_golden import jet_log_fields
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden", "ref_ex5_jet160x40.json")
+DRIVER = os.path.join(ROOT, "sweeping_fd_weno_amd", "fortran", "swf_jet")
+
+
+def _digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def test_jet160x40_exact_matches_reference_run():
+    g = json.load(open(GOLD))["full"]
+    cfg = S.jet_config(160, 40, "fp64_exact")
+    with S.Solver(cfg) as s:
+        s.upload(S.init_jet(cfg))
+        s.bc()
+        t, k = s.advance(0.0, 0.001)
+        u = s.download()
+        st = s.stats()
+    print("steps", k, "sweeps", st["sweep_count"], flush=True)
+    for name, v in jet_log_fields(u, 160, 40).items():
+        assert _digest(v) == g["sha256"][name], name
+    assert st["total_sweep"] == g["stats"]["total sweep"] and st["max_sweep"] == g["stats"]["max sweep"]
+    assert st["total_neg"] == g["stats"]["total negative pairs"] and st["max_neg"] == g["stats"]["max negative pairs"]
+    assert st["total_pos"] == g["stats"]["total positive neighbors"]
+    assert st["max_pos"] == g["stats"]["max positive neighbors"]
+    assert st["den_count"] == g["stats"]["den_count"] and st["pre_count"] == g["stats"]["pre_count"]
+    assert k // 20 * 20 == g["last_progress_step"]
+
+
+def test_jet_driver_reproduces_reference_output_files(tmp_path):
+    g = json.load(open(GOLD))["full"]
+    res = subprocess.run([DRIVER, "160", "40"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    for fn, digest in g["file_sha256"].items():
+        assert hashlib.sha256((tmp_path / fn).read_bytes()).hexdigest() == digest, fn
+    assert hashlib.sha256(res.stdout.encode()).hexdigest() == g["stdout_sha256"]

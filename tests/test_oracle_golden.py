@@ -8,6 +8,10 @@
 - ref_ex3_l1_vortex180.json: the same program built with its switch use_limiter = 1
   (the Zhang-Shu flux limiter in wenofd instead of the sweeping post-process) ->
   final fields bitwise, the printed limiter counts and error norms exactly.
+- ref_ex5_jet160x40.json: ex5_mach2000_jet on a 160 x 40 grid to T = 1.5e-4 (90 steps,
+  124 sweeps, 401 negative-neighbour pairs) -> the log density / pressure / p-over-rho
+  fields it writes bitwise (logarithms by the C library, as the program's), the printed
+  statistics exactly.
 - ref_ex2_sedov1d.json: ex2_sedov_1d (the 1D program, 800 points) as shipped, at the
   1D benchmark settings (cfl 0.6, ambient E 1e-12, swp_num 2) and as a 40-step cfl 1.5
   run with 16 sweeps -> den/pre/vel bitwise, the sweep statistics exactly, the step
@@ -162,3 +166,38 @@ def test_flang_dot_product_sums_left_to_right():
     assert exact != s  # the data distinguish the orders
     want = "%016X" % np.float64(s).view(np.uint64)
     assert dot_hex == want and loop_hex == want
+
+
+def _libm_log(a):
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.log.restype = ctypes.c_double
+    libm.log.argtypes = [ctypes.c_double]
+    return np.array([libm.log(float(v)) for v in np.ravel(a)])
+
+
+def jet_log_fields(u, nx, ny, gamma=5.0 / 3.0):
+    """The unmirrored half of ex5's den.dat / pre.dat / m1.dat (ex5_mach2000_jet/main.f90:834-847)
+    from a state u (ny+6, nx+6, 4): (j outer, i inner) log fields."""
+    v = u[3:3 + ny, 3:3 + nx]
+    r, m, n, e = v[:, :, 0], v[:, :, 1], v[:, :, 2], v[:, :, 3]
+    p = (gamma - 1.0) * ((e - (0.5 * (m * m)) / r) - (0.5 * (n * n)) / r)
+    return {"log_rho": _libm_log(np.abs(r)), "log_p": _libm_log(np.abs(p)), "log_p_over_rho": _libm_log(np.abs(p / r))}
+
+
+@pytest.mark.slow
+def test_ex5_jet_160x40_short_run_matches_reference_bitwise():
+    g = json.load(open(os.path.join(GOLD, "ref_ex5_jet160x40.json")))["short"]
+    cfg = O.jet_config(160, 40)
+    o = O.Oracle(cfg)
+    o.set_state(O.init_jet(cfg))
+    o.bc()
+    rc, t, k = o.run(0.0, 0.00015)
+    assert rc == 0
+    for name, v in jet_log_fields(o.state(), 160, 40).items():
+        assert _digest(v) == g["sha256"][name], name
+    st = o.stats()
+    assert st["total_sweep"] == g["stats"]["total sweep"] and st["total_neg"] == g["stats"]["total negative pairs"]
+    assert st["total_pos"] == g["stats"]["total positive neighbors"] and st["pre_count"] == g["stats"]["pre_count"]
+    assert st["max_sweep"] == g["stats"]["max sweep"] and st["max_neg"] == g["stats"]["max negative pairs"]
+    assert k // 20 * 20 == g["last_progress_step"]
