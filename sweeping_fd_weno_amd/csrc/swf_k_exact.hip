@@ -197,8 +197,9 @@ void launch_bc(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out, int pa
   hipLaunchKernelGGL(k_bc, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, k, b, slot_out, parity);
 }
 void launch_line_alpha(const KCfg &k, const Bufs &b, hipStream_t s, int slot, int parity) {
-  hipLaunchKernelGGL(k_line_alpha, dim3(la_gx(k), la_gy(k)), dim3(256), 0, s, k, b, parity);
-  hipLaunchKernelGGL(k_line_alpha_reduce, dim3(1), dim3(1024), 0, s, k, b, slot, parity);
+  hipLaunchKernelGGL(k_line_alpha, dim3(la_gx(k), la_gy(k)), dim3(256), 0, s, k, b, slot, parity);
+  const int nthreads = (k.nx + 6 + k.ny + 6) * LA_LANES;
+  hipLaunchKernelGGL(k_line_alpha_reduce, dim3((nthreads + 255) / 256), dim3(256), 0, s, k, b, slot, parity);
 }
 void launch_step_reflect(const KCfg &k, const Bufs &b, hipStream_t s, int dir, int parity) {
   const int n = dir == 0 ? 3 * (std::min(k.ny + 1, 6 * k.step_total) + 1) : 3 * k.step_total;

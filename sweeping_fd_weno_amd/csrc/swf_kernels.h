@@ -946,10 +946,11 @@ __global__ __launch_bounds__(256) void k_bc(KCfg k, Bufs b, int slot_out, int pa
 // alpha(i,j) of the stage input over every cell, the max of each row and each
 // column of a 64 x 8 tile into lpart (rows first: la_gx x (ny+6), then columns:
 // la_gy x (nx+6)), then k_line_alpha_reduce.
-__global__ __launch_bounds__(256) void k_line_alpha(KCfg k, Bufs b, int parity) {
+__global__ __launch_bounds__(256) void k_line_alpha(KCfg k, Bufs b, int slot, int parity) {
   __shared__ double cred[4][LA_W];
-  const DevScalars *g = b.g;
+  DevScalars *g = b.g;
   if (parity >= 0 && (!g->active[parity] || g->status != ST_OK)) return;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) g->alpha_bits[slot] = 0ull;  // +0.0
   const int tx = threadIdx.x & (LA_W - 1), ty = threadIdx.x / LA_W;
   const int i = -2 + blockIdx.x * LA_W + tx;
   double colmax = 0.0;
@@ -976,36 +977,45 @@ __global__ __launch_bounds__(256) void k_line_alpha(KCfg k, Bufs b, int parity) 
   }
 }
 
-// arow, acol and alphamax (into alpha_bits[slot]) from the tile partials; one workgroup
-__global__ __launch_bounds__(1024) void k_line_alpha_reduce(KCfg k, Bufs b, int slot, int parity) {
-  __shared__ double red[16];
+// arow, acol and alphamax from the tile partials: 8 lanes per line (rows first,
+// then columns), each block's max into alpha_bits[slot] (cleared by k_line_alpha)
+constexpr int LA_LANES = 8;
+__global__ __launch_bounds__(256) void k_line_alpha_reduce(KCfg k, Bufs b, int slot, int parity) {
+  __shared__ double red[4];
   DevScalars *g = b.g;
   if (parity >= 0 && (!g->active[parity] || g->status != ST_OK)) return;
   const int nyg = k.ny + 6, nxg = k.nx + 6, gxa = la_gx(k), gya = la_gy(k);
-  double gm = 0.0;
-  for (int t = threadIdx.x; t < nyg + nxg; t += 1024) {
-    double m = 0.0;
-    if (t < nyg) {
-      for (int q = 0; q < gxa; ++q) m = fmax2(m, b.lpart[(long long)q * nyg + t]);
-      b.arow[t] = m;
-    } else {
-      const int c = t - nyg;
-      for (int q = 0; q < gya; ++q) m = fmax2(m, b.lpart[(long long)gxa * nyg + (long long)q * nxg + c]);
-      b.acol[c] = m;
-    }
-    gm = fmax2(gm, m);
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int line = t / LA_LANES, r = t % LA_LANES;
+  double m = 0.0;
+  if (line < nyg) {
+    for (int q = r; q < gxa; q += LA_LANES) m = fmax2(m, b.lpart[(long long)q * nyg + line]);
+  } else if (line < nyg + nxg) {
+    const int c = line - nyg;
+    const double *cp = b.lpart + (long long)gxa * nyg + c;
+    for (int q = r; q < gya; q += LA_LANES) m = fmax2(m, cp[(long long)q * nxg]);
   }
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const double o = __shfl_xor(gm, off, 64);
-    gm = (o > gm) ? o : gm;
+  for (int off = LA_LANES / 2; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(m, off, 64);
+    m = (o > m) ? o : m;
   }
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = gm;
+  if (r == 0) {
+    if (line < nyg)
+      b.arow[line] = m;
+    else if (line < nyg + nxg)
+      b.acol[line - nyg] = m;
+  }
+#pragma unroll
+  for (int off = 32; off >= LA_LANES; off >>= 1) {
+    const double o = __shfl_xor(m, off, 64);
+    m = (o > m) ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double m = 0.0;
-    for (int w = 0; w < 16; ++w) m = fmax2(m, red[w]);
-    g->alpha_bits[slot] = (unsigned long long)__double_as_longlong(m);
+    const double bm = fmax2(fmax2(red[0], red[1]), fmax2(red[2], red[3]));
+    atomicMax(&g->alpha_bits[slot], (unsigned long long)__double_as_longlong(bm));  // alpha >= 0: bits order
   }
 }
 
