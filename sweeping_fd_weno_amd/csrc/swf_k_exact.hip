@@ -40,13 +40,13 @@ void launch_xflux_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int
 }
 void launch_ystage_m0(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags) {
   if (k.use_limiter == 1)
-    hipLaunchKernelGGL((k_ystage<MODE_EXACT, true>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity,
+    hipLaunchKernelGGL((k_ystage<MODE_EXACT, true>), grid_ystage(k), dim3(YS_T), 0, s, k, b, stage, slot_in, parity,
                        flags);
   else if (k.alpha_lines)
-    hipLaunchKernelGGL((k_ystage<MODE_EXACT, false, true>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in,
+    hipLaunchKernelGGL((k_ystage<MODE_EXACT, false, true>), grid_ystage(k), dim3(YS_T), 0, s, k, b, stage, slot_in,
                        parity, flags);
   else
-    hipLaunchKernelGGL((k_ystage<MODE_EXACT, false>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity,
+    hipLaunchKernelGGL((k_ystage<MODE_EXACT, false>), grid_ystage(k), dim3(YS_T), 0, s, k, b, stage, slot_in, parity,
                        flags);
 }
 void launch_zs_edges_m0(const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
@@ -80,10 +80,10 @@ void launch_xflux_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, 
 void launch_ystage_batch_m0(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
                             int slot_in, int parity, int flags, int step) {
   if (step)
-    hipLaunchKernelGGL((k_ystage_batch<MODE_EXACT, true>), dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in,
+    hipLaunchKernelGGL((k_ystage_batch<MODE_EXACT, true>), dim3(nblocks), dim3(YS_T), 0, s, inst, map, stage, slot_in,
                        parity, flags);
   else
-    hipLaunchKernelGGL(k_ystage_batch<MODE_EXACT>, dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
+    hipLaunchKernelGGL(k_ystage_batch<MODE_EXACT>, dim3(nblocks), dim3(YS_T), 0, s, inst, map, stage, slot_in, parity,
                        flags);
 }
 // step: the blocks of step-program instances (their wenofd variant)

@@ -36,10 +36,10 @@ static void xflux(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int sl
 template <int M, bool LIM>
 static void ystage(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity, int flags) {
   if (!LIM && k.alpha_lines)
-    hipLaunchKernelGGL((k_ystage<M, false, true>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity,
+    hipLaunchKernelGGL((k_ystage<M, false, true>), grid_ystage(k), dim3(YS_T), 0, s, k, b, stage, slot_in, parity,
                        flags);
   else
-    hipLaunchKernelGGL((k_ystage<M, LIM>), grid_ystage(k), dim3(256), 0, s, k, b, stage, slot_in, parity, flags);
+    hipLaunchKernelGGL((k_ystage<M, LIM>), grid_ystage(k), dim3(YS_T), 0, s, k, b, stage, slot_in, parity, flags);
 }
 
 void launch_xflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
@@ -71,7 +71,7 @@ static void xflux_batch(const BatchInst *inst, const int2 *map, int nblocks, hip
 template <int M, bool STEP>
 static void ystage_batch(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage, int slot_in,
                          int parity, int flags) {
-  hipLaunchKernelGGL((k_ystage_batch<M, STEP>), dim3(nblocks), dim3(256), 0, s, inst, map, stage, slot_in, parity,
+  hipLaunchKernelGGL((k_ystage_batch<M, STEP>), dim3(nblocks), dim3(YS_T), 0, s, inst, map, stage, slot_in, parity,
                      flags);
 }
 void launch_xflux_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,

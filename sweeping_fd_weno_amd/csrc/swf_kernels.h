@@ -50,9 +50,9 @@ struct YSmem {
   double fn[4][YT_PTS][YT_W];
   double pr[4][YT_PTS][YT_W];
   double G[4][YT_H][YT_W];  // net y-interface fluxes of the tile
-  double red[4];
-  double ured[4][4];        // weight*u sums of the four waves
-  int zred[8];
+  double red[YS_T / 64];
+  double ured[4][YS_T / 64];  // weight*u sums of the waves
+  int zred[2 * YS_T / 64];
 };
 
 // split fluxes and Roe primitives of stencil point u
@@ -72,22 +72,23 @@ SWF_D void stage_point(const KCfg &k, const double u[4], double alpha, double ia
   pr[3] = p.h;
 }
 
-// Stage the points p = tid and p = tid + 256 of a tile (at most two per thread):
+// Stage the points p = tid and p = tid + NT of a tile of NT threads (at most two
+// per thread):
 // both loads are issued before either point is processed, so the tile waits for
 // one memory round trip.  point(p, i, j) maps a point to its cell and returns
 // false outside the grid; store(p, fp, fn, pr) writes the staged values.
 // STEP (the step program, alpha_lines): the split of a point uses the alpha of its
 // line, row j for the x-pass (aline[j+2]) and column i for the y-pass (aline[i+2]).
-template <int M, int DIR, int NPTS, bool STEP, class PointF, class StoreF>
+template <int M, int DIR, int NT, int NPTS, bool STEP, class PointF, class StoreF>
 SWF_D void stage_tile(const KCfg &k, const Bufs &b, double alpha, double ia, const double *aline, int tid,
                       PointF point, StoreF store) {
-  static_assert(NPTS <= 512, "at most two points per thread");
+  static_assert(NPTS <= 2 * NT, "at most two points per thread");
   double u[2][4];
   bool ok[2];
   int pp[2], line[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    pp[s] = tid + 256 * s;
+    pp[s] = tid + NT * s;
     int i = 0, j = 0;
     ok[s] = pp[s] < NPTS && point(pp[s], i, j);
     line[s] = DIR == 0 ? j : i;
@@ -145,9 +146,11 @@ SWF_D int iface_flux(const KCfg &k, const double *pr, const double *fp, const do
 SWF_D int zs_den(int lim) { return (lim & 1) + ((lim >> 2) & 1); }
 SWF_D int zs_pre(int lim) { return ((lim >> 1) & 1) + ((lim >> 3) & 1); }
 
-// sums of two counts over a 256-thread block, stored by thread 0 as one
-// (den, pre) pair of the limiter count partials
+// sums of two counts over a block of NT threads (4 or 8 waves), stored by thread 0
+// as one (den, pre) pair of the limiter count partials
+template <int NT>
 SWF_D void zs_store(int den, int pre, int *pair, int *red, int tid) {
+  static_assert(NT == 256 || NT == 512, "4 or 8 waves");
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     den += __shfl_xor(den, off, 64);
@@ -159,14 +162,22 @@ SWF_D void zs_store(int den, int pre, int *pair, int *red, int tid) {
   }
   __syncthreads();
   if (tid == 0) {
-    pair[0] = (red[0] + red[2]) + (red[4] + red[6]);
-    pair[1] = (red[1] + red[3]) + (red[5] + red[7]);
+    int d = 0, p = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+      d += red[2 * w];
+      p += red[2 * w + 1];
+    }
+    pair[0] = d;
+    pair[1] = p;
   }
 }
 
 // Occupancy of the fast-mode flux kernels: 4 waves per SIMD (at most 128 VGPRs);
 // the exact mode's double-double and binary128 sites need more registers.
+// x-flux workgroups have 256 threads, stage workgroups YS_T.
 #define SWF_FLUX_BOUNDS(M) __launch_bounds__(256, (M) == MODE_EXACT ? 1 : 4)
+#define SWF_STAGE_BOUNDS(M) __launch_bounds__(YS_T, (M) == MODE_EXACT ? 1 : 4)
 
 // wenofd x-pass for interfaces k0..k0+63 of rows j0..j0+3 -> Fx.  In an RK stage
 // (stage = 1, 2, 3; 0 for a stand-alone evaluation) the launch also resets the
@@ -202,7 +213,7 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
   const double ia = 1.0 / alpha;
   const int k0 = bx * XT_W - XT_SHIFT, j0 = 1 + by * XT_H;
   const double *aline = STEP ? b.arow : nullptr;
-  stage_tile<M, 0, XT_H * XT_PTS, STEP>(
+  stage_tile<M, 0, 256, XT_H * XT_PTS, STEP>(
       k, b, alpha, ia, aline, tid,
       [&](int p, int &i, int &j) {
         const int r = p / XT_PTS;
@@ -231,7 +242,7 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
 #pragma unroll
     for (int c = 0; c < 4; ++c) b.Fx[c * k.plane + o] = F[c];
   }
-  if (LIM && stage > 0) zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (bx + by * gx), sm.zred, tid);
+  if (LIM && stage > 0) zs_store<256>(zs_den(lim), zs_pre(lim), b.zs + 2 * (bx + by * gx), sm.zred, tid);
 }
 
 template <int M, bool LIM, bool STEP = false>
@@ -308,6 +319,15 @@ SWF_D void write_ghosts(const KCfg &k, const Bufs &b, int i, int j, const double
   }
 }
 
+// sum of n = 2^m values as a balanced tree: ((r0 + r1) + (r2 + r3)) + ...
+template <int N>
+SWF_D double pair_sum(const double *r) {
+  if constexpr (N == 1)
+    return r[0];
+  else
+    return pair_sum<N / 2>(r) + pair_sum<N / 2>(r + N / 2);
+}
+
 // block-wide max of a non-negative value over NT threads; result valid in thread 0
 template <int NT>
 SWF_D double block_max(double a, double *red, int tid) {
@@ -349,12 +369,14 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
       const int grp = tid >> 8, lt = tid & 255;
       const int blk = blks[grp];
       double a = 0.0;
-      if (blk >= 0 && blk < nblk && lt < UB_W * UB_H) {
-        const int i = 1 + (blk % gx) * UB_W + lt % UB_W, j = 1 + (blk / gx) * UB_H + lt / UB_W;
-        if (i <= k.nx && j <= k.ny) {
-          double v[4];
-          load_ij(k, b, i, j, v);
-          a = cell_alpha(k, v[0], v[1], v[2], v[3]);
+      if (blk >= 0 && blk < nblk) {
+        for (int c = lt; c < UB_W * UB_H; c += 256) {  // a tile's cells, 256 at a time
+          const int i = 1 + (blk % gx) * UB_W + c % UB_W, j = 1 + (blk / gx) * UB_H + c / UB_W;
+          if (i <= k.nx && j <= k.ny) {
+            double v[4];
+            load_ij(k, b, i, j, v);
+            a = fmax2(a, cell_alpha(k, v[0], v[1], v[2], v[3]));
+          }
         }
       }
       // max within each 256-thread group: waves 4*grp .. 4*grp+3
@@ -424,7 +446,7 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
   const int j = jf;
   const bool in = ty >= 1 && i <= k.nx && j <= k.ny;
   const double *aline = STEP ? b.acol : nullptr;
-  stage_tile<M, 1, YT_PTS * YT_W, STEP>(
+  stage_tile<M, 1, YS_T, YT_PTS * YT_W, STEP>(
       k, b, alpha, ia, aline, tid,
       [&](int p, int &i, int &j) {
         const int r = p / YT_W;
@@ -521,7 +543,7 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
     }
   }
   if (rhs_only) return;
-  const double am = block_max<256>(anew, sm.red, tid);
+  const double am = block_max<YS_T>(anew, sm.red, tid);
   if (tid == 0 && scalars) b.apart[bx + by * gx] = am;
   if (usum) {
 #pragma unroll
@@ -536,15 +558,14 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
       for (int c = 0; c < 4; ++c) sm.ured[c][tid >> 6] = wu[c];
     }
     __syncthreads();
-    if (tid < 4)
-      b.upart[4 * (bx + by * gx) + tid] = (sm.ured[tid][0] + sm.ured[tid][1]) + (sm.ured[tid][2] + sm.ured[tid][3]);
+    if (tid < 4) b.upart[4 * (bx + by * gx) + tid] = pair_sum<YS_T / 64>(sm.ured[tid]);
   }
   if (LIM && scalars)
-    zs_store(zs_den(lim), zs_pre(lim), b.zs + 2 * (xf_gx(k) * xf_gy(k) + bx + by * gx), sm.zred, tid);
+    zs_store<YS_T>(zs_den(lim), zs_pre(lim), b.zs + 2 * (xf_gx(k) * xf_gy(k) + bx + by * gx), sm.zred, tid);
 }
 
 template <int M, bool LIM, bool STEP = false>
-__global__ SWF_FLUX_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_in, int parity, int flags) {
+__global__ SWF_STAGE_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_in, int parity, int flags) {
   __shared__ YSmem sm;
   ystage_tile<M, LIM, STEP>(k, b, stage, slot_in, parity, flags, blockIdx.x, blockIdx.y, gridDim.x, sm);
 }
@@ -624,7 +645,7 @@ __global__ __launch_bounds__(256) void k_zs_edges(KCfg k, Bufs b, int slot_in, i
     else
       lim = right ? zs_edge_half<M, 1, true>(k, b, line, a, alpha, ia) : zs_edge_half<M, 1, false>(k, b, line, a, alpha, ia);
   }
-  zs_store(lim & 1, (lim >> 1) & 1, b.zs + 2 * (xf_gx(k) * xf_gy(k) + ub_gx(k) * ub_gy(k) + blockIdx.x), zred, tid);
+  zs_store<256>(lim & 1, (lim >> 1) & 1, b.zs + 2 * (xf_gx(k) * xf_gy(k) + ub_gx(k) * ub_gy(k) + blockIdx.x), zred, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -655,7 +676,7 @@ __global__ SWF_FLUX_BOUNDS(M) void k_xflux_batch(const BatchInst *inst, const in
 }
 
 template <int M, bool STEP = false>
-__global__ SWF_FLUX_BOUNDS(M) void k_ystage_batch(const BatchInst *inst, const int2 *map, int stage, int slot_in,
+__global__ SWF_STAGE_BOUNDS(M) void k_ystage_batch(const BatchInst *inst, const int2 *map, int stage, int slot_in,
                                                   int parity, int flags) {
   __shared__ YSmem sm;
   const int2 e = map[blockIdx.x];
