@@ -242,11 +242,12 @@ SWF_HD void roe_prims(const KCfg &k, const double u[4], double &w, double &vx, d
 
 // Split fluxes and Roe primitives of one stencil point in the fast modes: the
 // formulas of split_point and roe_prims with one reciprocal of rho shared by the
-// velocities and a multiplication by 1/alpha.
+// velocities and a multiplication by 1/alpha.  Reciprocals are the refined
+// hardware estimate (within 1 ulp of the division).
 template <int DIR>
 SWF_HD void stage_point_fast(const KCfg &k, const double u[4], double ia, double fp[4], double fn[4], Prim &pr) {
   const double rho = u[0], mx = u[1], my = u[2], e = u[3];
-  const double r = 1.0 / rho;
+  const double r = rcp64(rho);
   const double vx0 = mx * r, vy0 = my * r;
   const bool vac = below_eps2(k, rho);
   const double vx = vac ? 0.0 : vx0, vy = vac ? 0.0 : vy0;
@@ -271,7 +272,7 @@ SWF_HD void stage_point_fast(const KCfg &k, const double u[4], double ia, double
   }
   double t0 = r, vex = vx0, vey = vy0;
   if (DIR == 1 && !k.ydens_plain) {
-    t0 = 1.0 / (rho + 1e-15);
+    t0 = rcp64(rho + 1e-15);
     vex = mx * t0;
     vey = my * t0;
   }
@@ -515,7 +516,7 @@ SWF_HD double cell_alpha(const KCfg &k, double u1, double u2, double u3, double 
 // findalpha per cell in the fast modes: the same expression with one
 // reciprocal of rho
 SWF_HD double cell_alpha_fast(const KCfg &k, double u1, double u2, double u3, double u4) {
-  const double r = 1.0 / u1;
+  const double r = rcp64(u1);
   const double ax = fabs(u2 * r), ay = fabs(u3 * r);
   double pres;
   if (k.findalpha_ex3) {
