@@ -242,8 +242,8 @@ SWF_HD void roe_prims(const KCfg &k, const double u[4], double &w, double &vx, d
 
 // Split fluxes and Roe primitives of one stencil point in the fast modes: the
 // formulas of split_point and roe_prims with one reciprocal of rho shared by the
-// velocities and a multiplication by 1/alpha.  Reciprocals are the refined
-// hardware estimate (within 1 ulp of the division).
+// velocities and a multiplication by 1/alpha.  Reciprocals and the square root
+// are the refined hardware estimates (within 1 ulp).
 template <int DIR>
 SWF_HD void stage_point_fast(const KCfg &k, const double u[4], double ia, double fp[4], double fn[4], Prim &pr) {
   const double rho = u[0], mx = u[1], my = u[2], e = u[3];
@@ -277,7 +277,7 @@ SWF_HD void stage_point_fast(const KCfg &k, const double u[4], double ia, double
     vey = my * t0;
   }
   const double pre = k.gamma1 * (e - 0.5 * (mx * vex + my * vey));
-  pr.w = sqrt(fabs(rho));
+  pr.w = sqrt64(fabs(rho));
   pr.vx = vex;
   pr.vy = vey;
   pr.h = (pre + e) * t0;
@@ -439,7 +439,7 @@ SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const 
   const double vym = t0 * pa.vy + t1 * pb.vy;
   const double hm = t0 * pa.h + t1 * pb.h;
   const double qm = 0.5 * (vxm * vxm + vym * vym);
-  const double cm = sqrt(fabs(g1 * (hm - qm))) + 1e-15;
+  const double cm = sqrt64(fabs(g1 * (hm - qm))) + 1e-15;
   const double vn = (DIR == 0) ? vxm : vym;
   const double vt = (DIR == 0) ? vym : vxm;
   const double rcm = rcp64(cm);
@@ -526,7 +526,7 @@ SWF_HD double cell_alpha_fast(const KCfg &k, double u1, double u2, double u3, do
     const double vx = vac ? 0.0 : u2 * r, vy = vac ? 0.0 : u3 * r;
     pres = k.gamma1 * ((u4 - (0.5 * u1) * (vx * vx)) - (0.5 * u1) * (vy * vy));
   }
-  const double sound = sqrt((k.gamma * pres) * r);
+  const double sound = sqrt64((k.gamma * pres) * r);
   return fmax2(ax + sound, ay + sound);
 }
 

@@ -57,6 +57,24 @@ SWF_HD double rcp64(double x) {
 #endif
 }
 
+// FP64 square root for x = 0 and x in [2^-900, 2^900]: the hardware reciprocal
+// square root y, then one Goldschmidt step on (g, h) = (x y, y / 2) and one
+// correction with the residual x - g^2 (within 1 ulp); the library routine on the
+// host.  The library's device routine adds range scaling and a second correction.
+SWF_HD double sqrt64(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  return (x == 0.0) ? 0.0 : fma(d, h, g);
+#else
+  return sqrt(x);
+#endif
+}
+
 // weno_right(a) - a(0) (RIGHT) or weno_left(a) - a(0), FP64
 template <bool RIGHT>
 SWF_HD double weno_d64(const double e[4], double q4) {
