@@ -49,7 +49,9 @@ struct YSmem {
   double fp[4][YT_PTS][YT_W];
   double fn[4][YT_PTS][YT_W];
   double pr[4][YT_PTS][YT_W];
-  double G[4][YT_H][YT_W];  // net y-interface fluxes of the tile
+  // net y-interface fluxes of each wave's last row (the next wave's first cells
+  // read them; within a wave the row below comes by a shuffle)
+  double G[YS_T / 64][4][YT_W];
   double red[YS_T / 64];
   double ured[4][YS_T / 64];  // weight*u sums of the waves
   int zred[2 * YS_T / 64];
@@ -471,9 +473,22 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
                                             STEP ? aline[i + 2] : alpha, G);
   // a tile's first interface is the previous tile's last: counted there
   if (ty == 0 && by > 0) lim = 0;
+  // G(j-1/2) of cell j = jf: the thread YT_W lanes below, across a wave boundary
+  // the previous wave's last row through LDS
+  constexpr int RPW = 64 / YT_W;  // rows per wave
+  const int wv = tid >> 6;
+  double Gb[4];
+  if (ty % RPW == RPW - 1) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) sm.G[c][ty][tx] = G[c];
+    for (int c = 0; c < 4; ++c) sm.G[wv][c][tx] = G[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) Gb[c] = __shfl_up(G[c], YT_W, 64);
   __syncthreads();
+  if (ty % RPW == 0 && wv > 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Gb[c] = sm.G[wv - 1][c][tx];
+  }
   // cell j = jf: Y = G(j+1/2) - G(j-1/2), X = F(i+1/2) - F(i-1/2)
   double anew = 0.0;
   double wu[4] = {0.0, 0.0, 0.0, 0.0};
@@ -484,7 +499,7 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
     for (int c = 0; c < 4; ++c) {
       const double *fx = b.Fx + c * k.plane;
       const double X = fx[o] - fx[o - 1];
-      const double Y = G[c] - sm.G[c][ty - 1][tx];
+      const double Y = G[c] - Gb[c];
       if (M == MODE_EXACT)
         rhs[c] = k.alpha_in_scheme ? -((alpha * X) / k.dx) - (alpha * Y) / k.dy : -(X / k.dx) - Y / k.dy;
       else
