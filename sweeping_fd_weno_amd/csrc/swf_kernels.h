@@ -348,51 +348,61 @@ SWF_D double block_max(double a, double *red, int tid) {
 }
 
 // After a sweep: alpha partials of the update tiles whose cells changed, and
-// the edge ghosts of the boundary band.  One workgroup of NT threads (NT >= 256).
+// the edge ghosts of the boundary band.  One workgroup of NT threads (NT >= 512).
+// The changed tiles go NT / 32 at a time, 32 threads per tile, each thread's
+// cells loaded together; the tile's max by shuffles within its 32 lanes.
 template <int NT>
 SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double *red, int tid) {
+  static_assert(NT % 64 == 0 && NT >= 512, "whole waves, 16 tiles per round at least");
+  constexpr int RT = NT / 32;                      // tiles per round
+  constexpr int CELLS = UB_W * UB_H;
+  constexpr int PER = (CELLS + 31) / 32;           // cells per thread
   const int gx = ub_gx(k), nblk = gx * ub_gy(k);
   const int nw = sh.all_dirty ? (nblk + 31) / 32 : min(SW_DIRTY_WORDS, (nblk + 31) / 32);
-  constexpr int PER = NT / 256;  // tiles handled at once
-  for (int w = 0; w < nw; ++w) {
-    unsigned word = sh.all_dirty ? 0xffffffffu : sh.dirty[w];
-    while (word) {
-      // up to PER dirty tiles of this word, one 256-thread group each
-      int blks[PER];
+  const int grp = tid >> 5, lt = tid & 31;
+  // the dirty-tile bitmap in order; every thread walks it alike (uniform)
+  int w = 0;
+  unsigned word = (nw > 0) ? (sh.all_dirty ? 0xffffffffu : sh.dirty[0]) : 0u;
+  while (true) {
+    int myblk = -1;
+    bool any = false;
 #pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        if (word) {
-          blks[q] = w * 32 + (__ffs(word) - 1);
-          word &= word - 1;
-        } else {
-          blks[q] = -1;
+    for (int q = 0; q < RT; ++q) {
+      while (!word && w + 1 < nw) {
+        ++w;
+        word = sh.all_dirty ? 0xffffffffu : sh.dirty[w];
+      }
+      if (word) {
+        const int blk = w * 32 + (__ffs(word) - 1);
+        word &= word - 1;
+        if (blk < nblk) {
+          any = true;
+          if (q == grp) myblk = blk;
         }
       }
-      const int grp = tid >> 8, lt = tid & 255;
-      const int blk = blks[grp];
-      double a = 0.0;
-      if (blk >= 0 && blk < nblk) {
-        for (int c = lt; c < UB_W * UB_H; c += 256) {  // a tile's cells, 256 at a time
-          const int i = 1 + (blk % gx) * UB_W + c % UB_W, j = 1 + (blk / gx) * UB_H + c / UB_W;
-          if (i <= k.nx && j <= k.ny) {
-            double v[4];
-            load_ij(k, b, i, j, v);
-            a = fmax2(a, cell_alpha(k, v[0], v[1], v[2], v[3]));
-          }
-        }
-      }
-      // max within each 256-thread group: waves 4*grp .. 4*grp+3
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double o = __shfl_xor(a, off, 64);
-        a = (o > a) ? o : a;
-      }
-      if ((tid & 63) == 0) red[tid >> 6] = a;
-      __syncthreads();
-      if (lt == 0 && blk >= 0 && blk < nblk)
-        b.apart[blk] = fmax2(fmax2(red[4 * grp], red[4 * grp + 1]), fmax2(red[4 * grp + 2], red[4 * grp + 3]));
-      __syncthreads();
     }
+    if (!any) break;
+    double a = 0.0;
+    if (myblk >= 0) {
+      const int i0 = 1 + (myblk % gx) * UB_W, j0 = 1 + (myblk / gx) * UB_H;
+      double v[PER][4];
+      bool ok[PER];
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const int c = lt + 32 * r, i = i0 + c % UB_W, j = j0 + c / UB_W;
+        ok[r] = c < CELLS && i <= k.nx && j <= k.ny;
+        if (ok[r]) load_ij(k, b, i, j, v[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < PER; ++r)
+        if (ok[r]) a = fmax2(a, cell_alpha(k, v[r][0], v[r][1], v[r][2], v[r][3]));
+    }
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) {
+      const double o = __shfl_xor(a, off, 64);
+      a = (o > a) ? o : a;
+    }
+    if (lt == 0 && myblk >= 0) b.apart[myblk] = a;
   }
   // ghosts of the boundary band
   const int nb = 6 * k.ny + 6 * k.nx;
