@@ -247,10 +247,33 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
   if (LIM && stage > 0) zs_store<256>(zs_den(lim), zs_pre(lim), b.zs + 2 * (bx + by * gx), sm.zred, tid);
 }
 
+// Workgroups are dealt to the eight XCDs in turn by linear id.  In the fast modes
+// each XCD gets a contiguous run of tiles instead, so that neighbouring tiles,
+// which share stencil halos, share an L2 (Sedov 640^2 FP64: 0.137 -> 0.131 ms per
+// step).  The exact mode keeps the dealt order: its tiles differ in cost (the
+// binary128 fallback gathers in near-constant regions), and contiguous runs left
+// the XCDs unbalanced (0.89 -> 1.04 ms per step).  Returns the tile of workgroup
+// (blockIdx.x, blockIdx.y).
+template <int M>
+SWF_D void xcd_tile(int gx, int gy, int &bx, int &by) {
+  if (M == MODE_EXACT) {
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    return;
+  }
+  const int n = gx * gy, b = blockIdx.x + blockIdx.y * gx;
+  const int q = n >> 3, r = n & 7, x = b & 7, kk = b >> 3;
+  const int t = x * q + min(x, r) + kk;
+  bx = t % gx;
+  by = t / gx;
+}
+
 template <int M, bool LIM, bool STEP = false>
 __global__ SWF_FLUX_BOUNDS(M) void k_xflux(KCfg k, Bufs b, int stage, int slot_in, int parity) {
   __shared__ XSmem sm;
-  xflux_tile<M, LIM, STEP>(k, b, stage, slot_in, parity, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, sm);
+  int bx, by;
+  xcd_tile<M>(gridDim.x, gridDim.y, bx, by);
+  xflux_tile<M, LIM, STEP>(k, b, stage, slot_in, parity, bx, by, gridDim.x, gridDim.y, sm);
 }
 
 // dt of the step (ex4_sedov_2d/main.f90:696-700; ex3 :570-572)
@@ -592,7 +615,9 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
 template <int M, bool LIM, bool STEP = false>
 __global__ SWF_STAGE_BOUNDS(M) void k_ystage(KCfg k, Bufs b, int stage, int slot_in, int parity, int flags) {
   __shared__ YSmem sm;
-  ystage_tile<M, LIM, STEP>(k, b, stage, slot_in, parity, flags, blockIdx.x, blockIdx.y, gridDim.x, sm);
+  int bx, by;
+  xcd_tile<M>(gridDim.x, gridDim.y, bx, by);
+  ystage_tile<M, LIM, STEP>(k, b, stage, slot_in, parity, flags, bx, by, gridDim.x, sm);
 }
 
 // One limited half flux outside the flux tiles (use_limiter == 1): the half of
