@@ -101,8 +101,8 @@ SWF_HD double weno_d64(const double e[4], double q4) {
   return ((b1 * c3 + b2 * c2) + b3 * c1) / (6.0 * ((b1 + b2) + b3));
 }
 
-// Unnormalised nonlinear weights in FP32 from the smoothness indicators' FP64
-// arguments x_k, y_k rounded to FP32: alpha_k = d_k (min(q)/q_k)^2 with
+// Unnormalised nonlinear weights in FP32 from the smoothness indicators'
+// arguments x_k, y_k in FP32: alpha_k = d_k (min(q)/q_k)^2 with
 // q = 4e-6 + 4 beta.  Every alpha_k carries its own relative rounding error only;
 // the weight of a stencil across a jump underflows to 0 as its exact value would
 // be negligible.  The smallest-q stencil has alpha = d_k >= 0.1, so the sum never
@@ -129,13 +129,12 @@ SWF_HD void alphas_d32(const float x[3], const float y[3], float a[3], float q4)
 template <bool RIGHT>
 SWF_HD double weno_d32_or_64(const double e[4], double q4, float q4f) {
   const double em2 = e[0], em1 = e[1], e0 = e[2], e1 = e[3];
-  const double x1 = e1 - e0, y1 = e1 - 3.0 * e0;
-  const double x2 = e0 - em1, y2 = em1 + e0;
-  const double x3 = em1 - em2, y3 = 3.0 * em1 - em2;
-  const float x[3] = {(float)x1, (float)x2, (float)x3}, y[3] = {(float)y1, (float)y2, (float)y3};
-  const float big = fmaxf(fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(y[0]))),
-                          fmaxf(fabsf(y[1]), fabsf(y[2])));
-  if (!(big <= 1e18f)) return weno_d64<RIGHT>(e, q4);
+  // the smoothness arguments in FP32 from the FP32 differences; |e| <= 2.5e17
+  // keeps |x|, |y| <= 1e18
+  const float f0 = (float)em2, f1 = (float)em1, f2 = (float)e0, f3 = (float)e1;
+  const float big = fmaxf(fmaxf(fabsf(f0), fabsf(f1)), fmaxf(fabsf(f2), fabsf(f3)));
+  if (!(big <= 2.5e17f)) return weno_d64<RIGHT>(e, q4);
+  const float x[3] = {f3 - f2, f2 - f1, f1 - f0}, y[3] = {f3 - 3.0f * f2, f1 + f2, 3.0f * f1 - f0};
   float al[3];
   alphas_d32<RIGHT>(x, y, al, q4f);
   const double a1 = al[0], a2 = al[1], a3 = al[2];
