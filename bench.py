@@ -1,23 +1,34 @@
-"""Benchmark: cell-stage updates/s on 2D Sedov 640^2 to T = 1, FP64 vs FP32-bulk.
+"""Benchmark: cell-stage updates/s on 2D Sedov 640^2, FP64 vs FP32-bulk at stated L1 error.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-full-run]
     python bench.py --workload sedov1d     (the 1D program's case: 800 cells to T = 1e-3)
 
 One step = one SSP-RK3 step (three stages: WENO fluxes, RK update with the
 positivity check, sweep when flagged) over the whole 640^2 domain, inputs
-resident in HBM.  By default the timed region is the complete run from t = 0 to
-T = 1 (28650 steps), so it contains every sweep the run triggers.  The state is
-re-initialised after the warmup steps (untimed).  Each precision mode is timed
-separately; `value` is the plain FP64 path, the FP32-bulk and FP64-exact paths are
-reported beside it together with their L1 density error against FP64-exact, which
-is bit-identical to the reference's FP64 run.
+resident in HBM.  The timed region is K steps from t = 0 (the program's initial
+data and initial bc), or the complete run to T = 1 (28 650 steps) when K reaches
+it; `config.workload` names which.  The W warmup steps run on the same solver
+before it (they capture its 16-step graph), then the initial state is uploaded
+again.  Each precision mode is timed separately; `value` is the plain FP64 path.
+
+Unless --no-full-run, every mode is also run from t = 0 to T = 1 (the accuracy
+target: positivity held, the sweep statistics, the relative L1 density
+difference from FP64-exact, which is bit-identical to the reference's FP64
+program) and reported under `full_run_to_T1`.
+
+cpu_baseline: the reference program itself (ex4_sedov_2d built as a one-rank
+copy by oracle/Makefile, `oracle/_ref/bin/ex4_640w`: 640^2 from t = 0 to
+T = 1.2e-5, three steps, timed by its own MPI_Wtime) on one host core; the C
+oracle's single step is reported beside it as `port`.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -54,8 +65,9 @@ FLOPS_PER_CELL_STAGE_1D = 673.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=STEPS_TO_T1)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--no-full-run", action="store_true", help="skip the runs to T = 1")
     ap.add_argument("--modes", default="fp64,fp32_bulk,fp64_exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -65,29 +77,76 @@ def parse():
     return ap.parse_args()
 
 
+def _fresh(s, u0):
+    """The program's start: initial data, then its bc (ex4_sedov_2d/main.f90:1186)."""
+    s.upload(u0)
+    s.bc()
+    s.sync()
+
+
 def run_mode(S, prec: str, steps: int, warmup: int):
     cfg = S.sedov_config(N, prec)
     u0 = S.init_sedov(cfg)
-    tfinal = T_END if steps <= STEPS_TO_T1 else 1e30
     s = S.Solver(cfg)
-    s.upload(u0)
-    if warmup:
-        s.advance(0.0, tfinal, max_steps=warmup)
-    s.close()
-    s = S.Solver(cfg)
-    s.upload(u0)
-    s.sync()
+    _fresh(s, u0)
+    s.advance(0.0, T_END, max_steps=warmup)  # also captures the solver's step graph (first advance)
+    _fresh(s, u0)
+    st0 = s.stats()
     t0 = time.perf_counter()
-    t, k = s.advance(0.0, tfinal, max_steps=steps)
+    t, k = s.advance(0.0, T_END, max_steps=steps)
     s.sync()
     wall = time.perf_counter() - t0
     rho = s.download()[3:-3, 3:-3, 0].copy()
-    stats = s.stats()
+    st1 = s.stats()
     xms = s.time_kernel("xflux", 20)
     yms = s.time_kernel("ystage", 20)
     s.close()
-    return dict(steps=k, t_end=t, seconds=wall, value=N * N * 3 * k / wall, ms_per_step=1e3 * wall / max(k, 1),
-                xflux_kernel_ms=xms, ystage_kernel_ms=yms, stats=stats), rho
+    window = {key: st1[key] - st0[key] for key in ("sweep_count", "total_sweep", "total_neg", "total_pos",
+                                                     "den_count", "pre_count")}
+    return dict(steps=k, t_reached=t, seconds=wall, value=N * N * 3 * k / wall, ms_per_step=1e3 * wall / max(k, 1),
+                xflux_kernel_ms=xms, ystage_kernel_ms=yms, window_stats=window), rho
+
+
+def run_full(S, prec: str):
+    """From t = 0 to T = 1 in one call (graph captured beforehand): rate, statistics,
+    final density, smallest density and pressure."""
+    cfg = S.sedov_config(N, prec)
+    u0 = S.init_sedov(cfg)
+    s = S.Solver(cfg)
+    _fresh(s, u0)
+    s.advance(0.0, T_END, max_steps=16)  # also captures the solver's step graph (first advance)
+    _fresh(s, u0)
+    z = s.stats()
+    t0 = time.perf_counter()
+    t, k = s.advance(0.0, T_END)
+    s.sync()
+    wall = time.perf_counter() - t0
+    u = s.download()[3:-3, 3:-3]
+    st = {key: v - z[key] if key in ("sweep_count", "den_count", "pre_count", "total_sweep", "total_neg",
+                                     "total_pos", "stage_sweeps", "steps") else v for key, v in s.stats().items()}
+    s.close()
+    pres = 0.4 * (u[..., 3] - 0.5 * u[..., 1] ** 2 / u[..., 0] - 0.5 * u[..., 2] ** 2 / u[..., 0])
+    return dict(steps=k, t_reached=t, seconds=wall, value=N * N * 3 * k / wall, stats=st,
+                min_rho=float(u[..., 0].min()), min_p=float(pres.min())), u[..., 0].copy()
+
+
+def cpu_baseline_reference():
+    """The reference program (ex4_sedov_2d, one-rank build copy, oracle/_ref/bin/ex4_640w):
+    640^2 from t = 0 to 1.2e-5, timed by its own MPI_Wtime around the time loop."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "bin", "ex4_640w")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe} {d}"], cwd=d, capture_output=True,
+                           text=True, timeout=600)
+    line = [l for l in r.stdout.splitlines() if l.strip().startswith("CPU Time")]
+    if r.returncode != 0 or not line:
+        return None
+    secs, steps = float(line[0].split()[2]), int(line[0].split()[3])
+    return {"value": N * N * 3 * steps / secs, "unit": "cell-stage updates/s", "cores": 1, "kind": "reference",
+            "sample": f"{steps} SSP-RK3 steps of Sedov 640^2 from t=0 to 1.2e-5 by the reference program "
+                      f"ex4_sedov_2d (one-rank build copy, flang -O3 -fdefault-real-8), {secs:.1f} s by its own "
+                      f"MPI_Wtime timer, single thread"}
 
 
 def cpu_baseline(steps: int):
@@ -102,6 +161,15 @@ def cpu_baseline(steps: int):
     return {"value": N * N * 3 * k / wall, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
             "sample": f"{k} SSP-RK3 step(s) of Sedov 640^2 from t=0 by the C oracle (FP64 with binary128 sites "
                       f"as the reference build), single thread, {wall:.1f} s"}
+
+
+def cpu_baselines(steps: int):
+    port = cpu_baseline(steps)
+    ref = cpu_baseline_reference()
+    if ref is None:
+        return port
+    ref["port"] = port
+    return ref
 
 
 def traffic(kernel: str):
@@ -204,6 +272,16 @@ def main():
         for m in modes:
             if m != "fp64_exact":
                 err[m] = float(np.abs(rho[m] - ref).sum() / np.abs(ref).sum())
+    full, frho = {}, {}
+    if not a.no_full_run:
+        for m in modes:
+            full[m], frho[m] = run_full(S, m)
+        if "fp64_exact" in frho:
+            ref = frho["fp64_exact"]
+            for m in modes:
+                full[m]["l1_density_rel_vs_fp64_exact"] = float(np.abs(frho[m] - ref).sum() / np.abs(ref).sum())
+        if "fp32_bulk" in full and "fp64" in full:
+            full["fp32_bulk_speedup_vs_fp64"] = full["fp32_bulk"]["value"] / full["fp64"]["value"]
     head = res.get("fp64") or res[modes[0]]
     # dominant kernel: the stage kernel (y-pass of wenofd + per-cell stage work)
     ach_y = (FLOPS_PER_CELL_DIR + FLOPS_PER_CELL_UPDATE) * N * N / (head["ystage_kernel_ms"] * 1e-3)
@@ -222,7 +300,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "sedov2d_640x640_to_T1", "grid": [N, N], "cfl": 0.5, "t_end": T_END,
+        "config": {"workload": ("sedov2d_640x640_to_T1" if not S.time_continue(head["t_reached"], T_END)
+                                else f"sedov2d_640x640_steps0-{head['steps']}"),
+                   "grid": [N, N], "cfl": 0.5, "t_start": 0.0, "t_reached": head["t_reached"],
                    "precision": "fp64", "parallelism": "single GPU, whole domain"},
         "roofline": {"bound": "valu", "achieved": ach_y / 1e12, "peak": FP64_VALU_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": ach_y / FP64_VALU_PEAK, "traffic": traffic("k_ystage<fp64>"),
@@ -238,8 +318,10 @@ def main():
     }
     if "fp32_bulk" in res and "fp64" in res:
         out["fp32_bulk_speedup_vs_fp64"] = res["fp32_bulk"]["value"] / res["fp64"]["value"]
+    if full:
+        out["full_run_to_T1"] = full
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_steps)
+        out["cpu_baseline"] = cpu_baselines(a.cpu_steps)
     print(json.dumps(out))
 
 

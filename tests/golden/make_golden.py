@@ -24,7 +24,18 @@ oracle/_ref/, run there, and only its OUTPUTS are stored here:
                           swp_num 2) and a 40-step cfl 1.5 run stopped by the program's
                           max_iter branch: SHA-256 of den/pre/vel, samples, statistics.
 
-Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5s] [ex5]
+  ref_ex4_sedov.json      ex4_sedov_2d as a one-rank build copy (oracle/Makefile: `use MPI`
+                          replaced by `include 'mpif.h'`, cut_x = cut_y = 1, grid and
+                          time_terminal PARAMETERs): 64^2 to T = 0.002, 128^2 to T = 0.01
+                          and the benchmark grid 640^2 to T = 1.2e-5 (3 steps, 2 sweeps):
+                          SHA-256 of the den/pre/m1/m2 columns and files, samples, the
+                          statistics, the step count and the printout minus `CPU Time`.
+  ref_ex6_step.json       ex6_shock_diffraction as one-rank build copies: total = 8
+                          (104 x 88) to the shipped T = 2.3 (150 steps, 190 sweeps, 57
+                          density sweeps) and total = 16 (208 x 176) to T = 0.5: the same
+                          records for den.dat / pre.dat.
+
+Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5s] [ex5] [ex4] [ex6]
 """
 from __future__ import annotations
 
@@ -245,6 +256,89 @@ def make_ex5_small() -> None:
         json.dump(out, f, indent=1)
 
 
+def _run_mpi1(build: str) -> tuple:
+    """Runs a one-rank build copy with its output directory as the first argument (as the
+    shipped programs take it); returns (run dir, printout)."""
+    rundir = os.path.join(REF, "run_" + build)
+    os.makedirs(rundir, exist_ok=True)
+    res = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + os.path.join(REF, build, "a") + " " + rundir],
+                         cwd=rundir, check=True, capture_output=True, text=True)
+    with open(os.path.join(rundir, "stdout.txt"), "w") as f:
+        f.write(res.stdout)
+    return rundir, res.stdout
+
+
+def _printout_minus_timer(text: str) -> str:
+    """The printout without the `CPU Time` line (wall-clock dependent); its step count and
+    final time are kept apart."""
+    return "\n".join(l for l in text.splitlines() if not l.strip().startswith("CPU Time")) + "\n"
+
+
+def _timer_line(text: str) -> tuple:
+    l = [l for l in text.splitlines() if l.strip().startswith("CPU Time")][0].split()
+    return float(l[2]), int(l[3]), l[4]
+
+
+def make_ex4() -> None:
+    cases = {
+        "n64": ("ex4_64", 64, 0.002, "total_x = total_y = 64, time_terminal = 0.002"),
+        "n128": ("ex4_128", 128, 0.01, "total_x = total_y = 128, time_terminal = 0.01"),
+        "n640w": ("ex4_640w", 640, 1.2e-5, "total_x = total_y = 640 (as shipped), time_terminal = 1.2e-5"),
+    }
+    out = {"layout": "value columns of den.dat / pre.dat / m1.dat / m2.dat: (j outer, i inner) over the "
+                     "interior; pre = gamma1*(E - 0.5*m1**2/rho - 0.5*m2**2/rho) as the program writes it",
+           "sample_stride": 97}
+    for key, (build, n, T, desc) in cases.items():
+        rundir, text = _run_mpi1(build)
+        fields = {}
+        for nm in ("den", "pre", "m1", "m2"):
+            fields[nm] = _tokens_skip(os.path.join(rundir, nm + ".dat"), 3, 2 if nm == "den" else 1)[:, 2]
+        secs, steps, tfin = _timer_line(text)
+        out[key] = {
+            "program": "ex4_sedov_2d, one-rank build copy (use MPI -> include 'mpif.h', cut_x = cut_y = 1), " + desc,
+            "n": n, "t_end": T, "steps": steps, "t_printed": tfin, "reference_seconds_1core": secs,
+            "sha256": {k: field_digest(v) for k, v in fields.items()},
+            "sample": {k: v[::97].tolist() for k, v in fields.items()},
+            "stats": _parse_stats(text),
+            "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
+                            for fn in ("den.dat", "pre.dat", "m1.dat", "m2.dat")},
+            "printout_minus_timer": _printout_minus_timer(text),
+        }
+    with open(os.path.join(GOLD, "ref_ex4_sedov.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+def make_ex6() -> None:
+    cases = {
+        "total8": ("ex6_8", 8, 2.3, "total = 8 (104 x 88), time_terminal = 2.3 (as shipped)"),
+        "total16": ("ex6_16", 16, 0.5, "total = 16 (208 x 176), time_terminal = 0.5"),
+    }
+    out = {"layout": "value columns of den.dat / pre.dat: (j outer, i inner) over the interior, 0 where "
+                     "i < total and j < 6*total (the program blanks the step)", "sample_stride": 53}
+    for key, (build, total, T, desc) in cases.items():
+        rundir, text = _run_mpi1(build)
+        fields = {nm: _tokens_skip(os.path.join(rundir, nm + ".dat"), 3, 1)[:, 2] for nm in ("den", "pre")}
+        secs, steps, tfin = _timer_line(text)
+        out[key] = {
+            "program": "ex6_shock_diffraction, one-rank build copy (use MPI -> include 'mpif.h', cut_x = cut_y = 1), "
+                       + desc,
+            "total": total, "t_end": T, "steps": steps, "t_printed": tfin, "reference_seconds_1core": secs,
+            "sha256": {k: field_digest(v) for k, v in fields.items()},
+            "sample": {k: v[::53].tolist() for k, v in fields.items()},
+            "stats": _parse_stats(text),
+            "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
+                            for fn in ("den.dat", "pre.dat")},
+            "printout_minus_timer": _printout_minus_timer(text),
+        }
+    with open(os.path.join(GOLD, "ref_ex6_step.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+def _tokens_skip(fn: str, ncol: int, skip: int) -> np.ndarray:
+    body = open(fn).read().split("\n", skip)[skip]
+    return np.array(body.split(), dtype=np.float64).reshape(-1, ncol)
+
+
 def make_ex5() -> None:
     rundir = os.path.join(REF, "run_ex5")
     if not os.path.exists(os.path.join(rundir, "DONE")):
@@ -287,3 +381,7 @@ if __name__ == "__main__":
         make_ex5_small()
     if "ex5" in what:
         make_ex5()
+    if "ex4" in what:
+        make_ex4()
+    if "ex6" in what:
+        make_ex6()
