@@ -141,6 +141,17 @@ int64_t swf_track(swf_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos, int64_t 
 int swf_weno_points(const double *a, double *right, double *left, int64_t n, int32_t precision,
                     int32_t device);
 
+/* swf_weno_points without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The device staging of
+ * the stencils and results lives in one array of the library per GPU, of
+ * SWF_WENO_NOALLOC_MAX stencils (3.5 MiB, held until the process ends); larger n
+ * returns SWF_E_INVALID.  Results equal swf_weno_points'.  Calls on one GPU share
+ * that array: calls that overlap on one GPU (from several host threads) race. */
+#define SWF_WENO_NOALLOC_MAX 65536
+int swf_weno_points_noalloc(const double *a, double *right, double *left, int64_t n, int32_t precision,
+                            int32_t device);
+
 /* Average duration (ms) of one launch of a hot kernel on the context's stream,
  * measured with HIP events over `nreps` back-to-back launches on the current state
  * (which: 0 = the x-flux kernel, the x-pass of `wenofd`; 1 = the stage kernel, the

@@ -98,6 +98,7 @@ def lib():
         L.swf_track.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int64), C.c_int64]
         L.swf_track.restype = C.c_int64
         L.swf_weno_points.argtypes = [dp, dp, dp, C.c_int64, C.c_int32, C.c_int32]
+        L.swf_weno_points_noalloc.argtypes = [dp, dp, dp, C.c_int64, C.c_int32, C.c_int32]
         L.swf_time_continue.argtypes = [C.c_double, C.c_double]
         L.swf_sync.argtypes = [vp]
         L.swf_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, dp]
@@ -230,6 +231,21 @@ def weno_points(a: np.ndarray, precision="fp64_exact", device: int = 0, eps12: b
     l = np.empty(a.shape[0])
     flags = _prec(precision) | (0x10 if eps12 else 0)
     _check(lib().swf_weno_points(_dp(a), _dp(r), _dp(l), a.shape[0], flags, device), "swf_weno_points")
+    return r, l
+
+
+WENO_NOALLOC_MAX = 65536  # SWF_WENO_NOALLOC_MAX
+
+
+def weno_points_noalloc(a: np.ndarray, precision="fp64_exact", device: int = 0, eps12: bool = False):
+    """weno_points through the entry point that allocates no device memory (at most
+    WENO_NOALLOC_MAX stencils per call)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 5)
+    r = np.empty(a.shape[0])
+    l = np.empty(a.shape[0])
+    flags = _prec(precision) | (0x10 if eps12 else 0)
+    _check(lib().swf_weno_points_noalloc(_dp(a), _dp(r), _dp(l), a.shape[0], flags, device),
+           "swf_weno_points_noalloc")
     return r, l
 
 

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -579,6 +580,62 @@ int swf_weno_points(const double *a, double *right, double *left, int64_t n, int
   if (e == hipSuccess) e = hipMemcpy(left, d + 6 * n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "swf_weno_points");
+  return SWF_OK;
+}
+
+}  // extern "C"
+
+// The staging of swf_weno_points_noalloc: stencils (5 per point), then the right and
+// left values, SWF_WENO_NOALLOC_MAX points.  Host code uses only the address that
+// hipGetSymbolAddress returns for the current device, never this name.
+static __device__ __align__(256) double scratch[7 * (size_t)SWF_WENO_NOALLOC_MAX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// this device's address of `scratch`, taken once per device with it current
+int scratch_address(int device, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    HIPC(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return SWF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int swf_weno_points_noalloc(const double *a, double *right, double *left, int64_t n, int32_t precision,
+                            int32_t device) {
+  const int e12 = (precision & SWF_WENO_EPS12) != 0;
+  precision &= ~SWF_WENO_EPS12;
+  if (!a || !right || !left || n < 0 || precision < 0 || precision > 2) return SWF_E_INVALID;
+  if (n > SWF_WENO_NOALLOC_MAX) {
+    snprintf(g_last_error, sizeof(g_last_error), "swf_weno_points_noalloc: %lld stencils exceed the maximum %d",
+             (long long)n, SWF_WENO_NOALLOC_MAX);
+    return SWF_E_INVALID;
+  }
+  if (n == 0) return SWF_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev || device >= kMaxDevices)
+    return SWF_E_NODEVICE;
+  HIPC(hipSetDevice(device));
+  double *d = nullptr;
+  int rc = scratch_address(device, &d);
+  if (rc) return rc;
+  hipError_t e = hipMemcpy(d, a, sizeof(double) * 5 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_weno_points(precision, d, d + 5 * n, d + 6 * n, n, e12, 0);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(right, d + 5 * n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(left, d + 6 * n, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "swf_weno_points_noalloc");
   return SWF_OK;
 }
 

@@ -23,6 +23,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 
 struct int2 {
   int x, y;
@@ -152,6 +154,11 @@ template <class T> hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((voi
 hipError_t hipFree(void *);
 hipError_t hipMemcpy(void *, const void *, size_t, hipMemcpyKind);
 hipError_t hipMemcpyAsync(void *, const void *, size_t, hipMemcpyKind, hipStream_t = nullptr);
+// a __device__ variable is host memory here: its address is the symbol's
+inline hipError_t hipGetSymbolAddress(void **p, const void *symbol) {
+  *p = const_cast<void *>(symbol);
+  return hipSuccess;
+}
 hipError_t hipMemsetAsync(void *, int, size_t, hipStream_t = nullptr);
 hipError_t hipGetLastError();
 const char *hipGetErrorString(hipError_t);
