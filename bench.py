@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--no-full-run", action="store_true", help="skip the runs to T = 1")
-    ap.add_argument("--modes", default="fp64,fp32_bulk,fp64_exact")
+    ap.add_argument("--modes", default="fp64,fp32_bulk,fp32_full,fp64_exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--workload", default="sedov2d", choices=["sedov2d", "sedov1d"])
@@ -280,8 +280,9 @@ def main():
             ref = frho["fp64_exact"]
             for m in modes:
                 full[m]["l1_density_rel_vs_fp64_exact"] = float(np.abs(frho[m] - ref).sum() / np.abs(ref).sum())
-        if "fp32_bulk" in full and "fp64" in full:
-            full["fp32_bulk_speedup_vs_fp64"] = full["fp32_bulk"]["value"] / full["fp64"]["value"]
+        for m in ("fp32_bulk", "fp32_full"):
+            if m in full and "fp64" in full:
+                full[m + "_speedup_vs_fp64"] = full[m]["value"] / full["fp64"]["value"]
     head = res.get("fp64") or res[modes[0]]
     # dominant kernel: the stage kernel (y-pass of wenofd + per-cell stage work)
     ach_y = (FLOPS_PER_CELL_DIR + FLOPS_PER_CELL_UPDATE) * N * N / (head["ystage_kernel_ms"] * 1e-3)
@@ -316,8 +317,9 @@ def main():
         "modes": res,
         "l1_density_rel_vs_fp64_exact": err,
     }
-    if "fp32_bulk" in res and "fp64" in res:
-        out["fp32_bulk_speedup_vs_fp64"] = res["fp32_bulk"]["value"] / res["fp64"]["value"]
+    for m in ("fp32_bulk", "fp32_full"):
+        if m in res and "fp64" in res:
+            out[m + "_speedup_vs_fp64"] = res[m]["value"] / res["fp64"]["value"]
     if full:
         out["full_run_to_T1"] = full
     if not a.no_cpu_baseline:

@@ -43,6 +43,12 @@ extern "C" {
 #define SWF_FP64_EXACT 0 /* bit-identical to the reference (binary128 sites emulated) */
 #define SWF_FP64 1       /* plain binary64 */
 #define SWF_FP32_BULK 2  /* FP32 smoothness indicators / weights, FP64 state and sweep */
+#define SWF_FP32_FULL 3  /* FP32-bulk with the eigen-projections, candidate values, reconstruction and
+                          * back-projection in FP32 too (Roe averages, centre values, RK, pressure,
+                          * positivity and the sweep in FP64): the Sedov and periodic (vortex)
+                          * problems.  The jets (Mach 2000: ill-conditioned projections) and the step
+                          * program (chaotic on coarse grids) run the SWF_FP32_BULK form under it.
+                          * 2D contexts and batches (no jet instances); swf1d_create takes 0..2 */
 
 /* boundary conditions */
 #define SWF_BC_SEDOV 0    /* ex4: reflective x=0, y=0; zero-gradient x=xm, y=ym */

@@ -17,7 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_HERE, "libswf.so")
 
-PRECISIONS = {"fp64_exact": 0, "fp64": 1, "fp32_bulk": 2}
+PRECISIONS = {"fp64_exact": 0, "fp64": 1, "fp32_bulk": 2, "fp32_full": 3}
 BC_SEDOV, BC_PERIODIC, BC_JET_HALF, BC_JET_FULL, BC_STEP = 0, 1, 2, 3, 4
 
 SWF_OK, SWF_E_PBAR, SWF_E_SWEEP_FAIL, SWF_E_PSUM = 0, 1, 2, 3

@@ -51,6 +51,17 @@ struct swf_ctx {
   hipGraphExec_t gexec;  // kGraphSteps steps, captured on first use
 };
 
+// The kernels' precision mode for a configuration.  SWF_FP32_FULL keeps the
+// FP32-bulk form (FP64 projections and candidates) for the jets, whose Mach 2000
+// makes the characteristic projections ill-conditioned (condition ~ Mach^2), and for
+// the step program, chaotic on its coarse grids (FP32 projections: 2.3e-6 relative
+// L1 density after 10 steps at 104 x 88, against 1e-6).
+static int kernel_mode(int precision, int bc) {
+  if (precision == SWF_FP32_FULL && (bc == SWF_BC_JET_HALF || bc == SWF_BC_JET_FULL || bc == SWF_BC_STEP))
+    return MODE_FP32;
+  return precision;
+}
+
 static int check_launch(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, what);
@@ -85,8 +96,8 @@ int swf_time_continue(double time, double tfinal) { return time_continue_q(time,
 int swf_create(const swf_config *cfg, swf_ctx **out) {
   if (!cfg || !out) return SWF_E_INVALID;
   *out = nullptr;
-  if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->dx > 0) || !(cfg->dy > 0) || cfg->precision < 0 || cfg->precision > 2 ||
-      cfg->bc < 0 || cfg->bc > 4 || cfg->use_limiter < 0 || cfg->use_limiter > 2 ||
+  if (cfg->nx < 4 || cfg->ny < 4 || !(cfg->dx > 0) || !(cfg->dy > 0) || cfg->precision < 0 ||
+      cfg->precision > SWF_FP32_FULL || cfg->bc < 0 || cfg->bc > 4 || cfg->use_limiter < 0 || cfg->use_limiter > 2 ||
       (cfg->use_limiter == 2 && cfg->swp_num < 1))
     return SWF_E_INVALID;
   // the step program: its bc and walls need the step inside the grid; the per-line
@@ -110,8 +121,9 @@ int swf_create(const swf_config *cfg, swf_ctx **out) {
   swf_ctx *c = new (std::nothrow) swf_ctx();
   if (!c) return SWF_E_NOMEM;
   c->cfg = *cfg;
-  c->mode = cfg->precision;
+  c->mode = kernel_mode(cfg->precision, cfg->bc);
   KCfg &k = c->k;
+  k.fast_div = c->mode != MODE_EXACT;
   k.nx = cfg->nx;
   k.ny = cfg->ny;
   k.P = row_pitch(cfg->nx);
@@ -564,7 +576,7 @@ int swf_sync(swf_ctx *c) {
 int swf_weno_points(const double *a, double *right, double *left, int64_t n, int32_t precision, int32_t device) {
   const int e12 = (precision & SWF_WENO_EPS12) != 0;
   precision &= ~SWF_WENO_EPS12;
-  if (!a || !right || !left || n < 0 || precision < 0 || precision > 2) return SWF_E_INVALID;
+  if (!a || !right || !left || n < 0 || precision < 0 || precision > SWF_FP32_FULL) return SWF_E_INVALID;
   if (n == 0) return SWF_OK;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SWF_E_NODEVICE;
@@ -614,7 +626,7 @@ int swf_weno_points_noalloc(const double *a, double *right, double *left, int64_
                             int32_t device) {
   const int e12 = (precision & SWF_WENO_EPS12) != 0;
   precision &= ~SWF_WENO_EPS12;
-  if (!a || !right || !left || n < 0 || precision < 0 || precision > 2) return SWF_E_INVALID;
+  if (!a || !right || !left || n < 0 || precision < 0 || precision > SWF_FP32_FULL) return SWF_E_INVALID;
   if (n > SWF_WENO_NOALLOC_MAX) {
     snprintf(g_last_error, sizeof(g_last_error), "swf_weno_points_noalloc: %lld stencils exceed the maximum %d",
              (long long)n, SWF_WENO_NOALLOC_MAX);
@@ -654,6 +666,10 @@ struct swf_batch {
   // the same for the step-program instances (their own wenofd variant), and their indices
   int2 *d_xmap_s = nullptr, *d_ymap_s = nullptr;
   int nxb_s = 0, nyb_s = 0;
+  // their per-line alpha tiles and reductions, and their wall images before the x-
+  // and the y-pass: one launch each over every step-program instance
+  int2 *d_lamap = nullptr, *d_lrmap = nullptr, *d_rf0map = nullptr, *d_rf1map = nullptr;
+  int nla = 0, nlr = 0, nrf0 = 0, nrf1 = 0;
   std::vector<int> steps;
   double *d_time = nullptr, *d_tfinal = nullptr;
   BatchStatus *d_status = nullptr;
@@ -708,10 +724,27 @@ int batch_set_maps(swf_batch *bt, int schedule) {
   batch_map(bt, true, schedule, false, ym);
   batch_map(bt, false, schedule, true, xs);
   batch_map(bt, true, schedule, true, ys);
+  // the step-program instances' small launches, instance by instance
+  std::vector<int2> la, lr, r0, r1;
+  for (int i = 0; i < bt->n; ++i) {
+    const KCfg &k = bt->ctx[i]->k;
+    if (k.alpha_lines) {
+      for (int q = 0; q < la_gx(k) * la_gy(k); ++q) la.push_back(make_int2(i, q));
+      for (int q = 0; q < la_reduce_blocks(k); ++q) lr.push_back(make_int2(i, q));
+    }
+    if (k.step_total > 0) {
+      for (int q = 0; q < (step_reflect_cells(k, 0) + 255) / 256; ++q) r0.push_back(make_int2(i, q));
+      for (int q = 0; q < (step_reflect_cells(k, 1) + 255) / 256; ++q) r1.push_back(make_int2(i, q));
+    }
+  }
   int rc = upload_map(xm, &bt->d_xmap, &bt->nxb);
   if (!rc) rc = upload_map(ym, &bt->d_ymap, &bt->nyb);
   if (!rc) rc = upload_map(xs, &bt->d_xmap_s, &bt->nxb_s);
   if (!rc) rc = upload_map(ys, &bt->d_ymap_s, &bt->nyb_s);
+  if (!rc) rc = upload_map(la, &bt->d_lamap, &bt->nla);
+  if (!rc) rc = upload_map(lr, &bt->d_lrmap, &bt->nlr);
+  if (!rc) rc = upload_map(r0, &bt->d_rf0map, &bt->nrf0);
+  if (!rc) rc = upload_map(r1, &bt->d_rf1map, &bt->nrf1);
   if (rc) return rc;
   bt->steps.clear();
   for (int i = 0; i < bt->n; ++i)
@@ -725,33 +758,26 @@ int batch_set_maps(swf_batch *bt, int schedule) {
 }
 
 // the flux launches of one stage over every instance: the step-program instances
-// (per-line alphas, 1d-12) have kernels of their own
+// (per-line alphas, 1d-12) have kernels of their own, in the mode their contexts use
 void batch_flux_launches(swf_batch *bt, bool y, int s, int slot_in, int p, int flags) {
+  const int ms = kernel_mode(bt->mode, SWF_BC_STEP);
   if (y) {
     if (bt->nyb) launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap, bt->nyb, bt->stream, s, slot_in, p, flags, 0);
-    if (bt->nyb_s)
-      launch_ystage_batch(bt->mode, bt->d_inst, bt->d_ymap_s, bt->nyb_s, bt->stream, s, slot_in, p, flags, 1);
+    if (bt->nyb_s) launch_ystage_batch(ms, bt->d_inst, bt->d_ymap_s, bt->nyb_s, bt->stream, s, slot_in, p, flags, 1);
   } else {
     if (bt->nxb) launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap, bt->nxb, bt->stream, s, slot_in, p, 0);
-    if (bt->nxb_s) launch_xflux_batch(bt->mode, bt->d_inst, bt->d_xmap_s, bt->nxb_s, bt->stream, s, slot_in, p, 1);
+    if (bt->nxb_s) launch_xflux_batch(ms, bt->d_inst, bt->d_xmap_s, bt->nxb_s, bt->stream, s, slot_in, p, 1);
   }
 }
 
 // one RK stage of every instance: the flux and stage-end launches over all of
-// them, and for each step-program instance its per-line alphas and wall images
+// them, and over the step-program instances their per-line alphas and wall images
 void batch_enqueue_stage(swf_batch *bt, int s, int p) {
   const int slot_in = s - 1, slot_out = s % 3;
-  for (int i : bt->steps) {
-    const swf_ctx *c = bt->ctx[i];
-    const Bufs b = stage_bufs(c, s);
-    if (c->k.alpha_lines) launch_line_alpha(c->k, b, bt->stream, slot_in, p);
-    if (c->k.step_total > 0) launch_step_reflect(c->k, b, bt->stream, 0, p);
-  }
+  launch_line_alpha_batch(bt->d_inst, bt->d_lamap, bt->nla, bt->d_lrmap, bt->nlr, bt->stream, s, slot_in, p);
+  launch_step_reflect_batch(bt->d_inst, bt->d_rf0map, bt->nrf0, bt->stream, s, 0, p);
   batch_flux_launches(bt, false, s, slot_in, p, 0);
-  for (int i : bt->steps) {
-    const swf_ctx *c = bt->ctx[i];
-    if (c->k.step_total > 0) launch_step_reflect(c->k, stage_bufs(c, s), bt->stream, 1, p);
-  }
+  launch_step_reflect_batch(bt->d_inst, bt->d_rf1map, bt->nrf1, bt->stream, s, 1, p);
   batch_flux_launches(bt, true, s, slot_in, p, 0);
   launch_stage_end_batch(bt->d_inst, bt->n, bt->stream, s, slot_out, p, bt->mode == MODE_EXACT ? 1 : 0);
 }
@@ -781,9 +807,16 @@ extern "C" {
 int swf_batch_create(const swf_config *cfgs, int32_t n, swf_batch **out) {
   if (!cfgs || !out || n < 1) return SWF_E_INVALID;
   *out = nullptr;
-  for (int i = 0; i < n; ++i)
+  for (int i = 0; i < n; ++i) {
     if (cfgs[i].precision != cfgs[0].precision || cfgs[i].device != cfgs[0].device || cfgs[i].use_limiter == 1)
       return SWF_E_INVALID;
+    // the jets run the FP32-bulk form (kernel_mode); the other instances share their kernels
+    if (cfgs[i].precision == SWF_FP32_FULL && kernel_mode(SWF_FP32_FULL, cfgs[i].bc) != SWF_FP32_FULL &&
+        cfgs[i].bc != SWF_BC_STEP) {
+      snprintf(g_last_error, sizeof(g_last_error), "swf_batch_create: SWF_FP32_FULL batches take no jet instances");
+      return SWF_E_INVALID;
+    }
+  }
   swf_batch *bt = new (std::nothrow) swf_batch;
   if (!bt) return SWF_E_NOMEM;
   bt->n = n;
@@ -828,6 +861,8 @@ void swf_batch_destroy(swf_batch *bt) {
   (void)hipFree(bt->d_ymap);
   if (bt->d_xmap_s) (void)hipFree(bt->d_xmap_s);
   if (bt->d_ymap_s) (void)hipFree(bt->d_ymap_s);
+  for (int2 *m : {bt->d_lamap, bt->d_lrmap, bt->d_rf0map, bt->d_rf1map})
+    if (m) (void)hipFree(m);
   (void)hipFree(bt->d_time);
   (void)hipFree(bt->d_status);
   if (bt->stream) (void)hipStreamDestroy(bt->stream);

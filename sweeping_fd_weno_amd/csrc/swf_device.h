@@ -7,9 +7,13 @@
 //           emulated in double-double (swf_dd.h).  Bit-identical to the reference.
 //   M == 1  FP64: the same scheme in binary64 with contraction; reconstructions in
 //           first-difference form with one reciprocal each (swf_weno.h).
-//   M == 2  FP32-bulk: as M == 1, with the smoothness indicators, nonlinear
-//           weights and candidate combinations in FP32; state, split fluxes,
-//           eigen-projections, RK, pressure, positivity tests and the sweep in FP64.
+//   M == 2  FP32-bulk: as M == 1, with the smoothness indicators and nonlinear
+//           weights in FP32; state, split fluxes, eigen-projections, candidate
+//           values, RK, pressure, positivity tests and the sweep in FP64.
+//   M == 3  FP32-full: as M == 2, with the eigen-projections of the flux
+//           differences, the candidate values, the reconstruction and the
+//           back-projection in FP32 too; the Roe averages, the split fluxes, the
+//           centre values, RK, pressure, positivity tests and the sweep in FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,7 +28,7 @@
 
 namespace swf {
 
-enum { MODE_EXACT = 0, MODE_FP64 = 1, MODE_FP32 = 2 };
+enum { MODE_EXACT = 0, MODE_FP64 = 1, MODE_FP32 = 2, MODE_FP32F = 3 };
 enum { BC_SEDOV = 0, BC_PERIODIC = 1, BC_JET_HALF = 2, BC_JET_FULL = 3, BC_STEP = 4 };
 enum { ST_OK = 0, ST_PBAR = 1, ST_SWEEP_FAIL = 2, ST_PSUM = 3 };
 enum { YS_RHS_ONLY = 1, YS_NO_SCALARS = 2 };  // flags of the stage kernel
@@ -43,6 +47,7 @@ struct KCfg {
   double in_rho, in_mx, in_e;                           // its post-shock inflow state
   int alpha_in_scheme, roe_eps_double, ydens_plain, eps2_quad_literal, findalpha_ex3;
   int dt_rule_dx53, poscheck_le, success_ge, pbar_check, fail_stop;
+  int fast_div;  // the sweep's divisions by refined reciprocals (FP64 / FP32 modes)
 };
 
 // the cells the step program's bc sets: the ghost frame (3 full rows below and
@@ -59,6 +64,14 @@ SWF_HD long long step_bc_cells(const KCfg &k) {
 constexpr int LA_W = 64, LA_H = 8;
 SWF_HD int la_gx(const KCfg &k) { return (k.nx + 6 + LA_W - 1) / LA_W; }
 SWF_HD int la_gy(const KCfg &k) { return (k.ny + 6 + LA_H - 1) / LA_H; }
+// their reduction: 8 lanes per row or column line, 256 threads per block
+constexpr int LA_LANES = 8;
+SWF_HD int la_reduce_blocks(const KCfg &k) { return ((k.nx + 6 + k.ny + 6) * LA_LANES + 255) / 256; }
+// the step's wall images (ex6): 3 cells per row 0 .. min(ny+1, 6T) before the
+// x-pass (dir 0), 3 per column 1..T before the y-pass (dir 1)
+SWF_HD int step_reflect_cells(const KCfg &k, int dir) {
+  return dir == 0 ? 3 * ((k.ny + 1 < 6 * k.step_total ? k.ny + 1 : 6 * k.step_total) + 1) : 3 * k.step_total;
+}
 
 // Device-resident scalars of one solver context.
 struct DevScalars {
@@ -316,6 +329,12 @@ constexpr WenoEps kWeps12 = {qconst::e12_hi, qconst::e12_lo, 4e-12, 4e-12f};
 // the fast modes use the first-difference forms of swf_weno.h.
 template <int M>
 SWF_HD double weno_right(const double *a, const WenoEps &we) {
+  if (M == MODE_FP32F) {
+    double e[4];
+    first_differences(a, e);
+    const float f[4] = {(float)e[0], (float)e[1], (float)e[2], (float)e[3]};
+    return a[0] + (double)weno_d32f<true>(f, we.q4, we.q4f);
+  }
   if (M != MODE_EXACT) {
     double e[4];
     first_differences(a, e);
@@ -336,6 +355,12 @@ SWF_HD double weno_right(const double *a, const WenoEps &we) {
 // weno_left: value at x_{i-1/2}; `a` points at average(0) (weno5.f90:86-113)
 template <int M>
 SWF_HD double weno_left(const double *a, const WenoEps &we) {
+  if (M == MODE_FP32F) {
+    double e[4];
+    first_differences(a, e);
+    const float f[4] = {(float)e[0], (float)e[1], (float)e[2], (float)e[3]};
+    return a[0] + (double)weno_d32f<false>(f, we.q4, we.q4f);
+  }
   if (M != MODE_EXACT) {
     double e[4];
     first_differences(a, e);
@@ -429,6 +454,12 @@ SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const
 //     out = f(0) + R * weno_d(L * (first differences of f))
 // which is R * weno(L * f) in exact arithmetic (R = L^-1, L linear).  Only the
 // correction passes through the eigenvectors and the nonlinear weights.
+// The acoustic rows 1 and 4 of L share A = (b2 d0 - tn dn - tt dt + b1 d3) / 2 and
+// differ by B = (vn d0 - dn) / (2 c): row 1 = A + B, row 4 = A - B, and the entropy
+// row 3 is d0 - 2 A (10 operations per projected difference instead of 13).
+// FP32-full (M == 3): the projections, the weights and candidates, and the
+// back-projection in FP32; the Roe average, the centre value and the R * L energy
+// term below in FP64.
 template <int M, int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
 SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   const double g1 = k.gamma1;
@@ -439,44 +470,71 @@ SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const 
   const double vym = t0 * pa.vy + t1 * pb.vy;
   const double hm = t0 * pa.h + t1 * pb.h;
   const double qm = 0.5 * (vxm * vxm + vym * vym);
-  const double cm = sqrt64(fabs(g1 * (hm - qm))) + 1e-15;
+  const double hq = hm - qm;
+  const double cm = sqrt64(fabs(g1 * hq)) + 1e-15;
   const double vn = (DIR == 0) ? vxm : vym;
   const double vt = (DIR == 0) ? vym : vxm;
   const double rcm = rcp64(cm);
-  const double b1 = (g1 * rcm) * rcm, b2 = qm * b1, tr = vn * rcm, tn = b1 * vn, t2 = 0.5 * b1, tt = b1 * vt;
-  // left eigenvector rows 1, 3, 4 in (rho, m_n, m_t, E) order; row 2 is (-vt, 0, 1, 0)
-  const double l00 = 0.5 * (b2 + tr), l01 = -(0.5 * (tn + rcm)), l02 = -(0.5 * tt);
-  const double l20 = 1. - b2;
-  const double l30 = 0.5 * (b2 - tr), l31 = -(0.5 * (tn - rcm));
+  const double b1 = (g1 * rcm) * rcm, b2 = qm * b1, tn = b1 * vn, tt = b1 * vt;
   const int cn = (DIR == 0) ? 1 : 2, ct = (DIR == 0) ? 2 : 1;
-  double e[4][4];  // e[field][difference]
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const double *f0 = st + (q - 2) * SS, *f1 = st + (q - 1) * SS;
-    const double d0 = f1[0] - f0[0], dn = f1[cn * CS] - f0[cn * CS], dt = f1[ct * CS] - f0[ct * CS],
-                 d3 = f1[3 * CS] - f0[3 * CS];
-    e[0][q] = ((l00 * d0 + l01 * dn) + l02 * dt) + t2 * d3;
-    e[1][q] = dt - vt * d0;
-    e[2][q] = ((l20 * d0 + tn * dn) + tt * dt) - b1 * d3;
-    e[3][q] = ((l30 * d0 + l31 * dn) + l02 * dt) + t2 * d3;
-  }
-  double r[4];
-#pragma unroll
-  for (int m = 0; m < 4; ++m)
-    r[m] = (M == MODE_FP64) ? weno_d64<RIGHT>(e[m], we.q4) : weno_d32_or_64<RIGHT>(e[m], we.q4, we.q4f);
-  // R * r with the right eigenvector columns (1, vn-c, vt, H-vn c), (0, 0, 1, vt),
-  // (1, vn, vt, q), (1, vn+c, vt, H+vn c)
-  const double s = (r[0] + r[2]) + r[3], dr = r[3] - r[0];
   double corr[4];
-  corr[0] = s;
-  corr[cn] = vn * s + cm * dr;
-  corr[ct] = vt * s + r[1];
-  corr[3] = (hm * (r[0] + r[3]) + (vn * cm) * dr) + (vt * r[1] + qm * r[2]);
+  if constexpr (M == MODE_FP32F) {
+    const float hb2 = (float)(0.5 * b2), htn = (float)(0.5 * tn), htt = (float)(0.5 * tt), hb1 = (float)(0.5 * b1);
+    const float hr = (float)(0.5 * rcm), fvn = (float)vn, fvt = (float)vt;
+    float e[4][4];  // e[field][difference]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double *f0 = st + (q - 2) * SS, *f1 = st + (q - 1) * SS;
+      const float d0 = (float)(f1[0] - f0[0]), dn = (float)(f1[cn * CS] - f0[cn * CS]),
+                  dt = (float)(f1[ct * CS] - f0[ct * CS]), d3 = (float)(f1[3 * CS] - f0[3 * CS]);
+      const float A = ((hb2 * d0 - htn * dn) - htt * dt) + hb1 * d3;
+      const float B = hr * (fvn * d0 - dn);
+      e[0][q] = A + B;
+      e[1][q] = dt - fvt * d0;
+      e[2][q] = d0 - 2.0f * A;
+      e[3][q] = A - B;
+    }
+    float r[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r[m] = weno_d32f<RIGHT>(e[m], we.q4, we.q4f);
+    const float fcm = (float)cm, fhm = (float)hm, fqm = (float)qm;
+    const float s = (r[0] + r[2]) + r[3], dr = r[3] - r[0];
+    corr[0] = s;
+    corr[cn] = fvn * s + fcm * dr;
+    corr[ct] = fvt * s + r[1];
+    corr[3] = (fhm * (r[0] + r[3]) + (fvn * fcm) * dr) + (fvt * r[1] + fqm * r[2]);
+  } else {
+    const double hb2 = 0.5 * b2, htn = 0.5 * tn, htt = 0.5 * tt, hb1 = 0.5 * b1, hr = 0.5 * rcm;
+    double e[4][4];  // e[field][difference]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double *f0 = st + (q - 2) * SS, *f1 = st + (q - 1) * SS;
+      const double d0 = f1[0] - f0[0], dn = f1[cn * CS] - f0[cn * CS], dt = f1[ct * CS] - f0[ct * CS],
+                   d3 = f1[3 * CS] - f0[3 * CS];
+      const double A = ((hb2 * d0 - htn * dn) - htt * dt) + hb1 * d3;
+      const double B = hr * (vn * d0 - dn);
+      e[0][q] = A + B;
+      e[1][q] = dt - vt * d0;
+      e[2][q] = d0 - 2.0 * A;
+      e[3][q] = A - B;
+    }
+    double r[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      r[m] = (M == MODE_FP64) ? weno_d64<RIGHT>(e[m], we.q4) : weno_d32_or_64<RIGHT>(e[m], we.q4, we.q4f);
+    // R * r with the right eigenvector columns (1, vn-c, vt, H-vn c), (0, 0, 1, vt),
+    // (1, vn, vt, q), (1, vn+c, vt, H+vn c)
+    const double s = (r[0] + r[2]) + r[3], dr = r[3] - r[0];
+    corr[0] = s;
+    corr[cn] = vn * s + cm * dr;
+    corr[ct] = vt * s + r[1];
+    corr[3] = (hm * (r[0] + r[3]) + (vn * cm) * dr) + (vt * r[1] + qm * r[2]);
+  }
   // The reference's R * L is not the identity: cm carries +1e-15, so
   // H != cm^2/gamma1 + q, and R * L = I + dH * e_E * (L row 1 + L row 4) with
   // dH = (hm - qm) - cm^2/gamma1.  Its centre value R * (L * f(0)) is therefore
   // f(0) plus this energy term (relative size ~1e-15/cm: 1e-12 where cm ~ 1e-3).
-  const double dH = (hm - qm) - (cm * cm) * k.ig1;
+  const double dH = hq - (cm * cm) * k.ig1;
   corr[3] += dH * (((b2 * st[0] - tn * st[cn * CS]) - tt * st[ct * CS]) + b1 * st[3 * CS]);
 #pragma unroll
   for (int c = 0; c < 4; ++c) out[c] = st[c * CS] + corr[c];
@@ -588,6 +646,19 @@ SWF_HD bool sweep_flag(double g1, double eps, double v0, double v1, double v2, d
 #pragma clang fp contract(off)
   double p = sweep_pres(g1, v0, v1, v2, v3);
   return (p <= eps) || (v0 <= eps);
+}
+// The same for a context: in the fast modes (k.fast_div) the division by rho is a
+// refined reciprocal (within 1 ulp), here and wherever the sweep forms a pressure,
+// so that every flag test of a run uses one function.
+SWF_HD double sweep_pres(const KCfg &k, double v0, double v1, double v2, double v3) {
+#pragma clang fp contract(off)
+  if (k.fast_div) return k.gamma1 * (v3 - (0.5 * (v1 * v1 + v2 * v2)) * rcp64(v0));
+  return sweep_pres(k.gamma1, v0, v1, v2, v3);
+}
+SWF_HD bool sweep_flag(const KCfg &k, double v0, double v1, double v2, double v3) {
+#pragma clang fp contract(off)
+  const double p = sweep_pres(k, v0, v1, v2, v3);
+  return (p <= k.eps) || (v0 <= k.eps);
 }
 
 // wave-wide max of a non-negative double, then one atomic per wave

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_flags(KCfg k, Bufs b) {
   const bool bad = k.poscheck_le ? (v[0] <= k.eps || ke <= k.eps) : (v[0] < k.eps || ke < k.eps);
   if (bad) atomicOr(&b.g->postag, 1);
   const int p = posI(k, i, j);
-  if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) atomicOr(&b.bmP[p >> 5], 1u << (p & 31));
+  if (sweep_flag(k, v[0], v[1], v[2], v[3])) atomicOr(&b.bmP[p >> 5], 1u << (p & 31));
   if (v[0] < k.delta) {
     atomicOr(&b.bmR[p >> 5], 1u << (p & 31));
     atomicOr(&b.g->rho_any, 1);
@@ -198,12 +198,20 @@ void launch_bc(const KCfg &k, const Bufs &b, hipStream_t s, int slot_out, int pa
 }
 void launch_line_alpha(const KCfg &k, const Bufs &b, hipStream_t s, int slot, int parity) {
   hipLaunchKernelGGL(k_line_alpha, dim3(la_gx(k), la_gy(k)), dim3(256), 0, s, k, b, slot, parity);
-  const int nthreads = (k.nx + 6 + k.ny + 6) * LA_LANES;
-  hipLaunchKernelGGL(k_line_alpha_reduce, dim3((nthreads + 255) / 256), dim3(256), 0, s, k, b, slot, parity);
+  hipLaunchKernelGGL(k_line_alpha_reduce, dim3(la_reduce_blocks(k)), dim3(256), 0, s, k, b, slot, parity);
 }
 void launch_step_reflect(const KCfg &k, const Bufs &b, hipStream_t s, int dir, int parity) {
-  const int n = dir == 0 ? 3 * (std::min(k.ny + 1, 6 * k.step_total) + 1) : 3 * k.step_total;
+  const int n = step_reflect_cells(k, dir);
   hipLaunchKernelGGL(k_step_reflect, dim3((n + 255) / 256), dim3(256), 0, s, k, b, dir, parity);
+}
+void launch_line_alpha_batch(const BatchInst *inst, const int2 *lmap, int nl, const int2 *rmap, int nr,
+                             hipStream_t s, int stage, int slot, int parity) {
+  if (nl) hipLaunchKernelGGL(k_line_alpha_batch, dim3(nl), dim3(256), 0, s, inst, lmap, stage, slot, parity);
+  if (nr) hipLaunchKernelGGL(k_line_alpha_reduce_batch, dim3(nr), dim3(256), 0, s, inst, rmap, stage, slot, parity);
+}
+void launch_step_reflect_batch(const BatchInst *inst, const int2 *map, int n, hipStream_t s, int stage, int dir,
+                               int parity) {
+  if (n) hipLaunchKernelGGL(k_step_reflect_batch, dim3(n), dim3(256), 0, s, inst, map, stage, dir, parity);
 }
 void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s) {
   hipLaunchKernelGGL(k_from_fortran, dim3(1024), dim3(256), 0, s, k, src, dst);

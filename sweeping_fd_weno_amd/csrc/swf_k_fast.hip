@@ -1,5 +1,7 @@
-// swf_k_fast.hip - stage kernels of the FP64 and FP32-bulk modes (compiled with
-// floating-point contraction enabled).
+// swf_k_fast.hip - stage kernels of the FP64, FP32-bulk and FP32-full modes
+// (compiled with floating-point contraction enabled).
+#include <type_traits>
+
 #include "swf_kernels.h"
 #include "swf_launch.h"
 
@@ -42,26 +44,37 @@ static void ystage(const KCfg &k, const Bufs &b, hipStream_t s, int stage, int s
     hipLaunchKernelGGL((k_ystage<M, LIM>), grid_ystage(k), dim3(YS_T), 0, s, k, b, stage, slot_in, parity, flags);
 }
 
+// f<M>() for the fast mode `mode` (1, 2 or 3)
+template <class F>
+static void by_mode(int mode, F f) {
+  if (mode == MODE_FP64)
+    f(std::integral_constant<int, MODE_FP64>());
+  else if (mode == MODE_FP32)
+    f(std::integral_constant<int, MODE_FP32>());
+  else
+    f(std::integral_constant<int, MODE_FP32F>());
+}
+
 void launch_xflux_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity) {
   const bool lim = k.use_limiter == 1;
-  if (mode == MODE_FP64)
-    (lim ? xflux<MODE_FP64, true> : xflux<MODE_FP64, false>)(k, b, s, stage, slot_in, parity);
-  else
-    (lim ? xflux<MODE_FP32, true> : xflux<MODE_FP32, false>)(k, b, s, stage, slot_in, parity);
+  by_mode(mode, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    (lim ? xflux<M, true> : xflux<M, false>)(k, b, s, stage, slot_in, parity);
+  });
 }
 void launch_ystage_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int stage, int slot_in, int parity,
                        int flags) {
   const bool lim = k.use_limiter == 1;
-  if (mode == MODE_FP64)
-    (lim ? ystage<MODE_FP64, true> : ystage<MODE_FP64, false>)(k, b, s, stage, slot_in, parity, flags);
-  else
-    (lim ? ystage<MODE_FP32, true> : ystage<MODE_FP32, false>)(k, b, s, stage, slot_in, parity, flags);
+  by_mode(mode, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    (lim ? ystage<M, true> : ystage<M, false>)(k, b, s, stage, slot_in, parity, flags);
+  });
 }
 void launch_zs_edges_m12(int mode, const KCfg &k, const Bufs &b, hipStream_t s, int slot_in, int parity) {
-  if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_zs_edges<MODE_FP64>, dim3(grid_zs_edges(k)), dim3(256), 0, s, k, b, slot_in, parity);
-  else
-    hipLaunchKernelGGL(k_zs_edges<MODE_FP32>, dim3(grid_zs_edges(k)), dim3(256), 0, s, k, b, slot_in, parity);
+  by_mode(mode, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    hipLaunchKernelGGL(k_zs_edges<M>, dim3(grid_zs_edges(k)), dim3(256), 0, s, k, b, slot_in, parity);
+  });
 }
 template <int M, bool STEP>
 static void xflux_batch(const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage, int slot_in,
@@ -76,28 +89,24 @@ static void ystage_batch(const BatchInst *inst, const int2 *map, int nblocks, hi
 }
 void launch_xflux_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
                             int slot_in, int parity, int step) {
-  if (mode == MODE_FP64)
-    (step ? xflux_batch<MODE_FP64, true> : xflux_batch<MODE_FP64, false>)(inst, map, nblocks, s, stage, slot_in,
-                                                                          parity);
-  else
-    (step ? xflux_batch<MODE_FP32, true> : xflux_batch<MODE_FP32, false>)(inst, map, nblocks, s, stage, slot_in,
-                                                                          parity);
+  by_mode(mode, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    (step ? xflux_batch<M, true> : xflux_batch<M, false>)(inst, map, nblocks, s, stage, slot_in, parity);
+  });
 }
 void launch_ystage_batch_m12(int mode, const BatchInst *inst, const int2 *map, int nblocks, hipStream_t s, int stage,
                              int slot_in, int parity, int flags, int step) {
-  if (mode == MODE_FP64)
-    (step ? ystage_batch<MODE_FP64, true> : ystage_batch<MODE_FP64, false>)(inst, map, nblocks, s, stage, slot_in,
-                                                                            parity, flags);
-  else
-    (step ? ystage_batch<MODE_FP32, true> : ystage_batch<MODE_FP32, false>)(inst, map, nblocks, s, stage, slot_in,
-                                                                            parity, flags);
+  by_mode(mode, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    (step ? ystage_batch<M, true> : ystage_batch<M, false>)(inst, map, nblocks, s, stage, slot_in, parity, flags);
+  });
 }
 void launch_weno_points_m12(int mode, const double *a, double *r, double *l, long long n, int e12, hipStream_t s) {
   const int blocks = (int)((n + 255) / 256);
-  if (mode == MODE_FP64)
-    hipLaunchKernelGGL(k_weno_points<MODE_FP64>, dim3(blocks), dim3(256), 0, s, a, r, l, n, e12);
-  else
-    hipLaunchKernelGGL(k_weno_points<MODE_FP32>, dim3(blocks), dim3(256), 0, s, a, r, l, n, e12);
+  by_mode(mode, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    hipLaunchKernelGGL(k_weno_points<M>, dim3(blocks), dim3(256), 0, s, a, r, l, n, e12);
+  });
 }
 
 }  // namespace swf

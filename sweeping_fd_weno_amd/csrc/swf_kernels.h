@@ -581,7 +581,7 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
       if (bad) atomicOr(&g->postag, 1);
       if (k.use_limiter == 2) {
         const int p = posI(k, i, j);
-        if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) atomicOr(&b.bmP[p >> 5], 1u << (p & 31));
+        if (sweep_flag(k, v[0], v[1], v[2], v[3])) atomicOr(&b.bmP[p >> 5], 1u << (p & 31));
         if (v[0] < k.delta) {
           atomicOr(&b.bmR[p >> 5], 1u << (p & 31));
           atomicOr(&g->rho_any, 1);
@@ -1015,18 +1015,18 @@ __global__ __launch_bounds__(256) void k_bc(KCfg k, Bufs b, int slot_out, int pa
 
 // Per-line alphas (ex6_shock_diffraction/main.f90:217-219, :449-451): findalpha's
 // alpha(i,j) of the stage input over every cell, the max of each row and each
-// column of a 64 x 8 tile into lpart (rows first: la_gx x (ny+6), then columns:
-// la_gy x (nx+6)), then k_line_alpha_reduce.
-__global__ __launch_bounds__(256) void k_line_alpha(KCfg k, Bufs b, int slot, int parity) {
-  __shared__ double cred[4][LA_W];
+// column of a 64 x 8 tile (bx, by) into lpart (rows first: la_gx x (ny+6), then
+// columns: la_gy x (nx+6)), then line_alpha_reduce_block.
+SWF_D void line_alpha_block(const KCfg &k, const Bufs &b, int slot, int parity, int bx, int by,
+                            double (&cred)[4][LA_W]) {
   DevScalars *g = b.g;
   if (parity >= 0 && (!g->active[parity] || g->status != ST_OK)) return;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) g->alpha_bits[slot] = 0ull;  // +0.0
+  if (bx == 0 && by == 0 && threadIdx.x == 0) g->alpha_bits[slot] = 0ull;  // +0.0
   const int tx = threadIdx.x & (LA_W - 1), ty = threadIdx.x / LA_W;
-  const int i = -2 + blockIdx.x * LA_W + tx;
+  const int i = -2 + bx * LA_W + tx;
   double colmax = 0.0;
   for (int r = ty; r < LA_H; r += 4) {
-    const int j = -2 + blockIdx.y * LA_H + r;
+    const int j = -2 + by * LA_H + r;
     double a = 0.0;
     if (i <= k.nx + 3 && j <= k.ny + 3) {
       const long long o = cidx(k, i, j);
@@ -1038,25 +1038,27 @@ __global__ __launch_bounds__(256) void k_line_alpha(KCfg k, Bufs b, int slot, in
       const double o = __shfl_xor(a, off, 64);
       a = (o > a) ? o : a;
     }
-    if (tx == 0 && j <= k.ny + 3) b.lpart[(long long)blockIdx.x * (k.ny + 6) + (j + 2)] = a;
+    if (tx == 0 && j <= k.ny + 3) b.lpart[(long long)bx * (k.ny + 6) + (j + 2)] = a;
   }
   cred[ty][tx] = colmax;
   __syncthreads();
   if (ty == 0 && i <= k.nx + 3) {
     const double m = fmax2(fmax2(cred[0][tx], cred[1][tx]), fmax2(cred[2][tx], cred[3][tx]));
-    b.lpart[(long long)la_gx(k) * (k.ny + 6) + (long long)blockIdx.y * (k.nx + 6) + (i + 2)] = m;
+    b.lpart[(long long)la_gx(k) * (k.ny + 6) + (long long)by * (k.nx + 6) + (i + 2)] = m;
   }
+}
+__global__ __launch_bounds__(256) void k_line_alpha(KCfg k, Bufs b, int slot, int parity) {
+  __shared__ double cred[4][LA_W];
+  line_alpha_block(k, b, slot, parity, blockIdx.x, blockIdx.y, cred);
 }
 
 // arow, acol and alphamax from the tile partials: 8 lanes per line (rows first,
-// then columns), each block's max into alpha_bits[slot] (cleared by k_line_alpha)
-constexpr int LA_LANES = 8;
-__global__ __launch_bounds__(256) void k_line_alpha_reduce(KCfg k, Bufs b, int slot, int parity) {
-  __shared__ double red[4];
+// then columns), each block's max into alpha_bits[slot] (cleared by line_alpha_block)
+SWF_D void line_alpha_reduce_block(const KCfg &k, const Bufs &b, int slot, int parity, int blk, double (&red)[4]) {
   DevScalars *g = b.g;
   if (parity >= 0 && (!g->active[parity] || g->status != ST_OK)) return;
   const int nyg = k.ny + 6, nxg = k.nx + 6, gxa = la_gx(k), gya = la_gy(k);
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int t = blk * 256 + threadIdx.x;
   const int line = t / LA_LANES, r = t % LA_LANES;
   double m = 0.0;
   if (line < nyg) {
@@ -1089,27 +1091,29 @@ __global__ __launch_bounds__(256) void k_line_alpha_reduce(KCfg k, Bufs b, int s
     atomicMax(&g->alpha_bits[slot], (unsigned long long)__double_as_longlong(bm));  // alpha >= 0: bits order
   }
 }
+__global__ __launch_bounds__(256) void k_line_alpha_reduce(KCfg k, Bufs b, int slot, int parity) {
+  __shared__ double red[4];
+  line_alpha_reduce_block(k, b, slot, parity, blockIdx.x, red);
+}
 
 // The step's walls inside wenofd (ex6_shock_diffraction/main.f90:188-197, :417-431),
 // on the stage input: dir 0, before the x-pass, the cells (T-2..T, j) mirror
 // (T+3..T+1, j) with m1 negated in rows 0 .. min(ny+1, 6T); dir 1, before the
 // y-pass, the cells (i, 6T-2..6T) mirror (i, 6T+3..6T+1) with m2 negated in
 // columns 1..T.  Sources and targets are disjoint.
-__global__ __launch_bounds__(256) void k_step_reflect(KCfg k, Bufs b, int dir, int parity) {
+SWF_D void step_reflect_block(const KCfg &k, const Bufs &b, int dir, int parity, int blk) {
   const DevScalars *g = b.g;
   if (parity >= 0 && (!g->active[parity] || g->status != ST_OK)) return;
   const int T = k.step_total;
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int t = blk * 256 + threadIdx.x;
+  if (t >= step_reflect_cells(k, dir)) return;
   int di, dj, si, sj;
   if (dir == 0) {
-    const int nr = min(k.ny + 1, 6 * T) + 1;  // rows 0 .. min(ny+1, 6T)
-    if (t >= 3 * nr) return;
-    const int r = t / 3, q = t - 3 * r;
+    const int r = t / 3, q = t - 3 * r;  // rows 0 .. min(ny+1, 6T)
     dj = sj = r;
     di = T - q;
     si = T + 1 + q;
   } else {
-    if (t >= 3 * T) return;
     const int r = t / T, c = t - r * T;
     di = si = 1 + c;
     dj = 6 * T - r;
@@ -1121,6 +1125,33 @@ __global__ __launch_bounds__(256) void k_step_reflect(KCfg k, Bufs b, int dir, i
     const double v = b.Uin[c * k.plane + so];
     b.Uin[c * k.plane + d] = (c == 1 + dir) ? -v : v;
   }
+}
+__global__ __launch_bounds__(256) void k_step_reflect(KCfg k, Bufs b, int dir, int parity) {
+  step_reflect_block(k, b, dir, parity, blockIdx.x);
+}
+
+// The same three for all step-program instances of a batch in one launch each:
+// map[block] = (instance, block of that instance's launch)
+__global__ __launch_bounds__(256) void k_line_alpha_batch(const BatchInst *inst, const int2 *map, int stage, int slot,
+                                                          int parity) {
+  __shared__ double cred[4][LA_W];
+  const int2 e = map[blockIdx.x];
+  const BatchInst &I = inst[e.x];
+  const int gx = la_gx(I.k);
+  line_alpha_block(I.k, batch_stage_bufs(I, stage), slot, parity, e.y % gx, e.y / gx, cred);
+}
+__global__ __launch_bounds__(256) void k_line_alpha_reduce_batch(const BatchInst *inst, const int2 *map, int stage,
+                                                                 int slot, int parity) {
+  __shared__ double red[4];
+  const int2 e = map[blockIdx.x];
+  const BatchInst &I = inst[e.x];
+  line_alpha_reduce_block(I.k, batch_stage_bufs(I, stage), slot, parity, e.y, red);
+}
+__global__ __launch_bounds__(256) void k_step_reflect_batch(const BatchInst *inst, const int2 *map, int stage, int dir,
+                                                            int parity) {
+  const int2 e = map[blockIdx.x];
+  const BatchInst &I = inst[e.x];
+  step_reflect_block(I.k, batch_stage_bufs(I, stage), dir, parity, e.y);
 }
 
 // Fortran-layout u(4,-2:nx+3,-2:ny+3) <-> SoA planes

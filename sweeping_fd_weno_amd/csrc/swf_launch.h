@@ -57,6 +57,12 @@ void launch_upload_reset(const Bufs &b, hipStream_t s);
 // the step program: per-line alphas of the stage input (two kernels) and the walls
 void launch_line_alpha(const KCfg &k, const Bufs &b, hipStream_t s, int slot, int parity);
 void launch_step_reflect(const KCfg &k, const Bufs &b, hipStream_t s, int dir, int parity);
+// the same over all step-program instances of a batch, one launch each (block maps
+// of (instance, block))
+void launch_line_alpha_batch(const BatchInst *inst, const int2 *lmap, int nl, const int2 *rmap, int nr,
+                             hipStream_t s, int stage, int slot, int parity);
+void launch_step_reflect_batch(const BatchInst *inst, const int2 *map, int n, hipStream_t s, int stage, int dir,
+                               int parity);
 void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 

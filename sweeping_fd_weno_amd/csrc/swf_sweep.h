@@ -90,6 +90,7 @@ struct SweepShared {
   int wscan[SW_WCH];
   unsigned dirty[SW_DIRTY_WORDS];
   unsigned char genuine[NT];
+  unsigned char psum_bad[NT];  // the slot's speculative walk failed ex6's pressure-sum check
   int nlist, last_end, flag, all_dirty, nsteps;
   long long tpos, tneg;
   double ubar[4];
@@ -120,23 +121,32 @@ SWF_D void store_ij(const KCfg &k, const Bufs &b, int i, int j, const double v[4
 // p1, p2, pb: the pressures of v1, v2 and vb, each g1*(E - (0.5*(m1^2 + m2^2))/rho)
 // as sweep_pres forms them (the reference's expression), passed in so that a walk
 // forms each cell's pressure once.
+// fast: the divisions by refined reciprocals and the square roots by sqrt64 (the
+// fast modes' arithmetic; each within 1 ulp), else IEEE operations as the reference.
 SWF_D bool nod_lim(double eps, double p1, double p2, double pb, const double v1[4], const double v2[4],
-                   const double vb[4], double out[4]) {
+                   const double vb[4], double out[4], bool fast = false) {
 #pragma clang fp contract(off)
   double t;
   bool pos;
   if (p2 > eps) {
-    t = (p1 - eps) / (p1 - p2);
+    t = fast ? (p1 - eps) * rcp64(p1 - p2) : (p1 - eps) / (p1 - p2);
     pos = true;
   } else {
-    double t_1 = (p1 - eps) / (p1 - pb);
-    double d12 = sqrt((((v2[3] - v1[3]) * (v2[3] - v1[3]) + (v2[2] - v1[2]) * (v2[2] - v1[2])) +
-                       (v2[1] - v1[1]) * (v2[1] - v1[1])) +
-                      (v2[0] - v1[0]) * (v2[0] - v1[0]));
-    double d1b = sqrt((((vb[3] - v1[3]) * (vb[3] - v1[3]) + (vb[2] - v1[2]) * (vb[2] - v1[2])) +
-                       (vb[1] - v1[1]) * (vb[1] - v1[1])) +
-                      (vb[0] - v1[0]) * (vb[0] - v1[0]));
-    t = fmin2((t_1 * d1b) / d12, 0.25);
+    const double s12 = (((v2[3] - v1[3]) * (v2[3] - v1[3]) + (v2[2] - v1[2]) * (v2[2] - v1[2])) +
+                        (v2[1] - v1[1]) * (v2[1] - v1[1])) +
+                       (v2[0] - v1[0]) * (v2[0] - v1[0]);
+    const double s1b = (((vb[3] - v1[3]) * (vb[3] - v1[3]) + (vb[2] - v1[2]) * (vb[2] - v1[2])) +
+                        (vb[1] - v1[1]) * (vb[1] - v1[1])) +
+                       (vb[0] - v1[0]) * (vb[0] - v1[0]);
+    if (fast) {
+      const double t_1 = (p1 - eps) * rcp64(p1 - pb);
+      t = fmin2((t_1 * sqrt64(s1b)) * rcp64(sqrt64(s12)), 0.25);
+    } else {
+      double t_1 = (p1 - eps) / (p1 - pb);
+      double d12 = sqrt(s12);
+      double d1b = sqrt(s1b);
+      t = fmin2((t_1 * d1b) / d12, 0.25);
+    }
     pos = false;
   }
 #pragma unroll
@@ -151,9 +161,11 @@ SWF_D bool nod_lim(double eps, double p1, double p2, double pb, const double v1[
 // The cells of the path are loaded PF = 2 positions ahead (clamped at the last
 // position); the values used are those of cells the walk reaches, which no other
 // walk writes.  Each cell's pressure is formed once (sweep_pres = nod_lim's form).
+// psum_hit (speculative walks): set when a step fails the step program's pressure-sum
+// check; the failure counts only if the walk turns out to be genuine.
 template <bool FWD, bool WRITE, int NT>
 SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], double pb, unsigned *next_map,
-                long long *npos, long long *nneg, SweepShared<NT> *sh, int slot) {
+                long long *npos, long long *nneg, SweepShared<NT> *sh, int slot, bool *psum_hit = nullptr) {
 #pragma clang fp contract(off)
   static_assert(NT <= SW_T, "walk records hold SW_T slots");
   const int N = k.nx * k.ny;
@@ -189,7 +201,7 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
       load_ij(k, b, ci, cj, *slots[d]);
     }
   }
-  double pv = sweep_pres(g1, v[0], v[1], v[2], v[3]);
+  double pv = sweep_pres(k, v[0], v[1], v[2], v[3]);
   unsigned posw = 0u;  // branch bits of the current 32 steps
   int s = 0;
   // one step with the neighbour in `slot_v`, which then receives cell nb+PF;
@@ -208,20 +220,24 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
       snake_ij(pre, order, ci, cj);
       load_ij(k, b, ci, cj, slot_v);
     }
-    const double pn = sweep_pres(g1, vn[0], vn[1], vn[2], vn[3]);
-    const bool pos = nod_lim(eps, pv, pn, pb, v, vn, ub, vt);
+    const double pn = sweep_pres(k, vn[0], vn[1], vn[2], vn[3]);
+    const bool pos = nod_lim(eps, pv, pn, pb, v, vn, ub, vt, k.fast_div != 0);
     double vn2[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) vn2[c] = vn[c] + wr * (v[c] - vt[c]);
     // the step program's check of an order-I adjustment (ex6_shock_diffraction/
     // main.f90:1418-1426, :1477-1485): total pressure of the pair must not rise
     if (order == 0 && k.psum_check &&
-        pv + pn > (sweep_pres(g1, vt[0], vt[1], vt[2], vt[3]) + sweep_pres(g1, vn2[0], vn2[1], vn2[2], vn2[3])) + eps)
-      b.g->status = ST_PSUM;
+        pv + pn > (sweep_pres(k, vt[0], vt[1], vt[2], vt[3]) + sweep_pres(k, vn2[0], vn2[1], vn2[2], vn2[3])) + eps) {
+      if (WRITE)
+        b.g->status = ST_PSUM;
+      else
+        *psum_hit = true;
+    }
     if (WRITE) {
       store_ij(k, b, i, j, vt);
       mark_dirty(k, *sh, i, j);
-      if (sweep_flag(g1, eps, vt[0], vt[1], vt[2], vt[3])) set_bit(next_map, p);
+      if (sweep_flag(k, vt[0], vt[1], vt[2], vt[3])) set_bit(next_map, p);
       if (pos)
         ++*npos;
       else
@@ -242,7 +258,7 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) v[c] = vn2[c];
-    pv = sweep_pres(g1, v[0], v[1], v[2], v[3]);
+    pv = sweep_pres(k, v[0], v[1], v[2], v[3]);
     const bool edge = FWD ? (p == N - 1) : (p == 0);
     if (edge || !((pv <= eps) || (v[0] <= eps))) {  // stop at the first cell not flagged
       if (WRITE) {
@@ -282,7 +298,7 @@ SWF_D void replay_step(const KCfg &k, const Bufs &b, int order, int f, int st, i
   store_ij(k, b, i, j, v);
   mark_dirty(k, *sh, i, j);
   if (st == adj) return;
-  if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) set_bit(next_map, p);
+  if (sweep_flag(k, v[0], v[1], v[2], v[3])) set_bit(next_map, p);
   if ((pw >> (st & 31)) & 1u)
     ++*npos;
   else
@@ -308,7 +324,7 @@ SWF_D int swalk(const KCfg &k, const Bufs &b, int f, int dir, SweepShared<NT> *s
       const long long o = cidx(k, i, j);
       b.U[o] = delta;
       mark_dirty(k, *sh, i, j);
-      if (sweep_flag(k.gamma1, k.eps, delta, b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]))
+      if (sweep_flag(k, delta, b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]))
         set_bit(b.bmP, p);
       else
         clear_bit(b.bmP, p);
@@ -323,7 +339,7 @@ SWF_D int swalk(const KCfg &k, const Bufs &b, int f, int dir, SweepShared<NT> *s
         const long long o = cidx(k, i, j);
         b.U[o] = v;
         mark_dirty(k, *sh, i, j);
-        if (sweep_flag(k.gamma1, k.eps, v, b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]))
+        if (sweep_flag(k, v, b.U[k.plane + o], b.U[2 * k.plane + o], b.U[3 * k.plane + o]))
           set_bit(b.bmP, p);
         else
           clear_bit(b.bmP, p);
@@ -444,7 +460,10 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
     if (idx < n) f = FWD ? sh.list[idx] : sh.list[n - 1 - idx];
     const int le = sh.last_end;
     const bool cand = (f >= 0) && (FWD ? (f > le && f <= N - 2) : (f < le && f >= 1));
-    sh.ends[tid] = cand ? pwalk<FWD, false, NT>(k, b, order, f, ub, pb, nullptr, nullptr, nullptr, nullptr, tid) : -1;
+    bool hit = false;
+    sh.ends[tid] =
+        cand ? pwalk<FWD, false, NT>(k, b, order, f, ub, pb, nullptr, nullptr, nullptr, nullptr, tid, &hit) : -1;
+    sh.psum_bad[tid] = hit ? 1 : 0;
     sh.genuine[tid] = 0;
     __syncthreads();
     if (tid == 0) {
@@ -465,6 +484,7 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
     // steps are independent), longer ones are walked again by their thread
     long long np = 0, nn = 0;
     int len = 0;  // cells written: the adjustments and the stop position
+    if (sh.genuine[tid] && sh.psum_bad[tid]) b.g->status = ST_PSUM;
     if (sh.genuine[tid]) {
       const int end = sh.ends[tid];
       const int adj = FWD ? end - f : f - end;
@@ -525,7 +545,7 @@ SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned
     cell_of(k, order, pe, i, j);
     double v[4];
     load_ij(k, b, i, j, v);
-    if (sweep_flag(k.gamma1, k.eps, v[0], v[1], v[2], v[3])) set_bit(next, pe);
+    if (sweep_flag(k, v[0], v[1], v[2], v[3])) set_bit(next, pe);
   }
   __syncthreads();
 }
@@ -569,7 +589,7 @@ SWF_D bool converged_at(const KCfg &k, const Bufs &b, int order, int p) {
   cell_of(k, order, p, i, j);
   double v[4];
   load_ij(k, b, i, j, v);
-  const double pr = sweep_pres(k.gamma1, v[0], v[1], v[2], v[3]);
+  const double pr = sweep_pres(k, v[0], v[1], v[2], v[3]);
   return !((pr < k.eps) || (v[0] < k.eps));
 }
 
@@ -690,7 +710,7 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
   // the density sweep changes cells after the stage kernel summed them
   compute_ubar<NT>(k, b, serial_ubar, g->rho_any ? nullptr : upart, sh, tid);
   double ub[4] = {sh.ubar[0], sh.ubar[1], sh.ubar[2], sh.ubar[3]};
-  const double pb = sweep_pres(k.gamma1, ub[0], ub[1], ub[2], ub[3]);  // P(u_bar)
+  const double pb = sweep_pres(k, ub[0], ub[1], ub[2], ub[3]);  // P(u_bar)
   if (k.pbar_check) {
     if (pb <= 0.) {
       if (tid == 0) {

@@ -144,6 +144,27 @@ SWF_HD double weno_d32_or_64(const double e[4], double q4, float q4f) {
   return ((a1 * c3 + a2 * c2) + a3 * c1) * rcp64(6.0 * ((a1 + a2) + a3));
 }
 
+// weno_right(a) - a(0) (RIGHT) or weno_left(a) - a(0) with everything in FP32: the
+// differences e, the smoothness indicators and weights (as weno_d32_or_64), the
+// candidate values and their weighted sum.  Beyond FP32 range of the indicators
+// (|e| > 2.5e17) the FP64 form of the same differences.
+template <bool RIGHT>
+SWF_HD float weno_d32f(const float e[4], double q4, float q4f) {
+  const float em2 = e[0], em1 = e[1], e0 = e[2], e1 = e[3];
+  const float big = fmaxf(fmaxf(fabsf(em2), fabsf(em1)), fmaxf(fabsf(e0), fabsf(e1)));
+  if (!(big <= 2.5e17f)) {
+    const double d[4] = {em2, em1, e0, e1};
+    return (float)weno_d64<RIGHT>(d, q4);
+  }
+  const float x[3] = {e1 - e0, e0 - em1, em1 - em2}, y[3] = {e1 - 3.0f * e0, em1 + e0, 3.0f * em1 - em2};
+  float al[3];
+  alphas_d32<RIGHT>(x, y, al, q4f);
+  const float c3 = RIGHT ? 4.0f * e0 - e1 : 2.0f * e1 - 5.0f * e0;
+  const float c2 = RIGHT ? em1 + 2.0f * e0 : -(2.0f * em1 + e0);
+  const float c1 = RIGHT ? 5.0f * em1 - 2.0f * em2 : em2 - 4.0f * em1;
+  return ((al[0] * c3 + al[1] * c2) + al[2] * c1) * rcp32(6.0f * ((al[0] + al[1]) + al[2]));
+}
+
 // the four first differences of a(-2..2)
 SWF_HD void first_differences(const double *a, double e[4]) {
   e[0] = a[-1] - a[-2];

@@ -74,11 +74,12 @@ def first_dt(cfg):
     return {"rc": rc, "dt": [dto, s.step(0.0, 0.001)]}
 
 
-def crafted_sweep(n, nbad, seed, chain=0):
+def crafted_sweep(n, nbad, seed, chain=0, psum=0):
     """swf_sweep on a state with nbad cells of slightly negative pressure (runs of
     consecutive flagged cells included) and three near-vacuum cells, against the
     oracle's sweep: fields bitwise and statistics equal."""
     cfg = S.sedov_config(n, "fp64_exact")
+    cfg.psum_check = psum  # the step program's pressure-sum check on these walks
     rng = np.random.default_rng(seed)
     u = np.zeros(S.state_shape(cfg))
     u[:, :, 0] = 1.0 + 0.1 * rng.random(u.shape[:2])
@@ -108,7 +109,13 @@ def crafted_sweep(n, nbad, seed, chain=0):
     rc = o.sweep()
     s = S.Solver(cfg)
     s.upload(u)
-    s.sweep()
+    try:
+        s.sweep()
+        rce = 0
+    except S.SwfError as e:
+        rce = e.code
+    if psum:
+        return {"rc": [rc, rce]}
     a, b = s.download()[3:-3, 3:-3], o.state()[3:-3, 3:-3]
     so, se = o.stats(), s.stats()
     return {"rc": rc, "steps": [0, 0], "time": [0.0, 0.0], "differ": int((a.view(np.int64) != b.view(np.int64)).sum()),
@@ -206,6 +213,9 @@ def main(name):
     elif name == "vortex_flux_limiter":
         cfg = S.vortex_config(90, "fp64_exact", use_limiter=1)
         out = run_both(cfg, S.init_vortex(cfg), 0.01, 100)
+    elif name.startswith("crafted_psum_"):
+        n, nbad, seed = (int(v) for v in name[len("crafted_psum_"):].split("_"))
+        out = crafted_sweep(n, nbad, seed, 0, 1)
     elif name.startswith("crafted_sweep_"):
         n, nbad, *rest = (int(v) for v in name[len("crafted_sweep_"):].split("_"))
         out = crafted_sweep(n, nbad, 7, rest[0] if rest else 0)

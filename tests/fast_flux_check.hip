@@ -1,4 +1,4 @@
-// Host check of the FP64 and FP32-bulk interface fluxes (half_flux_fast of
+// Host check of the FP64, FP32-bulk and FP32-full interface fluxes (half_flux_fast of
 // sweeping_fd_weno_amd/csrc/swf_device.h, compiled for the host) against the
 // bit-exact path (half_flux_exact, which the GPU tests pin to the reference) on
 // Sedov states produced by the oracle.
@@ -6,12 +6,12 @@
 //   fast_flux_check <n> <steps>
 //
 // For the states after 0..steps oracle steps, prints per state
-//   state <s> l1 <fp64> <fp32> worst <fp64> <fp32> emin <E>
+//   state <s> l1 <fp64> <fp32> <fp32f> worst <fp64> <fp32> <fp32f> emin <E32> <E32f>
 // with l1 = sum|F_mode - F_exact| / sum|F_exact| over all x and y interfaces,
 // worst = max |F_mode - F_exact| / (|f+| + |f-|) per interface and component, and
 // emin = the smallest energy after one forward-Euler stage (dt = 0.5 dx/alpha)
-// with the FP32-bulk fluxes: the ambient gas holds E = 1e-12, so a flux error
-// above ~1e-12 shows up as a negative energy there.
+// with the FP32-bulk / FP32-full fluxes: the ambient gas holds E = 1e-12, so a
+// flux error above ~1e-12 shows up as a negative energy there.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -101,9 +101,10 @@ int main(int argc, char **argv) {
   for (int s = 0; s <= steps; ++s) {
     swo_get_state(o, u.data());
     const double alpha = swo_alpha_max(o), dx = oc.dx, dt = 0.5 * dx / alpha;
-    double sum64 = 0, sum32 = 0, sumx = 0, worst64 = 0, worst32 = 0;
-    // net fluxes of the FP32-bulk mode for the energy check: F(i+1/2, j), G(i, j+1/2)
+    double sum64 = 0, sum32 = 0, sum3f = 0, sumx = 0, worst64 = 0, worst32 = 0, worst3f = 0;
+    // net fluxes of the FP32 modes for the energy check: F(i+1/2, j), G(i, j+1/2)
     std::vector<double> F((size_t)4 * W * W, 0.0), G((size_t)4 * W * W, 0.0);
+    std::vector<double> Ff((size_t)4 * W * W, 0.0), Gf((size_t)4 * W * W, 0.0);
     for (int dir = 0; dir < 2; ++dir)
       for (int line = 1; line <= n; ++line)
         for (int kk = 0; kk <= n; ++kk) {
@@ -111,44 +112,53 @@ int main(int argc, char **argv) {
           for (int d = 0; d < NP; ++d)
             for (int c = 0; c < 4; ++c) st[d][c] = dir == 0 ? at(c, kk - 2 + d, line) : at(c, line, kk - 2 + d);
           Line Lx, Lf;
-          double ax[4], cx[4], a64[4], c64[4], a32[4], c32[4];
+          double ax[4], cx[4], a64[4], c64[4], a32[4], c32[4], a3f[4], c3f[4];
           if (dir == 0) {
             stage_line<MODE_EXACT, 0>(k, st, alpha, Lx);
             stage_line<MODE_FP64, 0>(k, st, alpha, Lf);
             halves<MODE_EXACT, 0>(k, Lx, ax, cx);
             halves<MODE_FP64, 0>(k, Lf, a64, c64);
             halves<MODE_FP32, 0>(k, Lf, a32, c32);
+            halves<MODE_FP32F, 0>(k, Lf, a3f, c3f);
           } else {
             stage_line<MODE_EXACT, 1>(k, st, alpha, Lx);
             stage_line<MODE_FP64, 1>(k, st, alpha, Lf);
             halves<MODE_EXACT, 1>(k, Lx, ax, cx);
             halves<MODE_FP64, 1>(k, Lf, a64, c64);
             halves<MODE_FP32, 1>(k, Lf, a32, c32);
+            halves<MODE_FP32F, 1>(k, Lf, a3f, c3f);
           }
           for (int c = 0; c < 4; ++c) {
             const double fx = alpha * ax[c] - alpha * cx[c];
             const double f64 = alpha * a64[c] - alpha * c64[c], f32 = alpha * a32[c] - alpha * c32[c];
+            const double f3f = alpha * a3f[c] - alpha * c3f[c];
             const double scale = fabs(alpha * ax[c]) + fabs(alpha * cx[c]) + 1e-300;
             sumx += fabs(fx);
             sum64 += fabs(f64 - fx);
             sum32 += fabs(f32 - fx);
+            sum3f += fabs(f3f - fx);
             worst64 = fmax(worst64, fabs(f64 - fx) / scale);
             worst32 = fmax(worst32, fabs(f32 - fx) / scale);
+            worst3f = fmax(worst3f, fabs(f3f - fx) / scale);
             if (!(f32 == f32)) worst32 = INFINITY;
+            if (!(f3f == f3f)) worst3f = INFINITY;
             const size_t o0 = c + 4 * (dir == 0 ? ((size_t)(kk + 2) + (size_t)W * (line + 2))
                                                 : ((size_t)(line + 2) + (size_t)W * (kk + 2)));
             (dir == 0 ? F : G)[o0] = f32;
+            (dir == 0 ? Ff : Gf)[o0] = f3f;
           }
         }
-    double emin = INFINITY;
+    double emin = INFINITY, eminf = INFINITY;
     for (int j = 1; j <= n; ++j)
       for (int i = 1; i <= n; ++i) {
         const size_t o3 = 3 + 4 * ((size_t)(i + 2) + (size_t)W * (j + 2));
         const double e = u[o3] + dt * (-((F[o3] - F[o3 - 4]) / dx) - (G[o3] - G[o3 - 4 * (size_t)W]) / dx);
+        const double ef = u[o3] + dt * (-((Ff[o3] - Ff[o3 - 4]) / dx) - (Gf[o3] - Gf[o3 - 4 * (size_t)W]) / dx);
         emin = (e == e) ? fmin(emin, e) : -INFINITY;
+        eminf = (ef == ef) ? fmin(eminf, ef) : -INFINITY;
       }
-    printf("state %d l1 %.3e %.3e worst %.3e %.3e emin %.3e\n", s, sum64 / sumx, sum32 / sumx, worst64, worst32,
-           emin);
+    printf("state %d l1 %.3e %.3e %.3e worst %.3e %.3e %.3e emin %.3e %.3e\n", s, sum64 / sumx, sum32 / sumx,
+           sum3f / sumx, worst64, worst32, worst3f, emin, eminf);
     if (s < steps) {
       double dts;
       if (swo_step(o, t, 1.0, &dts) != SWO_OK) {

@@ -67,6 +67,17 @@ def test_emulated_kernels_match_oracle(emu_lib, name):
         assert r["stats"]["pre_count"][0] > 0
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+def test_emulated_pressure_sum_check_agrees_with_oracle(emu_lib, seed):
+    """Crafted sweeps with the step program's pressure-sum check on: the device raises
+    SWF_E_PSUM only for genuine walks (speculative walks that are discarded do not
+    count), exactly when the oracle's serial sweep stops.  Pressure is concave in the
+    conserved variables, so a pair adjustment does not lower the pair's total pressure
+    beyond rounding, and on these states neither side stops."""
+    r = scenario(f"crafted_psum_48_400_{seed}")
+    assert r["rc"][0] == r["rc"][1]
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-7), ("fp32_bulk", 1e-4)])
 def test_emulated_fast_modes_jet_sweeps(emu_lib, prec, tol):
     """FP64 / FP32-bulk full jet 64x32, 25 steps with sweeps at every stage from

@@ -82,3 +82,20 @@ def test_batch_tune_leaves_states_untouched():
     assert pick in (0, 1, 2) and all(m > 0 for m in ms)
     for a, b in zip(res, single):
         assert a[1] == b[1] and np.array_equal(a[2].view(np.int64), b[2].view(np.int64))
+
+
+def test_batch_fp32_full_rejects_jets_and_matches_single_runs():
+    """SWF_FP32_FULL batches: Sedov, vortex and step instances equal their runs alone bit
+    for bit (the step instance runs the FP32-bulk form in both); a jet is rejected (its
+    FP32-full run would use the FP32-bulk kernels, which the other instances do not)."""
+    cfgs, inits, tf = _instances("fp32_full")
+    with pytest.raises(S.SwfError):
+        S.Batch(cfgs)
+    keep = [i for i, c in enumerate(cfgs) if c.bc not in (S.BC_JET_HALF, S.BC_JET_FULL)]
+    cfgs, inits, tf = [cfgs[i] for i in keep], [inits[i] for i in keep], tf[keep]
+    single = _singles(cfgs, inits, tf, 30)
+    res, st, _ = _batch(cfgs, inits, tf, 30)
+    assert (st == 0).all()
+    for i, (a, b) in enumerate(zip(res, single)):
+        assert a[0] == b[0] and a[1] == b[1], i
+        assert np.array_equal(a[2].view(np.int64), b[2].view(np.int64)), i

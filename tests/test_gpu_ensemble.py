@@ -6,10 +6,10 @@ sweeping_fd_weno_amd.batch) advanced in one batch.
 - FP64-exact, 20 steps: every instance bit for bit equal to its run alone, with equal
   statistics; the first instance of each family (Sedov 90^2, vortex 90^2, step 104 x 88)
   bitwise to the oracle.
-- FP32-bulk, 10 steps: each instance's relative L1 density difference from the
-  FP64-exact batch within 1e-6 (the step problem is chaotic beyond ~10 steps on its
+- FP32-bulk and FP32-full, 10 steps: each instance's relative L1 density difference
+  from the FP64-exact batch within 1e-6 (the step problem is chaotic beyond ~10 steps on its
   coarse grids: a 1e-15 perturbation grows to 1.4e-4 after 20 steps).
-- FP32-bulk, 200 steps (the ensemble run of `python -m sweeping_fd_weno_amd.batch`): no
+- FP32-bulk and FP32-full, 200 steps (the ensemble run of `python -m sweeping_fd_weno_amd.batch`): no
   stop, every instance finite with positive density and pressure.
 """
 import numpy as np
@@ -85,10 +85,11 @@ def test_ensemble48_exact_equals_single_runs_and_oracle():
             assert so[key] == res[i][3][key], (label, key)
 
 
-def test_ensemble48_fp32_bulk_close_to_exact():
+@pytest.mark.parametrize("prec", ["fp32_bulk", "fp32_full"])
+def test_ensemble48_fp32_close_to_exact(prec):
     steps = 10
     ex, _ = _run_batch(B.ensemble("fp64_exact"), steps)
-    ens = B.ensemble("fp32_bulk")
+    ens = B.ensemble(prec)
     fb, st = _run_batch(ens, steps)
     assert (st == 0).all()
     worst = {}
@@ -103,8 +104,9 @@ def test_ensemble48_fp32_bulk_close_to_exact():
     print("worst relative L1 density per family:", worst)
 
 
-def test_ensemble48_fp32_bulk_200_steps_positive():
-    ens = B.ensemble("fp32_bulk")
+@pytest.mark.parametrize("prec", ["fp32_bulk", "fp32_full"])
+def test_ensemble48_fp32_200_steps_positive(prec):
+    ens = B.ensemble(prec)
     res, st = _run_batch(ens, 200)
     assert (st == 0).all()
     for i, (label, c, _, _) in enumerate(ens):

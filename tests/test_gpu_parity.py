@@ -62,7 +62,7 @@ def test_weno_exact_matches_reference_bitwise():
     assert_bitwise(l, g["left"], "weno_left")
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-14), ("fp32_bulk", 5e-7)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-14), ("fp32_bulk", 5e-7), ("fp32_full", 2e-6)])
 def test_weno_fast_modes_close(prec, tol):
     """Fast-mode reconstructions on the device against the reference's, relative
     to the size of the correction max|a(k) - a(0)| (the device's reciprocal
@@ -375,7 +375,7 @@ def test_vortex_flux_limiter_fast_modes_l1_vs_exact(prec, tol):
 
 
 @pytest.mark.parametrize("n,steps", [(128, 40), (640, 1500)])
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32_bulk", 1e-6)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32_bulk", 1e-6), ("fp32_full", 1e-6)])
 def test_sedov_fast_modes_l1_vs_exact(prec, tol, n, steps):
     """L1 density difference (relative to the L1 norm of density) after `steps`
     steps of Sedov n^2, against the bit-exact path; the FP64 path also keeps the

@@ -9,7 +9,7 @@ in the three precision modes, from the program's initial data and initial bc
   are identical in every mode.
 - Relative L1 density difference from the FP64-exact run (bit-identical to the
   reference's FP64 program; see test_gpu_reference_runs.py for the 640^2 window against
-  the reference itself): at most 1e-12 for FP64 and 1e-6 for FP32-bulk.
+  the reference itself): at most 1e-12 for FP64 and 1e-6 for FP32-bulk and FP32-full.
 """
 import numpy as np
 import pytest
@@ -39,7 +39,7 @@ def _run_to_t1(prec):
 
 
 def test_sedov640_to_t1_all_modes():
-    runs = {p: _run_to_t1(p) for p in ("fp64_exact", "fp64", "fp32_bulk")}
+    runs = {p: _run_to_t1(p) for p in ("fp64_exact", "fp64", "fp32_bulk", "fp32_full")}
     ex = runs["fp64_exact"]
     assert not S.time_continue(ex[0], 1.0) and ex[1] == 28650
     rho_ex = ex[2][3:-3, 3:-3, 0]
@@ -54,6 +54,6 @@ def test_sedov640_to_t1_all_modes():
         print(f"{p}: {k} steps, t = {t!r}, L1 rho vs exact {l1:.3e}, stats {st}")
         if p == "fp64":
             assert l1 <= 1e-12
-        elif p == "fp32_bulk":
+        elif p in ("fp32_bulk", "fp32_full"):
             assert l1 <= 1e-6
     assert ex[3]["sweep_count"] > 0

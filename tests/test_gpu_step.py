@@ -74,7 +74,7 @@ def test_step_exact_matches_oracle_bitwise():
     assert so["sweep_count"] >= 30 and so["den_count"] >= 3 and so["total_neg"] >= 1
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32_bulk", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32_bulk", 1e-6), ("fp32_full", 1e-6)])
 def test_step_fast_modes_close_to_exact(precision, tol):
     ue, te, ke, _ = _device_run(8, "fp64_exact", 10)
     u, t, k, _ = _device_run(8, precision, 10)
@@ -105,3 +105,13 @@ def test_step_create_rejects_inconsistent_configs():
     bad.ny = 6 * 8 + 2                  # no room above the step
     with pytest.raises(S.SwfError):
         S.Solver(bad)
+
+
+def test_step_fp32_full_runs_the_fp32_bulk_form():
+    """The step program keeps the FP32-bulk form under SWF_FP32_FULL (its coarse grids are
+    chaotic: FP32 projections measured 2.3e-6 relative L1 density after 10 steps at
+    104 x 88): the two modes give the same bits."""
+    a = _device_run(8, "fp32_bulk", 10)
+    b = _device_run(8, "fp32_full", 10)
+    assert a[2] == b[2] and a[1] == b[1]
+    assert np.array_equal(a[0].view(np.int64), b[0].view(np.int64))

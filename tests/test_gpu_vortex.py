@@ -6,7 +6,8 @@
 - The accuracy series 45^2 .. 360^2 (config of the north-star vortex row): the
   exact mode's 180^2 norms equal the reference's; FP64 stays within 1e-12 and
   FP32-bulk within 1e-6 of the exact run in relative L1 density (tolerances
-  written here); the order between 180^2 and 360^2 exceeds 4 in every mode.
+  written here; FP32-full, with the projections and reconstructions in FP32 too, also
+  within 1e-6); the order between 180^2 and 360^2 exceeds 4 in every mode.
 """
 import hashlib
 import json
@@ -57,7 +58,7 @@ def test_vortex_driver_flux_limiter_reproduces_reference_output_files(tmp_path):
 
 def test_vortex_accuracy_series():
     g = json.load(open(GOLD))
-    rows = A.vortex_series((45, 90, 180, 360), ("fp64_exact", "fp64", "fp32_bulk"))
+    rows = A.vortex_series((45, 90, 180, 360), ("fp64_exact", "fp64", "fp32_bulk", "fp32_full"))
     for r in rows:
         print(r["precision"], r["n"], r["steps"], "L1 %.6e (%.2f) Linf %.6e (%.2f)" %
               (r["L1"], r["L1_order"], r["Linf"], r["Linf_order"]), r.get("L1_vs_exact"), r.get("Linf_vs_exact"))
@@ -69,7 +70,8 @@ def test_vortex_accuracy_series():
     for n in (45, 90, 180, 360):
         assert by[("fp64", n)]["L1_vs_exact"] <= 1e-12
         assert by[("fp32_bulk", n)]["L1_vs_exact"] <= 1e-6
-    for prec in ("fp64_exact", "fp64", "fp32_bulk"):
+        assert by[("fp32_full", n)]["L1_vs_exact"] <= 1e-6
+    for prec in ("fp64_exact", "fp64", "fp32_bulk", "fp32_full"):
         assert by[(prec, 360)]["L1_order"] > 4.0
 
 
