@@ -575,8 +575,15 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
         wu[c] = k.weight * un;
       }
       write_ghosts(k, b, i, j, v);
-      // checkpositivity (ex4_sedov_2d/main.f90:779-801)
-      const double ke = (v[3] - ((v[2] * v[2]) / v[0]) / 2.) - ((v[1] * v[1]) / v[0]) / 2.;
+      // checkpositivity (ex4_sedov_2d/main.f90:779-801); the fast modes divide by rho
+      // through the refined reciprocal that sweep_flag and cell_alpha_fast form as well
+      double ke;
+      if (M == MODE_EXACT) {
+        ke = (v[3] - ((v[2] * v[2]) / v[0]) / 2.) - ((v[1] * v[1]) / v[0]) / 2.;
+      } else {
+        const double r = rcp64(v[0]);
+        ke = (v[3] - (0.5 * (v[2] * v[2])) * r) - (0.5 * (v[1] * v[1])) * r;
+      }
       const bool bad = k.poscheck_le ? (v[0] <= k.eps || ke <= k.eps) : (v[0] < k.eps || ke < k.eps);
       if (bad) atomicOr(&g->postag, 1);
       if (k.use_limiter == 2) {
