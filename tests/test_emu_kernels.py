@@ -36,19 +36,40 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EMU = os.path.join(ROOT, "tests", "emu")
 
 
+# every scenario of this module: run concurrently, one process each, when the module starts
+SCENARIOS = ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter", "crafted_sweep_24_14",
+             "crafted_sweep_128_3000", "crafted_sweep_160_12000", "crafted_sweep_64_20_100",
+             "crafted_sweep_64_20_300", "crafted_psum_48_400_1", "crafted_psum_48_400_2", "jet_full_sweeps_fp64",
+             "jet_full_sweeps_fp32_bulk", "batch_fp64_exact", "batch_fp32_bulk", "first_dt_fp64_exact",
+             "first_dt_fp64", "first_dt_fp32_bulk", "line_small", "line_stress_fp64_exact", "line_cfg1",
+             "step_fp64_exact_4_40"]
+_RESULTS = {}
+
+
 @pytest.fixture(scope="module")
 def emu_lib():
     subprocess.run(["make", "-s", "-C", EMU], check=True, capture_output=True)
     lib = os.path.join(EMU, "_build", "libswf_emu.so")
     assert os.path.exists(lib)
+    # the longest first, at most as many at once as the host has cores (8 at most)
+    order = sorted(SCENARIOS, key=lambda n: not n.startswith(("batch", "step", "jet_full", "vortex")))
+    width = max(1, min(8, os.cpu_count() or 1))
+    running = []
+    for name in order + [None] * width:
+        while running and (name is None or len(running) >= width):
+            nm, p = running.pop(0)
+            out, err = p.communicate(timeout=900)
+            _RESULTS[nm] = (p.returncode, out, err)
+        if name is not None:
+            running.append((name, subprocess.Popen([sys.executable, os.path.join(EMU, "scenarios.py"), name],
+                                                   stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
     return lib
 
 
 def scenario(name):
-    res = subprocess.run([sys.executable, os.path.join(EMU, "scenarios.py"), name], capture_output=True, text=True,
-                         timeout=600)
-    assert res.returncode == 0, res.stdout + res.stderr
-    return json.loads(res.stdout.strip().splitlines()[-1])
+    rc, out, err = _RESULTS[name]
+    assert rc == 0, out + err
+    return json.loads(out.strip().splitlines()[-1])
 
 
 @pytest.mark.parametrize("name", ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter",
