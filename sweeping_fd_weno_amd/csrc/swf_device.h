@@ -134,9 +134,6 @@ struct BatchStatus {
 // i + XOFF, so interior cell 1 starts a 128-byte line and each 16-cell tile row of
 // the stage kernel is exactly one cache line; the pitch P is a multiple of 16.
 constexpr int XOFF = 15;
-// x-flux tile origins are 64 bx - XT_SHIFT: the tile's first stencil point
-// (origin - 2) then starts a line
-constexpr int XT_SHIFT = XOFF + 46;
 SWF_HD int row_pitch(int nx) { return (nx + 3 + XOFF + 1 + 15) / 16 * 16; }
 
 // index of cell (i, j) (ghost rows from j = -2) inside a plane
@@ -152,9 +149,10 @@ SWF_HD int ub_gx(const KCfg &k) { return (k.nx + UB_W - 1) / UB_W; }
 SWF_HD int ub_gy(const KCfg &k) { return (k.ny + UB_H - 1) / UB_H; }
 SWF_HD int ub_of(const KCfg &k, int i, int j) { return ((j - 1) / UB_H) * ub_gx(k) + (i - 1) / UB_W; }
 
-// x-flux workgroups: 64 interfaces x 4 rows, tile origins 64 bx - XT_SHIFT
-// covering interfaces 0..nx
-SWF_HD int xf_gx(const KCfg &k) { return (k.nx + XT_SHIFT + 64) / 64; }
+// x-flux workgroups: 64 centres x 4 rows, tile origins XT_C bx covering the
+// interfaces 0..nx (lanes 0..XT_C-1 of a row give interfaces)
+constexpr int XT_C = 63;
+SWF_HD int xf_gx(const KCfg &k) { return (k.nx + 1 + XT_C - 1) / XT_C; }
 SWF_HD int xf_gy(const KCfg &k) { return (k.ny + 3) / 4; }
 
 // Flux-limiter half fluxes that `scheme` does not read but the reference's
@@ -387,30 +385,47 @@ SWF_HD double weno_left(const double *a, const WenoEps &we) {
 // ---------------------------------------------------------------------------
 // FP64-exact: the reference's operations in its order, structural zeros kept
 // (they decide signs of zero), binary128 sites emulated.
-// E12: the weights' constant is 1d-12 (the step program's weno5.f90), else 1d-6
-template <int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
-SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+// The Roe system at (a|b) (ex4_sedov_2d/main.f90:232-276, :452-497): the scalars
+// every entry of the eigenvector matrices is formed from, in the reference's
+// expressions.  One system serves both halves of a centre (fp_flux(i) and
+// fn_flux(i) share Roe(i, i+1)).
+struct RoeExact {
+  double vn, vt, hm, qm, cm, tc, rcm, b1, b2, tr, tn, t2, tt;
+};
+template <int DIR>
+SWF_HD RoeExact roe_exact(const KCfg &k, const Prim &pa, const Prim &pb) {
   const double g1 = k.gamma1;
-  const WenoEps we = E12 ? kWeps12 : kWeps6;
+  RoeExact r;
   const double t0 = k.roe_eps_double ? pa.w / ((pa.w + pb.w) + 1e-15) : roe_weight_q(pa.w, pb.w);
   const double t1 = 1. - t0;
   const double vxm = t0 * pa.vx + t1 * pb.vx;
   const double vym = t0 * pa.vy + t1 * pb.vy;
-  const double hm = t0 * pa.h + t1 * pb.h;
-  const double qm = 0.5 * (vxm * vxm + vym * vym);
-  const double sq = sqrt(fabs(g1 * (hm - qm)));
-  const double cm = k.roe_eps_double ? sq + 1e-15 : cm_q(sq);
+  r.hm = t0 * pa.h + t1 * pb.h;
+  r.qm = 0.5 * (vxm * vxm + vym * vym);
+  const double sq = sqrt(fabs(g1 * (r.hm - r.qm)));
+  r.cm = k.roe_eps_double ? sq + 1e-15 : cm_q(sq);
   // normal (vn) and tangential (vt) velocity for this direction
-  const double vn = (DIR == 0) ? vxm : vym;
-  const double vt = (DIR == 0) ? vym : vxm;
-  const double tc = vn * cm;
-  const double rcm = 1. / cm;
-  const double b1 = (g1 * rcm) * rcm;
-  const double b2 = qm * b1;
-  const double tr = vn * rcm;
-  const double tn = b1 * vn;
-  const double t2 = 0.5 * b1;
-  const double tt = b1 * vt;
+  r.vn = (DIR == 0) ? vxm : vym;
+  r.vt = (DIR == 0) ? vym : vxm;
+  r.tc = r.vn * r.cm;
+  r.rcm = 1. / r.cm;
+  r.b1 = (g1 * r.rcm) * r.rcm;
+  r.b2 = r.qm * r.b1;
+  r.tr = r.vn * r.rcm;
+  r.tn = r.b1 * r.vn;
+  r.t2 = 0.5 * r.b1;
+  r.tt = r.b1 * r.vt;
+  return r;
+}
+
+// One half of a centre's flux from its Roe system: projection of the 5-point
+// stencil `st`, 4 reconstructions, back-projection (ex4_sedov_2d/main.f90:278-304).
+// E12: the weights' constant is 1d-12 (the step program's weno5.f90), else 1d-6
+template <int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
+SWF_HD void half_exact(const RoeExact &q, const double *st, double out[4]) {
+  const WenoEps we = E12 ? kWeps12 : kWeps6;
+  const double vn = q.vn, vt = q.vt, hm = q.hm, qm = q.qm, cm = q.cm, tc = q.tc, rcm = q.rcm;
+  const double b1 = q.b1, b2 = q.b2, tr = q.tr, tn = q.tn, t2 = q.t2, tt = q.tt;
   // Left eigenvectors, rows k = 1..4, written in (normal, tangential) form;
   // component order (rho, m_n, m_t, E) is mapped to (rho, mx, my, E) below.
   double L[4][4];
@@ -449,6 +464,12 @@ SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const
   }
 }
 
+// One half flux with the Roe system at (a|b) (ex4_sedov_2d/main.f90:228-304, :452-526)
+template <int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
+SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+  half_exact<DIR, RIGHT, SS, CS, E12>(roe_exact<DIR>(k, pa, pb), st, out);
+}
+
 // FP64 / FP32-bulk: the same Roe eigen-system, with the reconstruction written
 // as centre value plus correction (swf_weno.h):
 //     out = f(0) + R * weno_d(L * (first differences of f))
@@ -460,22 +481,38 @@ SWF_HD void half_flux_exact(const KCfg &k, const Prim &pa, const Prim &pb, const
 // FP32-full (M == 3): the projections, the weights and candidates, and the
 // back-projection in FP32; the Roe average, the centre value and the R * L energy
 // term below in FP64.
-template <int M, int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
-SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+// The Roe system of the fast modes at (a|b): what both halves of a centre need.
+struct RoeFast {
+  double vn, vt, hm, qm, hq, cm, rcm, b1, b2, tn, tt;
+};
+template <int DIR>
+SWF_HD RoeFast roe_fast(const KCfg &k, const Prim &pa, const Prim &pb) {
   const double g1 = k.gamma1;
-  const WenoEps we = E12 ? kWeps12 : kWeps6;
+  RoeFast q;
   const double t0 = pa.w * rcp64((pa.w + pb.w) + 1e-15);
   const double t1 = 1. - t0;
   const double vxm = t0 * pa.vx + t1 * pb.vx;
   const double vym = t0 * pa.vy + t1 * pb.vy;
-  const double hm = t0 * pa.h + t1 * pb.h;
-  const double qm = 0.5 * (vxm * vxm + vym * vym);
-  const double hq = hm - qm;
-  const double cm = sqrt64(fabs(g1 * hq)) + 1e-15;
-  const double vn = (DIR == 0) ? vxm : vym;
-  const double vt = (DIR == 0) ? vym : vxm;
-  const double rcm = rcp64(cm);
-  const double b1 = (g1 * rcm) * rcm, b2 = qm * b1, tn = b1 * vn, tt = b1 * vt;
+  q.hm = t0 * pa.h + t1 * pb.h;
+  q.qm = 0.5 * (vxm * vxm + vym * vym);
+  q.hq = q.hm - q.qm;
+  q.cm = sqrt64(fabs(g1 * q.hq)) + 1e-15;
+  q.vn = (DIR == 0) ? vxm : vym;
+  q.vt = (DIR == 0) ? vym : vxm;
+  q.rcm = rcp64(q.cm);
+  q.b1 = (g1 * q.rcm) * q.rcm;
+  q.b2 = q.qm * q.b1;
+  q.tn = q.b1 * q.vn;
+  q.tt = q.b1 * q.vt;
+  return q;
+}
+
+// One half of a centre's flux in the fast modes from its Roe system.
+template <int M, int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
+SWF_HD void half_fast(const KCfg &k, const RoeFast &rs, const double *st, double out[4]) {
+  const WenoEps we = E12 ? kWeps12 : kWeps6;
+  const double vn = rs.vn, vt = rs.vt, hm = rs.hm, qm = rs.qm, hq = rs.hq, cm = rs.cm, rcm = rs.rcm;
+  const double b1 = rs.b1, b2 = rs.b2, tn = rs.tn, tt = rs.tt;
   const int cn = (DIR == 0) ? 1 : 2, ct = (DIR == 0) ? 2 : 1;
   double corr[4];
   if constexpr (M == MODE_FP32F) {
@@ -540,12 +577,36 @@ SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const 
   for (int c = 0; c < 4; ++c) out[c] = st[c * CS] + corr[c];
 }
 
+// One half flux with the Roe system at (a|b)
+template <int M, int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
+SWF_HD void half_flux_fast(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
+  half_fast<M, DIR, RIGHT, SS, CS, E12>(k, roe_fast<DIR>(k, pa, pb), st, out);
+}
+
 template <int M, int DIR, bool RIGHT, int SS, int CS, bool E12 = false>
 SWF_HD void half_flux(const KCfg &k, const Prim &pa, const Prim &pb, const double *st, double out[4]) {
   if (M == MODE_EXACT)
     half_flux_exact<DIR, RIGHT, SS, CS, E12>(k, pa, pb, st, out);
   else
     half_flux_fast<M, DIR, RIGHT, SS, CS, E12>(k, pa, pb, st, out);
+}
+
+// Both halves of centre i with the one Roe system of (i, i+1): fp_flux(i) =
+// R weno_right(L fp) from the stencil `stp` and fn_flux(i) = R weno_left(L fn)
+// from `stn` (ex4_sedov_2d/main.f90:228-304: the reference forms both in one
+// iteration of its i loop).  Each result equals half_flux's for that half.
+template <int M, int DIR, int SS, int CS, bool E12 = false>
+SWF_HD void centre_fluxes(const KCfg &k, const Prim &pa, const Prim &pb, const double *stp, const double *stn,
+                          double outp[4], double outn[4]) {
+  if constexpr (M == MODE_EXACT) {
+    const RoeExact q = roe_exact<DIR>(k, pa, pb);
+    half_exact<DIR, true, SS, CS, E12>(q, stp, outp);
+    half_exact<DIR, false, SS, CS, E12>(q, stn, outn);
+  } else {
+    const RoeFast q = roe_fast<DIR>(k, pa, pb);
+    half_fast<M, DIR, true, SS, CS, E12>(k, q, stp, outp);
+    half_fast<M, DIR, false, SS, CS, E12>(k, q, stn, outn);
+  }
 }
 
 // ---------------------------------------------------------------------------

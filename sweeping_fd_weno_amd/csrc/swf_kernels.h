@@ -24,9 +24,10 @@
 
 namespace swf {
 
-// x tile: 64 interfaces x 4 rows; stencil points k0-2 .. k0+66 with the tile
-// origin k0 = 64 bx - XT_SHIFT (first stencil point at the start of a line)
-constexpr int XT_W = 64, XT_H = 4, XT_PTS = XT_W + 5;
+// x tile: 64 centres (k0 .. k0+63) of each of 4 rows; lanes 0..62 give the
+// interfaces k0+1/2 .. k0+62+1/2, so tile origins step by XT_C = 63.  Stencil points
+// k0-2 .. k0+65 of each row.
+constexpr int XT_H = 4, XT_PTS = 64 + 4;
 // y tile: 16 columns x 16 interfaces (jf0 .. jf0+15) -> 15 cells (jf0+1 .. jf0+15);
 // stencil points jf0-2 .. jf0+18
 constexpr int YT_W = UB_W, YT_H = UB_H + 1, YT_PTS = YT_H + 5;
@@ -181,11 +182,14 @@ SWF_D void zs_store(int den, int pre, int *pair, int *red, int tid) {
 #define SWF_FLUX_BOUNDS(M) __launch_bounds__(256, (M) == MODE_EXACT ? 1 : 4)
 #define SWF_STAGE_BOUNDS(M) __launch_bounds__(YS_T, (M) == MODE_EXACT ? 1 : 4)
 
-// wenofd x-pass for interfaces k0..k0+63 of rows j0..j0+3 -> Fx.  In an RK stage
-// (stage = 1, 2, 3; 0 for a stand-alone evaluation) the launch also resets the
-// stage's sweep bitmaps and flags (the stage kernel sets them), and at stage 1
-// of an inactive step keeps the next step inactive.
-// Tile (bx, by) of a gx x gy grid of x-flux tiles.
+// wenofd x-pass -> Fx for the centres k0 = XT_C bx .. k0 + 63 of rows j0 = 1 + 4 by
+// .. j0 + 3.  Each lane forms both halves of its centre k with the one Roe system
+// of (k, k+1) (centre_fluxes), and lanes 0..XT_C-1 the net interface flux
+// F(k+1/2) = fp_flux(k) - fn_flux(k+1), fn_flux(k+1) from the next lane; the
+// tile's last centre is the next tile's first.  In an RK stage (stage = 1, 2, 3; 0
+// for a stand-alone evaluation) the launch also resets the stage's sweep bitmaps
+// and flags (the stage kernel sets them), and at stage 1 of an inactive step keeps
+// the next step inactive.  Tile (bx, by) of a gx x gy grid of x-flux tiles.
 // STEP: the step diffraction program's wenofd (ex6_shock_diffraction/main.f90:
 // 170-647): each row (x-pass) / column (y-pass) split and scaled with its own
 // alpha max, and 1d-12 in the WENO weights.
@@ -213,7 +217,7 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
   }
   const double alpha = alpha_from_bits(g->alpha_bits[slot_in]);
   const double ia = 1.0 / alpha;
-  const int k0 = bx * XT_W - XT_SHIFT, j0 = 1 + by * XT_H;
+  const int k0 = bx * XT_C, j0 = 1 + by * XT_H;
   const double *aline = STEP ? b.arow : nullptr;
   stage_tile<M, 0, 256, XT_H * XT_PTS, STEP>(
       k, b, alpha, ia, aline, tid,
@@ -221,7 +225,7 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
         const int r = p / XT_PTS;
         i = k0 - 2 + (p - r * XT_PTS);
         j = j0 + r;
-        return j <= k.ny && i >= -2 && i <= k.nx + 3;
+        return j <= k.ny && i <= k.nx + 3;
       },
       [&](int p, const double *fp, const double *fn, const double *pr) {
         const int r = p / XT_PTS, q = p - r * XT_PTS;
@@ -233,16 +237,30 @@ SWF_D void xflux_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int 
         }
       });
   __syncthreads();
-  const int tx = tid & (XT_W - 1), ty = tid / XT_W;
-  const int kk = k0 + tx, j = j0 + ty;
+  const int tx = tid & 63, ty = tid >> 6;
+  const int kc = k0 + tx, j = j0 + ty;
+  const bool own = kc <= k.nx + 1 && j <= k.ny;
+  double a[4] = {0.0, 0.0, 0.0, 0.0}, c[4] = {0.0, 0.0, 0.0, 0.0};
   int lim = 0;
-  if (kk >= 0 && kk <= k.nx && j <= k.ny) {
-    double F[4];
-    lim = iface_flux<M, 0, 1, CS, LIM, STEP>(k, &sm.pr[0][ty][tx + 2], &sm.fp[0][ty][tx + 2], &sm.fn[0][ty][tx + 2],
-                                       STEP ? aline[j + 2] : alpha, F);
-    const long long o = cidx(k, kk, j);
+  if (own) {
+    centre_fluxes<M, 0, 1, CS, STEP>(k, load_prim<CS>(&sm.pr[0][ty][tx + 2]), load_prim<CS>(&sm.pr[0][ty][tx + 3]),
+                                     &sm.fp[0][ty][tx + 2], &sm.fn[0][ty][tx + 2], a, c);
+    if (LIM) {
+      // the halves fp(0..nx) and fn(1..nx+1) of the row, each counted once (the
+      // tile's last centre by the next tile, unless it ends the row)
+      if (kc <= k.nx) lim |= zs_limit<CS>(k.gamma1, &sm.fp[0][ty][tx + 2], a);
+      if (kc >= 1) lim |= zs_limit<CS>(k.gamma1, &sm.fn[0][ty][tx + 2], c) << 2;
+      if (tx == XT_C && kc != k.nx + 1) lim = 0;
+    }
+  }
+  double cn[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) b.Fx[c * k.plane + o] = F[c];
+  for (int q = 0; q < 4; ++q) cn[q] = __shfl_down(c[q], 1, 64);
+  if (own && tx < XT_C && kc <= k.nx) {
+    const double al = STEP ? aline[j + 2] : alpha;
+    const long long o = cidx(k, kc, j);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b.Fx[q * k.plane + o] = k.alpha_in_scheme ? (a[q] - cn[q]) : (al * a[q] - al * cn[q]);
   }
   if (LIM && stage > 0) zs_store<256>(zs_den(lim), zs_pre(lim), b.zs + 2 * (bx + by * gx), sm.zred, tid);
 }
@@ -506,6 +524,25 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
                                             STEP ? aline[i + 2] : alpha, G);
   // a tile's first interface is the previous tile's last: counted there
   if (ty == 0 && by > 0) lim = 0;
+  // the cell's loads, all issued here, before the exchange and before any store
+  // (at stage 3 the new state is written over U0 itself)
+  const long long o = cidx(k, i, j);
+  double fxo[4], fxm[4], uin[4], u0v[4];
+  if (in) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      fxo[c] = b.Fx[c * k.plane + o];
+      fxm[c] = b.Fx[c * k.plane + o - 1];
+    }
+    if (!rhs_only) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) uin[c] = b.Uin[c * k.plane + o];
+      if (stage > 1) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) u0v[c] = b.U0[c * k.plane + o];
+      }
+    }
+  }
   // G(j-1/2) of cell j = jf: the thread YT_W lanes below, across a wave boundary
   // the previous wave's last row through LDS
   constexpr int RPW = 64 / YT_W;  // rows per wave
@@ -526,12 +563,10 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
   double anew = 0.0;
   double wu[4] = {0.0, 0.0, 0.0, 0.0};
   if (in) {
-    const long long o = cidx(k, i, j);
     double rhs[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const double *fx = b.Fx + c * k.plane;
-      const double X = fx[o] - fx[o - 1];
+      const double X = fxo[c] - fxm[c];
       const double Y = G[c] - Gb[c];
       if (M == MODE_EXACT)
         rhs[c] = k.alpha_in_scheme ? -((alpha * X) / k.dx) - (alpha * Y) / k.dy : -(X / k.dx) - Y / k.dy;
@@ -559,15 +594,15 @@ SWF_D void ystage_tile(const KCfg &k, const Bufs &b, int stage, int slot_in, int
       double v[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const double x = b.Uin[c * k.plane + o] + dt * rhs[c];
+        const double x = uin[c] + dt * rhs[c];
         double un;
         if (stage == 1) {
           un = x;
         } else if (stage == 2) {
-          const double u0 = b.U0[c * k.plane + o];
+          const double u0 = u0v[c];
           un = (M == MODE_EXACT) ? rk2_q(u0, x) : 0.75 * u0 + 0.25 * x;
         } else {
-          const double u0 = b.U0[c * k.plane + o];
+          const double u0 = u0v[c];
           un = (M == MODE_EXACT) ? rk3_q(u0, x) : (u0 + 2.0 * x) * (1.0 / 3.0);
         }
         v[c] = un;

@@ -78,8 +78,8 @@ inline int run1d_threads(int N) {
   return (per + 63) / 64 * 64;
 }
 
-// x-flux tiles: 64 interfaces x 4 rows covering interfaces 0..nx from origin
-// -XT_SHIFT (swf_kernels.h); stage tiles: UB_W x UB_H cells
+// x-flux tiles: 64 centres x 4 rows, XT_C interfaces per row (swf_kernels.h);
+// stage tiles: UB_W x UB_H cells
 inline dim3 grid_xflux(const KCfg &k) { return dim3(xf_gx(k), xf_gy(k)); }
 inline unsigned grid_zs_edges(const KCfg &k) { return (unsigned)((zs_edge_halves(k) + 255) / 256); }
 inline dim3 grid_ystage(const KCfg &k) { return dim3(ub_gx(k), ub_gy(k)); }

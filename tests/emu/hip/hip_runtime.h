@@ -118,6 +118,26 @@ T __shfl_up(T v, unsigned d, int width = 64) {
   return out;
 }
 
+template <class T>
+T __shfl_down(T v, unsigned d, int width = 64) {
+  static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+  const int t = emu::lin_tid();
+  unsigned long long w = 0;
+  memcpy(&w, &v, sizeof(T));
+  emu::shfl_buf[t] = w;
+  emu::wave_sync();
+  const int lane = t & (width - 1);
+  const int nthr = (int)(emu::bDim.x * emu::bDim.y * emu::bDim.z);
+  const int src = (lane + (int)d < width && t + (int)d < nthr) ? t + (int)d : t;
+  unsigned long long r = emu::shfl_buf[src];
+  emu::wave_sync();
+  T out;
+  memcpy(&out, &r, sizeof(T));
+  return out;
+}
+
+// the value of the wave's first lane (every lane active in this build's uses)
+inline int __builtin_amdgcn_readfirstlane(int v) { return __shfl(v, 0); }
 inline int __ffs(unsigned x) { return __builtin_ffs((int)x); }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
 inline long long __double_as_longlong(double d) { long long r; memcpy(&r, &d, 8); return r; }
