@@ -239,6 +239,46 @@ int64_t swf1d_track(swf1d_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos, int6
  * counterpart in the reference: measurement support. */
 int swf1d_last_ms(swf1d_ctx *ctx, double *ms);
 
+/*
+ * The detonation program, ex7_detonation/main.f90, on one domain (cut_x = cut_y = 1),
+ * FP64-exact: a fifth conserved variable rho*Y (unburnt mass fraction), pressure
+ * gamma1*(E - |m|^2/(2 rho) - q_const*rho*Y), the source -k_const*rho*Y*exp(-t_const*rho/p)
+ * in the rho*Y equation, the 5x5 characteristic system in wenofd, per-line alphas, the
+ * step's walls written into u inside wenofd, its bc and its sweep.  Entry points:
+ *   swf7_create / swf7_destroy   the PARAMETER block and arrays (:11-56)
+ *   swf7_upload / swf7_download  u(5,-2:nx+3,-2:ny+3) (:20), Fortran layout
+ *   swf7_bc                      `bc` (:915-1071), as `initdata` ends (:1259)
+ *   swf7_advance                 the time loop `DO WHILE (time <= time_terminal - 1d-10)`
+ *                                with `rk` (:97-102, :810-888)
+ *   swf7_stats / swf7_track      the statistics printed by `output` (:1364-1393)
+ * Return codes as above; SWF_E_ORDER2_EDGE when the program would read its sweep order
+ * array at position 0 (an order-II forward pass flagging the first cell, :1609), which
+ * has no defined result.
+ */
+#define SWF_E_ORDER2_EDGE 4
+
+typedef struct swf7_config {
+  int32_t total;  /* nx = ny = 5*total */
+  int32_t step;   /* the step spans columns 1..total-1 and rows 1..step*total-1 (2) */
+  double dx, dy;  /* xlen/nx, ylen/ny as initdata forms them */
+  double gamma, q_const, t_const, k_const;  /* 1.2, 50, 50, 2566.4 */
+  double eps, eps2, cfl;                    /* 1d-13, 1d-8, 0.88 */
+  int32_t swp_num;                          /* 20000 */
+  int32_t device;
+} swf7_config;
+
+typedef struct swf7_ctx swf7_ctx;
+
+int swf7_create(const swf7_config *cfg, swf7_ctx **out);
+void swf7_destroy(swf7_ctx *ctx);
+int swf7_upload(swf7_ctx *ctx, const double *u);
+int swf7_download(swf7_ctx *ctx, double *u);
+int swf7_bc(swf7_ctx *ctx);
+int swf7_advance(swf7_ctx *ctx, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out);
+int swf7_stats(swf7_ctx *ctx, swf_sweep_stats *out);
+/* per sweep (q, negative pairs, positive neighbours); returns sweep_count */
+int64_t swf7_track(swf7_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap);
+
 int swf_time_continue(double time, double tfinal);
 int swf_sync(swf_ctx *ctx);
 const char *swf_strerror(int code);

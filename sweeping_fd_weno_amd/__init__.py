@@ -15,10 +15,12 @@ from ._lib import (SwfConfig, SwfError, Solver, Batch, lib, library_path, sedov_
                    BC_PERIODIC, BC_JET_HALF, BC_JET_FULL, BC_STEP, step_config, step_inflow, Swf1dConfig, Solver1D, sedov1d_config,
                    state1d_shape)
 from .problems import init_sedov, init_vortex, init_jet, vortex_exact, init_sedov1d, init_step
+from .detonation import Swf7Config, Solver7, det_config, init_detonation
 
 __all__ = [
     "SwfConfig", "SwfError", "Solver", "Batch", "lib", "library_path", "sedov_config", "vortex_config", "jet_config",
     "weno_points", "time_continue", "state_shape", "PRECISIONS", "BC_SEDOV", "BC_PERIODIC", "BC_JET_HALF",
     "BC_JET_FULL", "init_sedov", "init_vortex", "init_jet", "vortex_exact", "Swf1dConfig", "Solver1D",
     "sedov1d_config", "state1d_shape", "init_sedov1d", "BC_STEP", "step_config", "step_inflow", "init_step",
+    "Swf7Config", "Solver7", "det_config", "init_detonation",
 ]

@@ -83,6 +83,7 @@ const char *swf_strerror(int code) {
     case SWF_E_PBAR: return "ERROR: P(u_bar) <= 0.";
     case SWF_E_SWEEP_FAIL: return "SWEEP FAIL: swp_num reached";
     case SWF_E_PSUM: return "Sweep check failed: total pressure increased after adjustment";
+    case SWF_E_ORDER2_EDGE: return "the detonation sweep's order-II pass reads its order array at position 0";
     case SWF_E_INVALID: return "invalid argument";
     case SWF_E_HIP: return "HIP runtime error";
     case SWF_E_NOMEM: return "out of device memory";
@@ -1176,6 +1177,248 @@ int swf1d_last_ms(swf1d_ctx *c, double *ms) {
   if (!c || !ms) return SWF_E_INVALID;
   *ms = c->last_ms;
   return SWF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The detonation program (swf_ex7.h)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kTrackCap7 = 1 << 16;
+}
+
+struct swf7_ctx {
+  swf7_config cfg;
+  ex7::K7 k;
+  ex7::B7 b;       // outside a step: Uin == U == U0 == S0
+  double *S1, *S2;
+  hipStream_t stream;
+  double *pool;    // every device array of the context
+  double *d_fortran;
+  size_t n_fortran;
+  long long steps_total;
+};
+
+static int status7(int st) {
+  switch (st) {
+    case ex7::ST7_OK: return SWF_OK;
+    case ex7::ST7_PBAR: return SWF_E_PBAR;
+    case ex7::ST7_SWEEP_FAIL: return SWF_E_SWEEP_FAIL;
+    case ex7::ST7_PSUM: return SWF_E_PSUM;
+    default: return SWF_E_ORDER2_EDGE;
+  }
+}
+
+static int read7(swf7_ctx *c, ex7::Dev7 *h) {
+  HIPC(hipMemcpyAsync(h, c->b.g, sizeof(ex7::Dev7), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return SWF_OK;
+}
+
+extern "C" {
+
+int swf7_create(const swf7_config *cfg, swf7_ctx **out) {
+  if (!cfg || !out) return SWF_E_INVALID;
+  *out = nullptr;
+  if (cfg->total < 2 || cfg->step < 1 || 5LL * cfg->total * 5LL * cfg->total >= (1LL << 31) ||
+      cfg->step * cfg->total + 3 > 5 * cfg->total || !(cfg->dx > 0) || !(cfg->dy > 0) || cfg->swp_num < 1)
+    return SWF_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
+    snprintf(g_last_error, sizeof(g_last_error), "no HIP device %d (found %d)", cfg->device, ndev);
+    return SWF_E_NODEVICE;
+  }
+  swf7_ctx *c = new (std::nothrow) swf7_ctx();
+  if (!c) return SWF_E_NOMEM;
+  c->cfg = *cfg;
+  ex7::K7 &k = c->k;
+  k.nx = k.ny = 5 * cfg->total;
+  k.P = row_pitch(k.nx);
+  k.plane = (long long)(k.ny + 6) * k.P;
+  k.total = cfg->total;
+  k.step = cfg->step;
+  k.swp_num = cfg->swp_num;
+  k.track_cap = kTrackCap7;
+  k.dx = cfg->dx;
+  k.dy = cfg->dy;
+  k.gamma = cfg->gamma;
+  k.gamma1 = cfg->gamma - 1.;
+  k.eps = cfg->eps;
+  k.eps2 = cfg->eps2;
+  k.q = cfg->q_const;
+  k.tc = cfg->t_const;
+  k.kc = cfg->k_const;
+  k.cfl = cfg->cfl;
+  {
+    // weights(i, j) = 1./(nSteps_x*nSteps_y) with nSteps the REAL(8) grid sizes
+    const double n = (double)k.nx;
+    k.weight = 1. / (n * n);
+  }
+  c->n_fortran = (size_t)ex7::NC * (k.nx + 6) * (k.ny + 6);
+  // 3 states, 4 flux arrays of 5 planes each, alpha, line maxima, the Fortran staging
+  const size_t st = (size_t)ex7::NC * k.plane;
+  const size_t ndbl = 7 * st + (size_t)k.plane + (size_t)(k.nx + k.ny + 12) + c->n_fortran;
+  hipError_t e = hipSetDevice(cfg->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->pool, sizeof(double) * ndbl);
+  ex7::Dev7 *g = nullptr;
+  long long *trk = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&g, sizeof(ex7::Dev7));
+  if (e == hipSuccess) e = hipMalloc(&trk, sizeof(long long) * 3 * kTrackCap7);
+  if (e == hipSuccess) e = hipMemsetAsync(c->pool, 0, sizeof(double) * ndbl, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(g, 0, sizeof(ex7::Dev7), c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    const int rc = (e == hipErrorOutOfMemory) ? SWF_E_NOMEM : hip_fail(e, "swf7_create");
+    if (c->pool) (void)hipFree(c->pool);
+    if (g) (void)hipFree(g);
+    if (trk) (void)hipFree(trk);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
+  double *p = c->pool;
+  c->b.U = c->b.Uin = c->b.U0 = p;
+  c->S1 = p + st;
+  c->S2 = p + 2 * st;
+  c->b.fpf = p + 3 * st;
+  c->b.fnf = p + 4 * st;
+  c->b.gpf = p + 5 * st;
+  c->b.gnf = p + 6 * st;
+  c->b.al = p + 7 * st;
+  c->b.arow = c->b.al + k.plane;
+  c->b.acol = c->b.arow + (k.ny + 6);
+  c->d_fortran = c->b.acol + (k.nx + 6);
+  c->b.g = g;
+  c->b.trk = trk;
+  *out = c;
+  return SWF_OK;
+}
+
+void swf7_destroy(swf7_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->cfg.device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->pool);
+  (void)hipFree(c->b.g);
+  (void)hipFree(c->b.trk);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int swf7_upload(swf7_ctx *c, const double *u) {
+  if (!c || !u) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  HIPC(hipMemcpyAsync(c->d_fortran, u, sizeof(double) * c->n_fortran, hipMemcpyHostToDevice, c->stream));
+  launch7_from_fortran(c->k, c->d_fortran, c->b.U, c->stream);
+  HIPC(hipStreamSynchronize(c->stream));
+  return check_launch("swf7_upload");
+}
+
+int swf7_download(swf7_ctx *c, double *u) {
+  if (!c || !u) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  launch7_to_fortran(c->k, c->b.U, c->d_fortran, c->stream);
+  HIPC(hipMemcpyAsync(u, c->d_fortran, sizeof(double) * c->n_fortran, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return check_launch("swf7_download");
+}
+
+int swf7_bc(swf7_ctx *c) {
+  if (!c) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  launch7_bc(c->k, c->b, c->stream);
+  HIPC(hipStreamSynchronize(c->stream));
+  return check_launch("swf7_bc");
+}
+
+// The time loop: each step's three stages on the device, then the host adds dt to the
+// time (time = time + dt in binary64, :99) and tests the loop condition.
+int swf7_advance(swf7_ctx *c, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out) {
+  if (!c || !time_inout || max_steps < 0) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  double time = *time_inout;
+  int64_t n = 0;
+  ex7::Dev7 h;
+  int rc = read7(c, &h);
+  if (rc) return rc;
+  if (h.status != ex7::ST7_OK) return status7(h.status);
+  h.tfinal = tfinal;
+  h.active = 1;
+  while (n < max_steps && time_continue_q(time, tfinal)) {
+    h.time = time;
+    HIPC(hipMemcpyAsync(c->b.g, &h, sizeof(ex7::Dev7), hipMemcpyHostToDevice, c->stream));
+    for (int s = 1; s <= 3; ++s) {
+      ex7::B7 b = c->b;
+      b.Uin = (s == 1) ? c->b.U : (s == 2) ? c->S1 : c->S2;
+      b.U = (s == 1) ? c->S1 : (s == 2) ? c->S2 : c->b.U;
+      launch7_stage(c->k, b, c->stream, s);
+    }
+    rc = check_launch("swf7_advance");
+    if (rc) return rc;
+    rc = read7(c, &h);
+    if (rc) return rc;
+    if (h.status != ex7::ST7_OK) break;
+    time = time + h.dt;
+    h.steps_done += 1;
+    ++n;
+  }
+  h.active = 0;
+  HIPC(hipMemcpyAsync(c->b.g, &h, sizeof(ex7::Dev7), hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  *time_inout = time;
+  if (steps_out) *steps_out = n;
+  c->steps_total += n;
+  return status7(h.status);
+}
+
+int swf7_stats(swf7_ctx *c, swf_sweep_stats *s) {
+  if (!c || !s) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  ex7::Dev7 h;
+  const int rc = read7(c, &h);
+  if (rc) return rc;
+  *s = swf_sweep_stats();
+  s->sweep_count = h.sweep_count;
+  s->den_count = h.den_count;
+  s->pre_count = h.pre_count;
+  s->stage_sweeps = h.sweep_count;
+  s->steps = c->steps_total;
+  const long long m = h.sweep_count < kTrackCap7 ? h.sweep_count : kTrackCap7;
+  if (m > 0) {
+    std::vector<long long> t(3 * (size_t)m);
+    HIPC(hipMemcpy(t.data(), c->b.trk, sizeof(long long) * 3 * m, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < m; ++i) {
+      s->total_sweep += t[3 * i];
+      s->total_neg += t[3 * i + 1];
+      s->total_pos += t[3 * i + 2];
+      s->max_sweep = i == 0 || t[3 * i] > s->max_sweep ? t[3 * i] : s->max_sweep;
+      s->max_neg = i == 0 || t[3 * i + 1] > s->max_neg ? t[3 * i + 1] : s->max_neg;
+      s->max_pos = i == 0 || t[3 * i + 2] > s->max_pos ? t[3 * i + 2] : s->max_pos;
+    }
+  }
+  return SWF_OK;
+}
+
+int64_t swf7_track(swf7_ctx *c, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap) {
+  if (!c) return SWF_E_INVALID;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SWF_E_HIP;
+  ex7::Dev7 h;
+  if (read7(c, &h)) return SWF_E_HIP;
+  const int64_t n = h.sweep_count < cap ? h.sweep_count : cap;
+  const int64_t m = n < kTrackCap7 ? n : kTrackCap7;
+  if (m > 0 && q && neg && pos) {
+    std::vector<long long> t(3 * (size_t)m);
+    if (hipMemcpy(t.data(), c->b.trk, sizeof(long long) * 3 * m, hipMemcpyDeviceToHost) != hipSuccess)
+      return SWF_E_HIP;
+    for (int64_t i = 0; i < m; ++i) {
+      q[i] = t[3 * i];
+      neg[i] = t[3 * i + 1];
+      pos[i] = t[3 * i + 2];
+    }
+  }
+  return h.sweep_count;
 }
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 
 #include "swf_1d.h"
 #include "swf_device.h"
+#include "swf_ex7.h"
 
 namespace swf {
 
@@ -65,6 +66,14 @@ void launch_step_reflect_batch(const BatchInst *inst, const int2 *map, int n, hi
                                int parity);
 void launch_from_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
 void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_t s);
+
+// the detonation program (swf_ex7.h, compiled in swf_k_exact.hip): one RK stage s
+// (findalpha, walls, both passes, scheme with the source, RK, checkpositivity, the
+// sweep when flagged, bc), the bc alone, and the layout transposes of 5 components
+void launch7_stage(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s, int stage);
+void launch7_bc(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s);
+void launch7_from_fortran(const ex7::K7 &k, const double *src, double *dst, hipStream_t s);
+void launch7_to_fortran(const ex7::K7 &k, const double *src, double *dst, hipStream_t s);
 
 // the 1D time loop (swf_1d.h): one workgroup of nt threads, cpt cells per thread
 // and lds bytes of dynamic LDS; mode 0 in swf_k_exact.hip, 1 and 2 in swf_k_fast.hip

@@ -5,7 +5,7 @@
 ! internal procedures `rk`, `wenofd`, `findalpha`, `checkpositivity`, `sweep_lim`
 ! and `bc` (ex4_sedov_2d/main.f90).  The state array keeps the reference layout
 ! u(4, -2:nx+3, -2:ny+3); the 1D program's (swf1d_*, ex2_sedov_1d/main.f90) keeps
-! u(3, -2:n+3).
+! u(3, -2:n+3), the detonation program's (swf7_*, ex7_detonation/main.f90) u(5, -2:nx+3, -2:ny+3).
 module swf
    use iso_c_binding
    implicit none
@@ -13,7 +13,8 @@ module swf
    integer(c_int32_t), parameter :: SWF_FP64_EXACT = 0, SWF_FP64 = 1, SWF_FP32_BULK = 2
    integer(c_int32_t), parameter :: SWF_BC_SEDOV = 0, SWF_BC_PERIODIC = 1, SWF_BC_JET_HALF = 2, &
                                     SWF_BC_JET_FULL = 3, SWF_BC_STEP = 4
-   integer(c_int), parameter :: SWF_OK = 0, SWF_E_PBAR = 1, SWF_E_SWEEP_FAIL = 2, SWF_E_PSUM = 3
+   integer(c_int), parameter :: SWF_OK = 0, SWF_E_PBAR = 1, SWF_E_SWEEP_FAIL = 2, SWF_E_PSUM = 3, &
+                                SWF_E_ORDER2_EDGE = 4
 
    type, bind(C) :: swf_config
       integer(c_int32_t) :: nx, ny
@@ -33,6 +34,15 @@ module swf
       real(c_double) :: dx, gamma, cfl, eps
       integer(c_int32_t) :: use_limiter, swp_num, precision, device
    end type swf1d_config
+
+   ! the detonation program (swf7_*, ex7_detonation/main.f90), u(5, -2:nx+3, -2:ny+3)
+   type, bind(C) :: swf7_config
+      integer(c_int32_t) :: total, step
+      real(c_double) :: dx, dy
+      real(c_double) :: gamma, q_const, t_const, k_const
+      real(c_double) :: eps, eps2, cfl
+      integer(c_int32_t) :: swp_num, device
+   end type swf7_config
 
    type, bind(C) :: swf_sweep_stats
       integer(c_int64_t) :: sweep_count, den_count, pre_count
@@ -142,6 +152,49 @@ module swf
          integer(c_int64_t), intent(out) :: q(*), neg(*), pos(*)
          integer(c_int64_t), value :: cap
       end function swf1d_track
+
+      integer(c_int) function swf7_create(cfg, ctx) bind(C, name='swf7_create')
+         import :: swf7_config, c_ptr, c_int
+         type(swf7_config), intent(in) :: cfg
+         type(c_ptr), intent(out) :: ctx
+      end function swf7_create
+
+      subroutine swf7_destroy(ctx) bind(C, name='swf7_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end subroutine swf7_destroy
+
+      integer(c_int) function swf7_upload(ctx, u) bind(C, name='swf7_upload')
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+      end function swf7_upload
+
+      integer(c_int) function swf7_download(ctx, u) bind(C, name='swf7_download')
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: u(*)
+      end function swf7_download
+
+      integer(c_int) function swf7_bc(ctx) bind(C, name='swf7_bc')
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+      end function swf7_bc
+
+      integer(c_int) function swf7_advance(ctx, time, tfinal, max_steps, steps) bind(C, name='swf7_advance')
+         import :: c_ptr, c_int, c_double, c_int64_t
+         type(c_ptr), value :: ctx
+         real(c_double), intent(inout) :: time
+         real(c_double), value :: tfinal
+         integer(c_int64_t), value :: max_steps
+         integer(c_int64_t), intent(out) :: steps
+      end function swf7_advance
+
+      integer(c_int) function swf7_stats(ctx, s) bind(C, name='swf7_stats')
+         import :: c_ptr, c_int, swf_sweep_stats
+         type(c_ptr), value :: ctx
+         type(swf_sweep_stats), intent(out) :: s
+      end function swf7_stats
 
       type(c_ptr) function swf_strerror(code) bind(C, name='swf_strerror')
          import :: c_ptr, c_int

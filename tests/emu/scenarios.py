@@ -201,6 +201,15 @@ def main(name):
     if name == "sedov_sweeps":
         cfg = S.sedov_config(32, "fp64_exact")
         out = run_both(cfg, S.init_sedov(cfg), 1.0, 10)
+    elif name.startswith("ex7_"):
+        # the detonation program against its reference run (fixture): fields and statistics
+        from sweeping_fd_weno_amd import detonation as D
+        import hashlib
+        total = int(name[len("ex7_"):])
+        r = D.run(total, 0.2)
+        out = {"steps": r["steps"], "code": r["code"], "stats": r["stats"],
+               "sha256": {k: hashlib.sha256(np.ascontiguousarray(v, dtype="<f8").tobytes()).hexdigest()
+                          for k, v in r["fields"].items()}}
     elif name == "jet_full_sweeps":
         cfg = S.jet_config(64, 32, "fp64_exact", cfl=0.5, half=False)
         out = run_both(cfg, S.init_jet(cfg), 0.001, 30)

@@ -35,7 +35,13 @@ oracle/_ref/, run there, and only its OUTPUTS are stored here:
                           density sweeps) and total = 16 (208 x 176) to T = 0.5: the same
                           records for den.dat / pre.dat.
 
-Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5s] [ex5] [ex4] [ex6]
+  ref_ex7_detonation.json ex7_detonation as one-rank build copies (the same edits, and mpirank
+                          initialised to 0, which the program tests before MPI_Init): total = 8
+                          (40 x 40, 17 steps, 3 sweeps) and total = 16 (80 x 80, 33 steps, 8
+                          sweeps) to T = 0.2: the same records for den.dat / pre.dat / rate.dat,
+                          and the 40 x 40 fields in full.
+
+Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5s] [ex5] [ex4] [ex6] [ex7]
 """
 from __future__ import annotations
 
@@ -334,6 +340,36 @@ def make_ex6() -> None:
         json.dump(out, f, indent=1)
 
 
+def make_ex7() -> None:
+    cases = {
+        "total8": ("ex7_8", 8, 0.2, "total = 8 (40 x 40), time_terminal = 0.2"),
+        "total16": ("ex7_16", 16, 0.2, "total = 16 (80 x 80), time_terminal = 0.2"),
+    }
+    out = {"layout": "value columns of den.dat / pre.dat / rate.dat: (j outer, i inner) over the interior, "
+                     "with the program's own values 100 / 300 / 20 where i <= total and j < step*total; "
+                     "pre = gamma1*(E - 0.5*m1**2/rho - 0.5*m2**2/rho - q_const*rhoY), rate = rhoY/rho",
+           "sample_stride": 41}
+    for key, (build, total, T, desc) in cases.items():
+        rundir, text = _run_mpi1(build)
+        fields = {nm: _tokens_skip(os.path.join(rundir, nm + ".dat"), 3, 1)[:, 2] for nm in ("den", "pre", "rate")}
+        secs, steps, tfin = _timer_line(text)
+        out[key] = {
+            "program": "ex7_detonation, one-rank build copy (use MPI -> include 'mpif.h', cut_x = cut_y = 1, "
+                       "mpirank initialised to 0), " + desc,
+            "total": total, "t_end": T, "steps": steps, "t_printed": tfin, "reference_seconds_1core": secs,
+            "sha256": {k: field_digest(v) for k, v in fields.items()},
+            "sample": {k: v[::41].tolist() for k, v in fields.items()},
+            "stats": _parse_stats(text),
+            "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
+                            for fn in ("den.dat", "pre.dat", "rate.dat")},
+            "printout_minus_timer": _printout_minus_timer(text),
+        }
+        if total == 8:
+            out[key]["fields"] = {k: v.tolist() for k, v in fields.items()}
+    with open(os.path.join(GOLD, "ref_ex7_detonation.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def _tokens_skip(fn: str, ncol: int, skip: int) -> np.ndarray:
     body = open(fn).read().split("\n", skip)[skip]
     return np.array(body.split(), dtype=np.float64).reshape(-1, ncol)
@@ -385,3 +421,5 @@ if __name__ == "__main__":
         make_ex4()
     if "ex6" in what:
         make_ex6()
+    if "ex7" in what:
+        make_ex7()

@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "swf.h")
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(swf(?:1d)?_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(swf(?:1d|7)?_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_header_declares_the_boundary():
@@ -21,7 +21,8 @@ def test_header_declares_the_boundary():
     for n in ("swf_create", "swf_destroy", "swf_upload", "swf_download", "swf_step", "swf_advance", "swf_rhs",
               "swf_alpha_max", "swf_check_positivity", "swf_sweep", "swf_bc", "swf_stats", "swf_track",
               "swf_weno_points", "swf_time_continue", "swf_strerror", "swf1d_create", "swf1d_destroy",
-              "swf1d_upload", "swf1d_download", "swf1d_advance", "swf1d_stats", "swf1d_track"):
+              "swf1d_upload", "swf1d_download", "swf1d_advance", "swf1d_stats", "swf1d_track", "swf7_create",
+              "swf7_destroy", "swf7_upload", "swf7_download", "swf7_bc", "swf7_advance", "swf7_stats", "swf7_track"):
         assert n in names
 
 
@@ -49,6 +50,16 @@ def test_line_config_struct_layout_matches_header():
     names = [f[0] for f in S.Swf1dConfig._fields_]
     text = open(HEADER).read()
     body = text[text.index("typedef struct swf1d_config"):text.index("} swf1d_config;")]
+    decl = re.findall(r"(?:int32_t|double)\s+([^;]+);", body)
+    flat = [v.strip() for d in decl for v in d.split(",")]
+    assert names == flat
+
+
+def test_detonation_config_struct_layout_matches_header():
+    from sweeping_fd_weno_amd.detonation import Swf7Config
+    names = [f[0] for f in Swf7Config._fields_]
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct swf7_config"):text.index("} swf7_config;")]
     decl = re.findall(r"(?:int32_t|double)\s+([^;]+);", body)
     flat = [v.strip() for d in decl for v in d.split(",")]
     assert names == flat

@@ -21,6 +21,8 @@ arithmetic and control flow, not races; the GPU tests check the real device.
   inflow ghosts (alpha ~ 800) in every precision mode.
 - The step diffraction program, 52 x 44 cells for 40 steps from its initial data
   (50 sweeps, 15 density sweeps, 13 negative-neighbour adjustments): bitwise.
+- The detonation program (ex7) at 40 x 40 to T = 0.2 (17 steps, 3 sweeps): den / pre /
+  rate bitwise to the reference's one-rank run, statistics equal.
 - The 1D program (one workgroup, whole time loop): 60 cells for 12 steps and 800
   cells for 40 steps at cfl 1.5 (16 sweeps, 5273 limited pairs) and 30 steps of the
   1D benchmark case: the whole state bitwise, statistics and tracks equal.
@@ -42,7 +44,7 @@ SCENARIOS = ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter
              "crafted_sweep_64_20_300", "crafted_psum_48_400_1", "crafted_psum_48_400_2", "jet_full_sweeps_fp64",
              "jet_full_sweeps_fp32_bulk", "batch_fp64_exact", "batch_fp32_bulk", "first_dt_fp64_exact",
              "first_dt_fp64", "first_dt_fp32_bulk", "line_small", "line_stress_fp64_exact", "line_cfg1",
-             "step_fp64_exact_4_40"]
+             "step_fp64_exact_4_40", "ex7_8"]
 _RESULTS = {}
 
 
@@ -153,3 +155,15 @@ def test_emulated_step_program_matches_oracle(emu_lib):
     for key, (o, e) in r["stats"].items():
         assert o == e, key
     assert r["stats"]["den_count"][0] > 0 and r["stats"]["total_neg"][0] > 0
+
+
+def test_emulated_detonation_matches_reference_run(emu_lib):
+    import json as _json
+    gold = _json.load(open(os.path.join(ROOT, "tests", "golden", "ref_ex7_detonation.json")))["total8"]
+    r = scenario("ex7_8")
+    assert r["code"] == 0 and r["steps"] == gold["steps"]
+    assert r["sha256"] == gold["sha256"]
+    st = gold["stats"]
+    assert (r["stats"]["total_sweep"], r["stats"]["total_neg"], r["stats"]["total_pos"]) == (
+        st["total sweep"], st["total negative pairs"], st["total positive neighbors"])
+    assert (r["stats"]["den_count"], r["stats"]["pre_count"]) == (st["den_count"], st["pre_count"])
