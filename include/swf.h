@@ -252,7 +252,7 @@ int swf1d_last_ms(swf1d_ctx *ctx, double *ms);
  *                                with `rk` (:97-102, :810-888)
  *   swf7_stats / swf7_track      the statistics printed by `output` (:1364-1393)
  * Return codes as above; SWF_E_ORDER2_EDGE when the program would read its sweep order
- * array at position 0 (an order-II forward pass flagging the first cell, :1609), which
+ * array at position 0 (an order-II forward pass flagging the first cell, :1619), which
  * has no defined result.
  */
 #define SWF_E_ORDER2_EDGE 4

@@ -14,9 +14,11 @@
 // launches come before k_stage_end: k_zs_edges<M> (the halves of the ghost rows
 // and columns, counted only) and k_bc (the new state's full ghost frame).
 // Each flux tile stages its stencil points (split fluxes and Roe primitives) in
-// LDS once; every thread then computes one interface: the f+ half with the Roe
-// system of (i, i+1) and the f- half with that of (i+1, i+2), eight WENO
-// reconstructions in all.
+// LDS once.  An x-flux lane computes one centre i: the f+ and f- halves of its
+// stencil with the one Roe system of (i, i+1), eight WENO reconstructions, and
+// the interface flux from the next lane's f-; a stage-kernel thread computes one
+// y interface: the f+ half with the Roe system of (j, j+1) and the f- half with
+// that of (j+1, j+2).
 #pragma once
 
 #include "swf_device.h"
