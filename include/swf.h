@@ -250,6 +250,8 @@ int swf1d_last_ms(swf1d_ctx *ctx, double *ms);
  *   swf7_bc                      `bc` (:915-1071), as `initdata` ends (:1259)
  *   swf7_advance                 the time loop `DO WHILE (time <= time_terminal - 1d-10)`
  *                                with `rk` (:97-102, :810-888)
+ *   swf7_sweep                   `sweep_lim` with `scalar_sweep` (:1433-1806) on the current
+ *                                state, as `rk` calls it when checkpositivity flags a stage
  *   swf7_stats / swf7_track      the statistics printed by `output` (:1364-1393)
  * Return codes as above; SWF_E_ORDER2_EDGE when the program would read its sweep order
  * array at position 0 (an order-II forward pass flagging the first cell, :1619), which
@@ -275,6 +277,7 @@ int swf7_upload(swf7_ctx *ctx, const double *u);
 int swf7_download(swf7_ctx *ctx, double *u);
 int swf7_bc(swf7_ctx *ctx);
 int swf7_advance(swf7_ctx *ctx, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out);
+int swf7_sweep(swf7_ctx *ctx);
 int swf7_stats(swf7_ctx *ctx, swf_sweep_stats *out);
 /* per sweep (q, negative pairs, positive neighbours); returns sweep_count */
 int64_t swf7_track(swf7_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos, int64_t cap);

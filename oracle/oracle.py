@@ -112,6 +112,9 @@ def lib():
         L.swo1d_track.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_int64), P(C.c_int64), C.c_int64]
         L.swo1d_init_sedov.argtypes = [P(Swo1dConfig), C.c_int, C.c_double, dp]
         L.swo1d_run.argtypes = [C.c_void_p, P(C.c_double), C.c_double, C.c_int64, P(C.c_int64)]
+        L.swo7_sweep.argtypes = [dp, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, P(C.c_int64),
+                                 P(C.c_int64)]
+        L.swo7_sweep.restype = C.c_int
         _lib = L
     return _lib
 
@@ -382,3 +385,15 @@ def pressure1d(u: np.ndarray, gamma: float = 1.4) -> np.ndarray:
     """The pre.dat column: gamma1*(E - 0.5*m**2/rho) (:481)."""
     g1 = gamma - 1.0
     return g1 * (u[:, 2] - (0.5 * (u[:, 1] * u[:, 1])) / u[:, 0])
+
+
+def sweep7(u: np.ndarray, total: int, gamma: float = 1.2, q_const: float = 50.0, eps: float = 1e-13,
+           swp_num: int = 20000) -> dict:
+    """The detonation program's sweep_lim on a copy of u (shape (n+6, n+6, 5), [j][i][c],
+    n = 5 total): the swept state, the stop code and the counts."""
+    a = np.ascontiguousarray(u, dtype=np.float64).copy()
+    cnt = (C.c_int64 * 3)()
+    trk = (C.c_int64 * 3)()
+    rc = lib().swo7_sweep(_dp(a), total, gamma, q_const, eps, swp_num, cnt, trk)
+    return {"u": a, "code": rc, "sweep_count": cnt[0], "den_count": cnt[1], "pre_count": cnt[2],
+            "q": trk[0], "neg": trk[1], "pos": trk[2]}

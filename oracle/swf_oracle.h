@@ -134,6 +134,14 @@ int64_t swo1d_track(const swo1d_ctx *ctx, int64_t *q, int64_t *neg, int64_t *pos
 void swo1d_init_sedov(const swo1d_config *cfg, int shipped_energy, double e_amb, double *u);
 int swo1d_run(swo1d_ctx *ctx, double *time_inout, double tfinal, int64_t max_steps, int64_t *steps_out);
 
+/* The detonation program's sweep (ex7_detonation/main.f90 :1399-1862) on its state
+ * u(5, -2:n+3, -2:n+3), n = 5 total (swf_oracle_ex7.c).  counts[0..2] += sweep_count,
+ * den_count, pre_count; track = (q, negative pairs, positive neighbours) of this sweep.
+ * Returns 0, or the stop: 1 P(u_bar) <= 0, 2 SWEEP FAIL, 3 the pressure-sum check,
+ * 4 the order-II pass read before its order arrays. */
+int swo7_sweep(double *u, int32_t total, double gamma, double q_const, double eps, int32_t swp_num,
+               int64_t *counts, int64_t *track);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1264,16 +1264,26 @@ int swf7_create(const swf7_config *cfg, swf7_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&c->pool, sizeof(double) * ndbl);
   ex7::Dev7 *g = nullptr;
   long long *trk = nullptr;
+  unsigned *bm = nullptr;
+  int *lst = nullptr;
+  const long long npos = (long long)k.nx * k.ny;
   if (e == hipSuccess) e = hipMalloc(&g, sizeof(ex7::Dev7));
   if (e == hipSuccess) e = hipMalloc(&trk, sizeof(long long) * 3 * kTrackCap7);
+  if (e == hipSuccess) e = hipMalloc(&bm, sizeof(unsigned) * ((npos + 31) / 32));
+  if (e == hipSuccess) e = hipMalloc(&lst, sizeof(int) * npos);
   if (e == hipSuccess) e = hipMemsetAsync(c->pool, 0, sizeof(double) * ndbl, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(g, 0, sizeof(ex7::Dev7), c->stream);
+  // the program's sweep_track / neg_track / pos_track start at 0; a sweep that stops
+  // leaves its entry there
+  if (e == hipSuccess) e = hipMemsetAsync(trk, 0, sizeof(long long) * 3 * kTrackCap7, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
     const int rc = (e == hipErrorOutOfMemory) ? SWF_E_NOMEM : hip_fail(e, "swf7_create");
     if (c->pool) (void)hipFree(c->pool);
     if (g) (void)hipFree(g);
     if (trk) (void)hipFree(trk);
+    if (bm) (void)hipFree(bm);
+    if (lst) (void)hipFree(lst);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return rc;
@@ -1292,6 +1302,8 @@ int swf7_create(const swf7_config *cfg, swf7_ctx **out) {
   c->d_fortran = c->b.acol + (k.nx + 6);
   c->b.g = g;
   c->b.trk = trk;
+  c->b.bm = bm;
+  c->b.lst = lst;
   *out = c;
   return SWF_OK;
 }
@@ -1303,6 +1315,8 @@ void swf7_destroy(swf7_ctx *c) {
   (void)hipFree(c->pool);
   (void)hipFree(c->b.g);
   (void)hipFree(c->b.trk);
+  (void)hipFree(c->b.bm);
+  (void)hipFree(c->b.lst);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1370,6 +1384,25 @@ int swf7_advance(swf7_ctx *c, double *time_inout, double tfinal, int64_t max_ste
   *time_inout = time;
   if (steps_out) *steps_out = n;
   c->steps_total += n;
+  return status7(h.status);
+}
+
+// sweep_lim on the current state: the positivity flag set, the sweep kernel alone
+int swf7_sweep(swf7_ctx *c) {
+  if (!c) return SWF_E_INVALID;
+  HIPC(hipSetDevice(c->cfg.device));
+  ex7::Dev7 h;
+  int rc = read7(c, &h);
+  if (rc) return rc;
+  if (h.status != ex7::ST7_OK) return status7(h.status);
+  h.postag = 1;
+  h.active = 1;
+  HIPC(hipMemcpyAsync(c->b.g, &h, sizeof(ex7::Dev7), hipMemcpyHostToDevice, c->stream));
+  launch7_sweep(c->k, c->b, c->stream);
+  rc = check_launch("swf7_sweep");
+  if (rc) return rc;
+  rc = read7(c, &h);
+  if (rc) return rc;
   return status7(h.status);
 }
 

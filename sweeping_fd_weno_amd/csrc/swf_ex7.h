@@ -11,9 +11,10 @@
 // Every operation follows the reference's order; the binary128 sites (the Roe weight and
 // sound speed with 1d-15, the 1d-12 of its weno5.f90, the RK combinations, the loop
 // test) use the double-double emulation of swf_dd.h / swf_q128.h, and exp is the host
-// C library's algorithm (glibc 2.35, x86-64 with FMA; exp_glibc below).  The kernels are
-// plain one-thread-per-item kernels: the program is not on the benchmark path, and
-// bit-identical results are the contract.
+// C library's algorithm (glibc 2.35, x86-64 with FMA; exp_glibc below).  The grid
+// kernels are one thread per item; the sweep is one workgroup that finds each pass's
+// flagged positions in parallel and walks the chains they start in the reference's
+// order (k7_sweep).  Bit-identical results are the contract.
 #pragma once
 
 #include "swf_device.h"
@@ -48,6 +49,8 @@ struct B7 {
   double *arow, *acol;         // its maxima per row (arow[j + 2]) and per column (acol[i + 2])
   Dev7 *g;
   long long *trk;              // 3 per sweep: q, negative pairs, positive neighbours
+  unsigned *bm;                // the sweep: flags of one pass, one bit per position
+  int *lst;                    // the sweep: flagged positions of one pass, ascending
 };
 
 enum { ST7_OK = 0, ST7_PBAR = 1, ST7_SWEEP_FAIL = 2, ST7_PSUM = 3, ST7_ORDER2_EDGE = 4 };
@@ -470,34 +473,47 @@ __global__ __launch_bounds__(256) void k7_alpha(K7 k, B7 b) {
   b.al[c7(k, i, j)] = alpha7(k, v);
 }
 
-// alpha_local of each row (x-pass) and column (y-pass); thread t < ny+6 a row, the
-// others a column
+// alpha_local of each row (x-pass) and column (y-pass), one wave per line: lines
+// 0..ny+5 the rows, the others the columns.  The maximum of non-negative alphas does
+// not depend on the order of the comparisons (a NaN never replaces the running value,
+// as in the reference's serial max).
 __global__ __launch_bounds__(256) void k7_lines(K7 k, B7 b) {
   if (!live7(b)) return;
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int line = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int nr = k.ny + 6, ncl = k.nx + 6;
-  if (t < nr) {
-    const int j = t - 2;
-    double a = 0.;
-    for (int i = -2; i <= k.nx + 3; ++i) a = fmax2(a, b.al[c7(k, i, j)]);
-    b.arow[t] = a;
-  } else if (t < nr + ncl) {
-    const int i = t - nr - 2;
-    double a = 0.;
-    for (int j = -2; j <= k.ny + 3; ++j) a = fmax2(a, b.al[c7(k, i, j)]);
-    b.acol[i + 2] = a;
+  if (line >= nr + ncl) return;  // per wave
+  double a = 0.;
+  if (line < nr) {
+    const int j = line - 2;
+    for (int i = -2 + lane; i <= k.nx + 3; i += 64) a = fmax2(a, b.al[c7(k, i, j)]);
+  } else {
+    const int i = line - nr - 2;
+    for (int j = -2 + lane; j <= k.ny + 3; j += 64) a = fmax2(a, b.al[c7(k, i, j)]);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) a = fmax2(a, __shfl_xor(a, off, 64));
+  if (lane == 0) {
+    if (line < nr)
+      b.arow[line] = a;
+    else
+      b.acol[line - nr] = a;
   }
 }
 
-// stage 1: alphamax, dt (rk :816-821) and the time after the step
-__global__ void k7_dt(K7 k, B7 b) {
+// stage 1: alphamax, dt (rk :816-821) and the time after the step; one wave
+__global__ __launch_bounds__(64) void k7_dt(K7 k, B7 b) {
   if (!live7(b)) return;
   Dev7 *g = b.g;
+  const int lane = threadIdx.x;
   double a = 0.;
-  for (int j = 0; j < k.ny + 6; ++j) a = fmax2(a, b.arow[j]);
-  g->amax_bits = dbits(a);
-  const double dtx = (k.dx * k.cfl) / a, dty = (k.dy * k.cfl) / a;
-  g->dt = fmin2(fmin2(dtx, dty), g->tfinal - g->time);
+  for (int j = lane; j < k.ny + 6; j += 64) a = fmax2(a, b.arow[j]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) a = fmax2(a, __shfl_xor(a, off, 64));
+  if (lane == 0) {
+    g->amax_bits = dbits(a);
+    const double dtx = (k.dx * k.cfl) / a, dty = (k.dy * k.cfl) / a;
+    g->dt = fmin2(fmin2(dtx, dty), g->tfinal - g->time);
+  }
 }
 
 // the walls inside wenofd: before the x-pass (dir 0) the three cells left of x =
@@ -663,60 +679,140 @@ SWF_HD void cell_II(const K7 &k, long long p, int &i, int &j) {
   j = (i & 1) ? r + 1 : k.ny - r;
 }
 
-__global__ void k7_sweep(K7 k, B7 b) {
-  if (!live7(b)) return;
-  Dev7 *g = b.g;
-  if (!g->postag) return;
-  double *U = b.U;
+// The sweep in one workgroup of SW7_T threads.  In each pass, the adjustment at
+// position p (when the cell there is flagged) changes only the cells of positions p
+// and p + 1 (p - 1 in a backward pass); in the order-II passes component 5 is changed
+// at the order-I positions p and p - 1, which no later position of the pass reads.
+// So a later position's flag can change only through the chain that reaches it from
+// its predecessor, and every other position is flagged at its turn exactly when it
+// was at the start of the pass.  A pass therefore (1) finds the positions flagged at
+// its start, all threads over contiguous words of a bitmap, then an ordered list by
+// a block scan, and (2) one thread walks the chains they start, in the reference's
+// order, re-testing each chain's next position on the current state.  The minima of
+// the density and pressure (the density sweep's test and the convergence test) are
+// folded over contiguous segments and combined in order, with the reference's
+// comparison (fmin2(x, m): a NaN replaces the running value, the next value replaces
+// the NaN), so they equal its serial loops bit for bit for any input.  u_bar is the
+// reference's serial sum, staged through LDS in chunks.
+constexpr int SW7_T = 1024;  // threads of the sweep workgroup
+constexpr int SW7_CH = 512;  // cells per u_bar chunk
+
+struct Sweep7Shared {
+  double ubc[NC][SW7_CH];
+  double pm[2][SW7_T];        // per-thread minima of a segment
+  unsigned char pf[2][SW7_T];  // 1: the segment has values, 2: one of them is a NaN
+  int wscan[SW7_T / 64];
+  int total;
+  int stop;
+  double ub[NC];
+};
+
+// one value into a segment's running minimum (started at +inf)
+SWF_D void min_push(double &m, unsigned char &f, double x) {
+  f |= (unsigned char)(1 | (x != x ? 2 : 0));
+  m = fmin2(x, m);
+}
+
+// the reference's serial minimum from 1e20 over the segments 0..SW7_T-1 in order
+SWF_D double min_combine(const double *pm, const unsigned char *pf) {
+  double acc = 1e20;
+  for (int t = 0; t < SW7_T; ++t) {
+    if (!(pf[t] & 1)) continue;
+    if ((pf[t] & 2) || acc != acc)
+      acc = pm[t];
+    else
+      acc = fmin2(pm[t], acc);
+  }
+  return acc;
+}
+
+// exclusive prefix sum over the workgroup in thread order; total to sh.total
+SWF_D int scan7(int x, Sweep7Shared &sh, int tid) {
+  int incl = x;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if ((tid & 63) >= off) incl += v;
+  }
+  if ((tid & 63) == 63) sh.wscan[tid >> 6] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int q = 0; q < SW7_T / 64; ++q) {
+      const int t = sh.wscan[q];
+      sh.wscan[q] = run;
+      run += t;
+    }
+    sh.total = run;
+  }
+  __syncthreads();
+  return sh.wscan[tid >> 6] + incl - x;
+}
+
+enum { PASS_DEN = 0, PASS_I = 1, PASS_II = 2 };
+
+// the cell's flag at position p of a pass of this kind, on the current state
+SWF_D bool flag7(const K7 &k, const double *U, int kind, long long p) {
+  int i, j;
+  if (kind == PASS_II)
+    cell_II(k, p, i, j);
+  else
+    cell_I(k, p, i, j);
+  const long long o = c7(k, i, j);
+  if (kind == PASS_DEN) return U[o] < 1e-12;
+  double v[NC];
+  for (int c = 0; c < 4; ++c) v[c] = U[c * k.plane + o];
+  if (kind == PASS_II) {
+    int i5, j5;
+    cell_I(k, p, i5, j5);
+    v[4] = U[4 * k.plane + c7(k, i5, j5)];
+  } else {
+    v[4] = U[4 * k.plane + o];
+  }
+  return pres_flag7(k, v) <= k.eps || v[0] <= k.eps;
+}
+
+// Step 1 of a pass: the positions 1..N flagged on the current state, ascending, into
+// b.lst; their number in sh.total.
+SWF_D void flagged_list(const K7 &k, const B7 &b, int kind, Sweep7Shared &sh, int tid) {
   const long long N = (long long)k.nx * k.ny;
-  auto at = [&](int c, int i, int j) -> double & { return U[c * k.plane + c7(k, i, j)]; };
-  // scalar_sweep: the density sweep when the smallest density is below delta
-  {
-    double rmin = 1e20;
-    for (int j = 1; j <= k.ny; ++j)
-      for (int i = 1; i <= k.nx; ++i) rmin = fmin2(at(0, i, j), rmin);
-    const double delta = 1e-12;
-    if (rmin < delta) {
-      for (long long p = 1; p <= N - 1; ++p) {
-        int i, j, ip, jp;
-        cell_I(k, p, i, j);
-        cell_I(k, p + 1, ip, jp);
-        const double vi = at(0, i, j), vip = at(0, ip, jp);
-        if (vi < delta) {
-          at(0, ip, jp) = vip + (k.weight / k.weight) * (vi - delta);
-          at(0, i, j) = delta;
-        }
-      }
-      for (long long p = N; p >= 2; --p) {
-        int i, j, im, jm;
-        cell_I(k, p, i, j);
-        cell_I(k, p - 1, im, jm);
-        const double vi = at(0, i, j), vim = at(0, im, jm);
-        if (vi < delta) {
-          at(0, im, jm) = vim + (k.weight / k.weight) * (vi - delta);
-          at(0, i, j) = delta;
-        }
-      }
-      g->den_count++;
+  const int nw = (int)((N + 31) / 32), wpt = (nw + SW7_T - 1) / SW7_T;
+  const int w0 = tid * wpt, w1 = min(nw, w0 + wpt);
+  int cnt = 0;
+  for (int w = w0; w < w1; ++w) {
+    unsigned bits = 0;
+    for (int bit = 0; bit < 32; ++bit) {
+      const long long p = 32LL * w + bit + 1;
+      if (p <= N && flag7(k, b.U, kind, p)) bits |= 1u << bit;
+    }
+    b.bm[w] = bits;
+    cnt += __popc(bits);
+  }
+  int o = scan7(cnt, sh, tid);
+  for (int w = w0; w < w1; ++w) {
+    unsigned bits = b.bm[w];
+    while (bits) {
+      const int bit = __ffs(bits) - 1;
+      bits &= bits - 1;
+      b.lst[o++] = 32 * w + bit + 1;
     }
   }
-  // u_bar in the reference's order (j outer, i inner)
-  double ub[NC] = {0., 0., 0., 0., 0.};
-  for (int j = 1; j <= k.ny; ++j)
-    for (int i = 1; i <= k.nx; ++i)
-      for (int c = 0; c < NC; ++c) ub[c] = ub[c] + k.weight * at(c, i, j);
-  const double pbar =
-      k.gamma1 * (((ub[3] - (0.5 * (ub[1] * ub[1])) / ub[0]) - (0.5 * (ub[2] * ub[2])) / ub[0]) - k.q * ub[4]);
-  if (pbar <= 0.) {
-    g->status = ST7_PBAR;
-    g->pbar = pbar;
-    return;
-  }
-  g->sweep_count++;
-  SweepCounts cnt{0, 0};
-  // one adjustment of the pair (cell a = (ia, ja), neighbour p = (ip, jp)); in order II
-  // component 5 is read and written at the order-I positions given (a5, p5)
-  auto adjust = [&](int ia, int ja, int ip, int jp, int ia5, int ja5, int ip5, int jp5, bool check) -> bool {
+  __syncthreads();
+}
+
+// thread 0's part of the sweep: the adjustments of sweep_lim / scalar_sweep
+struct Walk7 {
+  const K7 &k;
+  const B7 &b;
+  double ub[NC];
+  SweepCounts cnt;
+  int q;
+
+  SWF_D double &at(int c, int i, int j) const { return b.U[c * k.plane + c7(k, i, j)]; }
+
+  // one adjustment of the pair (cell a, neighbour p); component 5 at (a5, p5).
+  // 0: the cell is not flagged, 1: adjusted, 2: the pressure-sum check failed
+  SWF_D int adjust(int ia, int ja, int ip, int jp, int ia5, int ja5, int ip5, int jp5, bool check) {
     double vi[NC], vp[NC], vit[NC], vpt[NC];
     for (int c = 0; c < 4; ++c) {
       vi[c] = at(c, ia, ja);
@@ -724,7 +820,7 @@ __global__ void k7_sweep(K7 k, B7 b) {
     }
     vi[4] = at(4, ia5, ja5);
     const double pvi = pres_flag7(k, vi);
-    if (!(pvi <= k.eps || vi[0] <= k.eps)) return true;
+    if (!(pvi <= k.eps || vi[0] <= k.eps)) return 0;
     vp[4] = at(4, ip5, jp5);
     nod_lim7(k, vi, vp, ub, vit, cnt);
     for (int c = 0; c < NC; ++c) {
@@ -737,86 +833,209 @@ __global__ void k7_sweep(K7 k, B7 b) {
     at(4, ip5, jp5) = vpt[4];
     if (check) {
       const double pvp = pres_flag7(k, vp), pvit = pres_sq7(k, vit), pvpt = pres_sq7(k, vpt);
-      if ((pvi + pvp) > (pvit + pvpt) + k.eps) return false;
+      if ((pvi + pvp) > (pvit + pvpt) + k.eps) return 2;
     }
-    return true;
-  };
-  for (int q = 1; q <= k.swp_num; ++q) {
-    if (q & 1) {
-      for (long long p = 1; p <= N - 1; ++p) {
-        int i, j, ip, jp;
-        cell_I(k, p, i, j);
-        cell_I(k, p + 1, ip, jp);
-        if (!adjust(i, j, ip, jp, i, j, ip, jp, true)) {
-          g->status = ST7_PSUM;
-          g->fail_q = q;
-          return;
-        }
+    return 1;
+  }
+
+  // position p of a pass (fwd: neighbour p + 1, else p - 1): 0 not flagged, 1 adjusted,
+  // 2 stopped (status set)
+  SWF_D int at_position(int kind, bool fwd, long long p) {
+    int i, j, in, jn;
+    if (kind == PASS_DEN) {
+      cell_I(k, p, i, j);
+      cell_I(k, fwd ? p + 1 : p - 1, in, jn);
+      const double vi = at(0, i, j), vn = at(0, in, jn);
+      if (!(vi < 1e-12)) return 0;
+      at(0, in, jn) = vn + (k.weight / k.weight) * (vi - 1e-12);
+      at(0, i, j) = 1e-12;
+      return 1;
+    }
+    if (kind == PASS_I) {
+      cell_I(k, p, i, j);
+      cell_I(k, fwd ? p + 1 : p - 1, in, jn);
+      const int r = adjust(i, j, in, jn, i, j, in, jn, true);
+      if (r == 2) {
+        b.g->status = ST7_PSUM;
+        b.g->fail_q = q;
       }
-      for (long long p = N; p >= 2; --p) {
-        int i, j, ip, jp;
-        cell_I(k, p, i, j);
-        cell_I(k, p - 1, ip, jp);
-        if (!adjust(i, j, ip, jp, i, j, ip, jp, true)) {
-          g->status = ST7_PSUM;
-          g->fail_q = q;
-          return;
+      return r;
+    }
+    // order II; component 5 at the order-I positions p (cell) and p - 1 (neighbour).  At
+    // p = 1 of the forward pass that neighbour is position 0, outside the order arrays:
+    // not reproducible, stop.
+    int i5, j5, in5 = 0, jn5 = 0;
+    cell_II(k, p, i, j);
+    cell_II(k, fwd ? p + 1 : p - 1, in, jn);
+    cell_I(k, p, i5, j5);
+    if (fwd) {
+      double v[NC];
+      for (int c = 0; c < 4; ++c) v[c] = at(c, i, j);
+      v[4] = at(4, i5, j5);
+      const bool flagged = pres_flag7(k, v) <= k.eps || v[0] <= k.eps;
+      if (!flagged) return 0;
+      if (p == 1) {
+        b.g->status = ST7_ORDER2_EDGE;
+        b.g->fail_q = q;
+        return 2;
+      }
+    }
+    cell_I(k, p - 1, in5, jn5);
+    return adjust(i, j, in, jn, i5, j5, in5, jn5, false);
+  }
+
+  // step 2 of a pass: the chains from the listed positions, in the pass's order;
+  // false when the pass stopped the sweep
+  SWF_D bool walk(int kind, bool fwd, int L) {
+    const long long N = (long long)k.nx * k.ny;
+    if (fwd) {
+      long long cur = 1;  // positions below cur are done
+      for (int idx = 0; idx < L; ++idx) {
+        long long p = b.lst[idx];
+        if (p < cur) continue;
+        if (p > N - 1) break;
+        while (true) {
+          const int r = at_position(kind, true, p);
+          if (r == 2) return false;
+          if (r == 0 || p + 1 > N - 1) {
+            cur = p + 1;
+            break;
+          }
+          ++p;
         }
       }
     } else {
-      // order II; component 5 at the order-I positions p (cell) and p - 1 (neighbour),
-      // as the reference indexes it (:1602-1727).  At p = 1 of the forward pass that
-      // neighbour is position 0, outside the order arrays: not reproducible, stop.
-      for (long long p = 1; p <= N - 1; ++p) {
-        int i, j, ip, jp, i5, j5, ip5 = 0, jp5 = 0;
-        cell_II(k, p, i, j);
-        cell_II(k, p + 1, ip, jp);
-        cell_I(k, p, i5, j5);
-        double v[NC];
-        for (int c = 0; c < 4; ++c) v[c] = at(c, i, j);
-        v[4] = at(4, i5, j5);
-        const bool flagged = pres_flag7(k, v) <= k.eps || v[0] <= k.eps;
-        if (flagged && p == 1) {
-          g->status = ST7_ORDER2_EDGE;
-          g->fail_q = q;
-          return;
+      long long cur = N;  // positions above cur are done
+      for (int idx = L - 1; idx >= 0; --idx) {
+        long long p = b.lst[idx];
+        if (p > cur) continue;
+        if (p < 2) break;
+        while (true) {
+          const int r = at_position(kind, false, p);
+          if (r == 2) return false;
+          if (r == 0 || p - 1 < 2) {
+            cur = p - 1;
+            break;
+          }
+          --p;
         }
-        if (flagged) cell_I(k, p - 1, ip5, jp5);
-        (void)adjust(i, j, ip, jp, i5, j5, ip5, jp5, false);
-      }
-      for (long long p = N; p >= 2; --p) {
-        int i, j, ip, jp, i5, j5, ip5, jp5;
-        cell_II(k, p, i, j);
-        cell_II(k, p - 1, ip, jp);
-        cell_I(k, p, i5, j5);
-        cell_I(k, p - 1, ip5, jp5);
-        (void)adjust(i, j, ip, jp, i5, j5, ip5, jp5, false);
       }
     }
+    return true;
+  }
+};
+
+// one pass of every thread; false when it stopped the sweep
+SWF_D bool pass7(const K7 &k, const B7 &b, Walk7 &w, int kind, bool fwd, Sweep7Shared &sh, int tid) {
+  flagged_list(k, b, kind, sh, tid);
+  if (tid == 0) sh.stop = w.walk(kind, fwd, sh.total) ? 0 : 1;
+  __syncthreads();
+  return !sh.stop;
+}
+
+__global__ __launch_bounds__(SW7_T) void k7_sweep(K7 k, B7 b) {
+  __shared__ Sweep7Shared sh;
+  if (!live7(b)) return;
+  Dev7 *g = b.g;
+  if (!g->postag) return;
+  const int tid = threadIdx.x;
+  const long long N = (long long)k.nx * k.ny;
+  const long long seg = (N + SW7_T - 1) / SW7_T, q0 = tid * seg, q1 = (q0 + seg < N) ? q0 + seg : N;
+  const double *U = b.U;
+  Walk7 w{k, b, {0., 0., 0., 0., 0.}, {0, 0}, 0};
+  // scalar_sweep: the density sweep when the smallest density is below delta
+  {
+    double m = __builtin_inf();
+    unsigned char f = 0;
+    for (long long q = q0; q < q1; ++q) {
+      const int i = (int)(q % k.nx) + 1, j = (int)(q / k.nx) + 1;
+      min_push(m, f, U[c7(k, i, j)]);
+    }
+    sh.pm[0][tid] = m;
+    sh.pf[0][tid] = f;
+    __syncthreads();
+    if (tid == 0) sh.stop = min_combine(sh.pm[0], sh.pf[0]) < 1e-12 ? 1 : 0;
+    __syncthreads();
+    if (sh.stop) {
+      pass7(k, b, w, PASS_DEN, true, sh, tid);
+      pass7(k, b, w, PASS_DEN, false, sh, tid);
+      if (tid == 0) g->den_count++;
+    }
+  }
+  // u_bar in the reference's order (j outer, i inner), one thread per component
+  double ubs = 0.;
+  for (long long c0 = 0; c0 < N; c0 += SW7_CH) {
+    const int n = (int)((N - c0 < SW7_CH) ? N - c0 : SW7_CH);
+    for (int t = tid; t < NC * n; t += SW7_T) {
+      const int c = t / n, e = t - c * n;
+      const long long q = c0 + e;
+      const int i = (int)(q % k.nx) + 1, j = (int)(q / k.nx) + 1;
+      sh.ubc[c][e] = U[c * k.plane + c7(k, i, j)];
+    }
+    __syncthreads();
+    if (tid < NC)
+      for (int e = 0; e < n; ++e) ubs = ubs + k.weight * sh.ubc[tid][e];
+    __syncthreads();
+  }
+  if (tid < NC) sh.ub[tid] = ubs;
+  __syncthreads();
+  if (tid == 0) {
+    const double *ub = sh.ub;
+    const double pbar =
+        k.gamma1 * (((ub[3] - (0.5 * (ub[1] * ub[1])) / ub[0]) - (0.5 * (ub[2] * ub[2])) / ub[0]) - k.q * ub[4]);
+    sh.stop = 0;
+    if (pbar <= 0.) {
+      g->status = ST7_PBAR;
+      g->pbar = pbar;
+      sh.stop = 1;
+    } else {
+      g->sweep_count++;
+    }
+  }
+  __syncthreads();
+  if (sh.stop) return;
+  for (int c = 0; c < NC; ++c) w.ub[c] = sh.ub[c];
+  for (int q = 1; q <= k.swp_num; ++q) {
+    w.q = q;
+    const int kind = (q & 1) ? PASS_I : PASS_II;
+    if (!pass7(k, b, w, kind, true, sh, tid)) return;
+    if (!pass7(k, b, w, kind, false, sh, tid)) return;
     // convergence: the pressure without the reaction term, and the density
-    double rmin = 1e20, pmin = 1e20;
-    for (int j = 1; j <= k.ny; ++j)
-      for (int i = 1; i <= k.nx; ++i) {
-        const double r = at(0, i, j), m1 = at(1, i, j), m2 = at(2, i, j), e = at(3, i, j);
-        pmin = fmin2(k.gamma1 * (e - (0.5 * (m1 * m1 + m2 * m2)) / r), pmin);
-        rmin = fmin2(r, rmin);
-      }
-    g->pre_count++;
-    if (pmin > k.eps && rmin > k.eps) {
-      const long long s = g->sweep_count - 1;
-      if (s < k.track_cap) {
-        b.trk[3 * s] = q;
-        b.trk[3 * s + 1] = cnt.neg;
-        b.trk[3 * s + 2] = cnt.pos;
-      }
-      g->postag = 0;
-      return;
+    double pm = __builtin_inf(), rm = __builtin_inf();
+    unsigned char pf = 0, rf = 0;
+    for (long long t = q0; t < q1; ++t) {
+      const int i = (int)(t % k.nx) + 1, j = (int)(t / k.nx) + 1;
+      const long long o = c7(k, i, j);
+      const double r = U[o], m1 = U[k.plane + o], m2 = U[2 * k.plane + o], e = U[3 * k.plane + o];
+      min_push(pm, pf, k.gamma1 * (e - (0.5 * (m1 * m1 + m2 * m2)) / r));
+      min_push(rm, rf, r);
     }
-    if (q == k.swp_num) {
-      g->status = ST7_SWEEP_FAIL;
-      g->fail_q = q;
-      return;
+    sh.pm[0][tid] = pm;
+    sh.pf[0][tid] = pf;
+    sh.pm[1][tid] = rm;
+    sh.pf[1][tid] = rf;
+    __syncthreads();
+    if (tid == 0) {
+      const double pmin = min_combine(sh.pm[0], sh.pf[0]), rmin = min_combine(sh.pm[1], sh.pf[1]);
+      g->pre_count++;
+      sh.stop = 0;
+      if (pmin > k.eps && rmin > k.eps) {
+        const long long s = g->sweep_count - 1;
+        if (s < k.track_cap) {
+          b.trk[3 * s] = q;
+          b.trk[3 * s + 1] = w.cnt.neg;
+          b.trk[3 * s + 2] = w.cnt.pos;
+        }
+        g->postag = 0;
+        sh.stop = 1;
+      } else if (q == k.swp_num) {
+        g->status = ST7_SWEEP_FAIL;
+        g->fail_q = q;
+        sh.stop = 1;
+      }
     }
+    __syncthreads();
+    if (sh.stop) return;
   }
 }
 

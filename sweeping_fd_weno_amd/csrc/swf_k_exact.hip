@@ -227,8 +227,8 @@ void launch7_stage(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s, int stage)
   const dim3 blk(64, 4);
   // wenofd: findalpha and the line alphas, the x walls and pass, the y walls and pass
   hipLaunchKernelGGL(k7_alpha, frame, blk, 0, s, k, b);
-  hipLaunchKernelGGL(k7_lines, dim3((k.nx + k.ny + 12 + 255) / 256), dim3(256), 0, s, k, b);
-  if (stage == 1) hipLaunchKernelGGL(k7_dt, dim3(1), dim3(1), 0, s, k, b);
+  hipLaunchKernelGGL(k7_lines, dim3((k.nx + k.ny + 12 + 3) / 4), dim3(256), 0, s, k, b);
+  if (stage == 1) hipLaunchKernelGGL(k7_dt, dim3(1), dim3(64), 0, s, k, b);
   const int nw0 = 3 * ((k.ny + 1 < k.step * k.total ? k.ny + 1 : k.step * k.total) + 1);
   hipLaunchKernelGGL(k7_walls, dim3((nw0 + 255) / 256), dim3(256), 0, s, k, b, 0);
   hipLaunchKernelGGL(k7_flux<0>, dim3(((k.nx + 2) * (k.ny + 2) + 127) / 128), dim3(128), 0, s, k, b);
@@ -237,11 +237,14 @@ void launch7_stage(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s, int stage)
   // scheme, RK, checkpositivity; the sweep when flagged; bc
   hipLaunchKernelGGL(k7_clear, dim3(1), dim3(1), 0, s, b);
   hipLaunchKernelGGL(k7_stage, cells, blk, 0, s, k, b, stage);
-  hipLaunchKernelGGL(k7_sweep, dim3(1), dim3(1), 0, s, k, b);
+  hipLaunchKernelGGL(k7_sweep, dim3(1), dim3(SW7_T), 0, s, k, b);
   hipLaunchKernelGGL(k7_bc, frame, blk, 0, s, k, b, 0);
 }
 void launch7_bc(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s) {
   hipLaunchKernelGGL(ex7::k7_bc, dim3((k.nx + 6 + 63) / 64, (k.ny + 6 + 3) / 4), dim3(64, 4), 0, s, k, b, 1);
+}
+void launch7_sweep(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s) {
+  hipLaunchKernelGGL(ex7::k7_sweep, dim3(1), dim3(ex7::SW7_T), 0, s, k, b);
 }
 void launch7_from_fortran(const ex7::K7 &k, const double *src, double *dst, hipStream_t s) {
   hipLaunchKernelGGL(ex7::k7_from_fortran, dim3(256), dim3(256), 0, s, k, src, dst);

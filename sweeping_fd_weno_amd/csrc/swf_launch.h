@@ -69,9 +69,11 @@ void launch_to_fortran(const KCfg &k, const double *src, double *dst, hipStream_
 
 // the detonation program (swf_ex7.h, compiled in swf_k_exact.hip): one RK stage s
 // (findalpha, walls, both passes, scheme with the source, RK, checkpositivity, the
-// sweep when flagged, bc), the bc alone, and the layout transposes of 5 components
+// sweep when flagged, bc), the bc alone, the sweep alone (when postag is set), and the
+// layout transposes of 5 components
 void launch7_stage(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s, int stage);
 void launch7_bc(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s);
+void launch7_sweep(const ex7::K7 &k, const ex7::B7 &b, hipStream_t s);
 void launch7_from_fortran(const ex7::K7 &k, const double *src, double *dst, hipStream_t s);
 void launch7_to_fortran(const ex7::K7 &k, const double *src, double *dst, hipStream_t s);
 

@@ -86,6 +86,7 @@ class Solver7:
         lib.swf7_bc.argtypes = [vp]
         lib.swf7_advance.argtypes = [vp, dp, C.c_double, C.c_int64, P(C.c_int64)]
         lib.swf7_stats.argtypes = [vp, P(L.SwfStats)]
+        lib.swf7_sweep.argtypes = [vp]
         lib.swf7_track.argtypes = [vp, P(C.c_int64), P(C.c_int64), P(C.c_int64), C.c_int64]
         lib.swf7_track.restype = C.c_int64
         self._lib = lib
@@ -126,6 +127,21 @@ class Solver7:
         if rc < 0:
             L._check(rc, "swf7_advance")
         return tt.value, n.value, rc
+
+    def sweep(self) -> int:
+        """sweep_lim on the current state; returns 0 or the stop's swf.h code."""
+        rc = self._lib.swf7_sweep(self._h)
+        if rc < 0:
+            L._check(rc, "swf7_sweep")
+        return rc
+
+    def track(self) -> np.ndarray:
+        """(q, negative pairs, positive neighbours) of each sweep so far."""
+        n = self.stats()["sweep_count"]
+        q, ng, ps = (np.zeros(max(n, 1), dtype=np.int64) for _ in range(3))
+        P = C.POINTER(C.c_int64)
+        m = self._lib.swf7_track(self._h, q.ctypes.data_as(P), ng.ctypes.data_as(P), ps.ctypes.data_as(P), n)
+        return np.stack([q[:m], ng[:m], ps[:m]], axis=1)
 
     def stats(self) -> dict:
         s = L.SwfStats()

@@ -6,10 +6,10 @@
 ! to [-0.25, 0.25] in den.dat / pre.dat / m1.dat; the sweep statistics) and replaces
 ! its time-stepping procedures (`rk` and callees) by the C-ABI of libswf.so.
 !
-!   swf_jet [nx ny precision]
+!   swf_jet [nx ny precision time_terminal]
 !
-! Defaults are the program's 800 x 200; precision 0 = FP64-exact (bit-identical to
-! the reference), 1 = FP64, 2 = FP32-bulk.  Compiled like the reference, with
+! Defaults are the program's 800 x 200 and T = 0.001; precision 0 = FP64-exact
+! (bit-identical to the reference), 1 = FP64, 2 = FP32-bulk.  Compiled like the reference, with
 ! -fdefault-real-8: the d-exponent literals (loop test, percentage) are binary128.
 program swf_jet
    use swf
@@ -17,9 +17,10 @@ program swf_jet
    implicit none
    ! parameter surface (ex5_mach2000_jet/main.f90:4-5, :597)
    integer, parameter :: swp_num = 20000
-   real(8), parameter :: time_terminal = 0.001, gamma = 5./3., gamma1 = gamma - 1., eps = 1d-13, eps2 = 1d-8
+   real(8), parameter :: gamma = 5./3., gamma1 = gamma - 1., eps = 1d-13, eps2 = 1d-8
    real(8), parameter :: ujet = 800.
    integer :: nx, ny, total_x, total_y, prec
+   real(8) :: time_terminal
    character(len=64) :: arg
    real(8), allocatable :: u(:, :, :)
    real(8) :: dx, dy, time, xleft, xright, yleft, yright, avg
@@ -34,6 +35,7 @@ program swf_jet
    nx = 800
    ny = 200
    prec = SWF_FP64_EXACT
+   time_terminal = 0.001
    nargs = command_argument_count()
    if (nargs >= 1) then
       call get_command_argument(1, arg)
@@ -46,6 +48,10 @@ program swf_jet
    if (nargs >= 3) then
       call get_command_argument(3, arg)
       read (arg, *) prec
+   end if
+   if (nargs >= 4) then
+      call get_command_argument(4, arg)
+      read (arg, *) time_terminal
    end if
    total_x = nx
    total_y = ny

@@ -190,6 +190,10 @@ module swf
          integer(c_int64_t), intent(out) :: steps
       end function swf7_advance
 
+      integer(c_int) function swf7_sweep(ctx) bind(C, name='swf7_sweep')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function swf7_sweep
       integer(c_int) function swf7_stats(ctx, s) bind(C, name='swf7_stats')
          import :: c_ptr, c_int, swf_sweep_stats
          type(c_ptr), value :: ctx

@@ -197,6 +197,33 @@ def line_run(prec, n, cfl, swp, shipped, nsteps):
             "track_equal": bool(np.array_equal(q, qo) and np.array_equal(ng, ngo) and np.array_equal(ps, pso))}
 
 
+def ex7_sweeps():
+    """swf7_sweep on the crafted states of tests/sweep7_cases.py against the oracle's
+    serial restatement of the program's sweep: per case the stop code, the counts and
+    the number of differing state values."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from sweep7_cases import CASES, crafted7
+    from sweeping_fd_weno_amd import detonation as D
+    res = []
+    for total, kind, seed in CASES:
+        u = crafted7(total, kind, seed)
+        ref = O.sweep7(u, total)
+        s = D.Solver7(D.det_config(total))
+        s.upload(u)
+        code = s.sweep()
+        got = s.download()
+        st = s.stats()
+        trk = s.track()
+        s.close()
+        res.append({"case": [total, kind, seed], "code": [code, ref["code"]],
+                    "counts": [[st["sweep_count"], st["den_count"], st["pre_count"]],
+                               [ref["sweep_count"], ref["den_count"], ref["pre_count"]]],
+                    "track": [trk[-1].tolist() if len(trk) else [],
+                              [ref["q"], ref["neg"], ref["pos"]] if ref["code"] == 0 else [0, 0, 0]],
+                    "differ": int((got != ref["u"]).sum()), "changed": int((ref["u"] != u).sum())})
+    return {"cases": res}
+
+
 def main(name):
     if name == "sedov_sweeps":
         cfg = S.sedov_config(32, "fp64_exact")
@@ -210,6 +237,8 @@ def main(name):
         out = {"steps": r["steps"], "code": r["code"], "stats": r["stats"],
                "sha256": {k: hashlib.sha256(np.ascontiguousarray(v, dtype="<f8").tobytes()).hexdigest()
                           for k, v in r["fields"].items()}}
+    elif name == "sweep7_crafted":
+        out = ex7_sweeps()
     elif name == "jet_full_sweeps":
         cfg = S.jet_config(64, 32, "fp64_exact", cfl=0.5, half=False)
         out = run_both(cfg, S.init_jet(cfg), 0.001, 30)

@@ -40,8 +40,13 @@ oracle/_ref/, run there, and only its OUTPUTS are stored here:
                           (40 x 40, 17 steps, 3 sweeps) and total = 16 (80 x 80, 33 steps, 8
                           sweeps) to T = 0.2: the same records for den.dat / pre.dat / rate.dat,
                           and the 40 x 40 fields in full.
+  ref_ex7_detonation400.json  the shipped grid (400 x 400, T = 0.6) to the program's own
+                          stop: progress lines, STOP text, run time on one core.
+  ref_ex5_jet800x200.json ex5_mach2000_jet at its shipped grid to T = 5e-5 (time_terminal
+                          set in a build copy): the records of ref_ex5_jet160x40.json.
 
-Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5s] [ex5] [ex4] [ex6] [ex7]
+Usage:  python tests/golden/make_golden.py [weno] [weno6] [ex3] [ex3_l1] [ex2] [ex5s] [ex5] [ex5ship] [ex4]
+        [ex6] [ex7] [ex7ship]
 """
 from __future__ import annotations
 
@@ -234,31 +239,47 @@ def make_ex2() -> None:
         json.dump(out, f, indent=1)
 
 
+def _ex5_entry(build: str, nx: int, ny: int, program: str, stride: int) -> dict:
+    """Runs ex5 build copy `build` (grid nx x ny) and forms its fixture entry."""
+    rundir = os.path.join(REF, "run_" + build)
+    text = _run(os.path.join(REF, build, "a"), rundir)
+    fields = {}
+    for name, fn, skip in (("log_rho", "den.dat", 2), ("log_p", "pre.dat", 1), ("log_p_over_rho", "m1.dat", 1)):
+        body = open(os.path.join(rundir, fn)).read().split("\n", skip)[skip]
+        t = np.array(body.split(), dtype=np.float64).reshape(-1, 3)
+        fields[name] = t[ny * nx:, 2]
+    steps = [int(l.split()[0]) for l in text.splitlines() if l.strip().endswith("%")]
+    return {
+        "program": program,
+        "sha256": {k: field_digest(v) for k, v in fields.items()},
+        "sample": {k: v[::stride].tolist() for k, v in fields.items()},
+        "stats": _parse_stats(text),
+        "last_progress_step": steps[-1] if steps else 0,
+        "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
+                        for fn in ("den.dat", "pre.dat", "m1.dat")},
+        "stdout_sha256": hashlib.sha256(text.encode()).hexdigest(),
+    }
+
+
 def make_ex5_small() -> None:
     """oracle/Makefile targets ex5_small (T = 0.001) and ex5_short (T = 1.5e-4), 160 x 40."""
-    nx, ny = 160, 40
     out = {"layout": "log fields of the unmirrored half (the file's second half), (j outer, i inner), "
                      "160 x 40 interior"}
     for key, build, desc in (("full", "ex5_small", "T = 0.001"), ("short", "ex5_short", "T = 1.5e-4")):
-        rundir = os.path.join(REF, "run_" + build)
-        text = _run(os.path.join(REF, build, "a"), rundir)
-        fields = {}
-        for name, fn, skip in (("log_rho", "den.dat", 2), ("log_p", "pre.dat", 1), ("log_p_over_rho", "m1.dat", 1)):
-            body = open(os.path.join(rundir, fn)).read().split("\n", skip)[skip]
-            t = np.array(body.split(), dtype=np.float64).reshape(-1, 3)
-            fields[name] = t[ny * nx:, 2]
-        steps = [int(l.split()[0]) for l in text.splitlines() if l.strip().endswith("%")]
-        out[key] = {
-            "program": f"ex5_mach2000_jet with nx = 160, ny = 40 (PARAMETER line), {desc}, otherwise as shipped",
-            "sha256": {k: field_digest(v) for k, v in fields.items()},
-            "sample": {k: v[::97].tolist() for k, v in fields.items()},
-            "stats": _parse_stats(text),
-            "last_progress_step": steps[-1] if steps else 0,
-            "file_sha256": {fn: hashlib.sha256(open(os.path.join(rundir, fn), "rb").read()).hexdigest()
-                            for fn in ("den.dat", "pre.dat", "m1.dat")},
-            "stdout_sha256": hashlib.sha256(text.encode()).hexdigest(),
-        }
+        out[key] = _ex5_entry(build, 160, 40, f"ex5_mach2000_jet with nx = 160, ny = 40 (PARAMETER line), {desc}, "
+                                              "otherwise as shipped", 97)
     with open(os.path.join(GOLD, "ref_ex5_jet160x40.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+def make_ex5_ship() -> None:
+    """oracle/Makefile target ex5_ship: the shipped 800 x 200 grid to T = 5e-5 (about half
+    an hour on one core)."""
+    out = {"layout": "log fields of the unmirrored half (the file's second half), (j outer, i inner), "
+                     "800 x 200 interior",
+           "ship": _ex5_entry("ex5_ship", 800, 200, "ex5_mach2000_jet as shipped (800 x 200, cfl 0.25) with "
+                                                    "time_terminal = 0.00005", 997)}
+    with open(os.path.join(GOLD, "ref_ex5_jet800x200.json"), "w") as f:
         json.dump(out, f, indent=1)
 
 
@@ -370,6 +391,28 @@ def make_ex7() -> None:
         json.dump(out, f, indent=1)
 
 
+def make_ex7_ship() -> None:
+    """ex7 at its shipped grid (total = 80, 400 x 400; oracle/Makefile target ex7_80) from
+    t = 0 to the program's own stop: its progress lines (every 50 steps), the STOP text
+    and the run time on one core (about 25 minutes)."""
+    import time as _t
+    rundir = os.path.join(REF, "run_ex7_80")
+    os.makedirs(rundir, exist_ok=True)
+    t0 = _t.perf_counter()
+    res = subprocess.run(["bash", "-c", "ulimit -s unlimited; exec " + os.path.join(REF, "ex7_80", "a") + " " + rundir],
+                         cwd=rundir, capture_output=True, text=True)
+    secs = _t.perf_counter() - t0
+    text = res.stdout + res.stderr
+    progress = [[int(l.split()[0]), l.split()[1]] for l in text.splitlines() if l.strip().endswith("%")]
+    stop = [l.strip() for l in text.splitlines() if "STOP" in l]
+    out = {"program": "ex7_detonation, one-rank build copy (use MPI -> include 'mpif.h', cut_x = cut_y = 1, "
+                      "mpirank initialised to 0), total = 80 (400 x 400) and time_terminal = 0.6 as shipped",
+           "progress": progress, "stop": stop, "returncode": res.returncode,
+           "reference_seconds_1core": round(secs, 1)}
+    with open(os.path.join(GOLD, "ref_ex7_detonation400.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def _tokens_skip(fn: str, ncol: int, skip: int) -> np.ndarray:
     body = open(fn).read().split("\n", skip)[skip]
     return np.array(body.split(), dtype=np.float64).reshape(-1, ncol)
@@ -417,9 +460,13 @@ if __name__ == "__main__":
         make_ex5_small()
     if "ex5" in what:
         make_ex5()
+    if "ex5ship" in what:
+        make_ex5_ship()
     if "ex4" in what:
         make_ex4()
     if "ex6" in what:
         make_ex6()
     if "ex7" in what:
         make_ex7()
+    if "ex7ship" in what:
+        make_ex7_ship()

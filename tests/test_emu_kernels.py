@@ -44,7 +44,7 @@ SCENARIOS = ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter
              "crafted_sweep_64_20_300", "crafted_psum_48_400_1", "crafted_psum_48_400_2", "jet_full_sweeps_fp64",
              "jet_full_sweeps_fp32_bulk", "batch_fp64_exact", "batch_fp32_bulk", "first_dt_fp64_exact",
              "first_dt_fp64", "first_dt_fp32_bulk", "line_small", "line_stress_fp64_exact", "line_cfg1",
-             "step_fp64_exact_4_40", "ex7_8"]
+             "step_fp64_exact_4_40", "ex7_8", "sweep7_crafted"]
 _RESULTS = {}
 
 
@@ -167,3 +167,18 @@ def test_emulated_detonation_matches_reference_run(emu_lib):
     assert (r["stats"]["total_sweep"], r["stats"]["total_neg"], r["stats"]["total_pos"]) == (
         st["total sweep"], st["total negative pairs"], st["total positive neighbors"])
     assert (r["stats"]["den_count"], r["stats"]["pre_count"]) == (st["den_count"], st["pre_count"])
+
+
+def test_emulated_detonation_sweep_equals_oracle_on_crafted_states(emu_lib):
+    """swf7_sweep (the workgroup-parallel sweep) against the oracle's serial restatement
+    of ex7's sweep_lim on the states of tests/sweep7_cases.py: bitwise, the same stop,
+    counts and track entry."""
+    r = scenario("sweep7_crafted")
+    codes = set()
+    for c in r["cases"]:
+        assert c["code"][0] == c["code"][1], c["case"]
+        assert c["counts"][0] == c["counts"][1], c["case"]
+        assert c["track"][0] == c["track"][1], c["case"]
+        assert c["differ"] == 0 and c["changed"] > 0, c["case"]
+        codes.add(c["code"][1])
+    assert codes == {0, 3, 4}

@@ -2,9 +2,15 @@
 reference runs (tests/golden/ref_ex7_detonation.json, one-rank build copies of the
 program): 40 x 40 (17 steps, 3 sweeps) and 80 x 80 (33 steps, 8 sweeps) to T = 0.2 --
 den / pre / rate as the program writes them bitwise (SHA-256), the step count and every
-printed statistic."""
+printed statistic.  The sweep alone (swf7_sweep) on crafted states against the oracle's
+serial restatement of the program's sweep (density sweep, negative-neighbour pairs,
+order-II passes and both stops), and the shipped 400 x 400 grid against the reference's
+own run (tests/golden/ref_ex7_detonation400.json): its progress lines and its stop."""
 import json
 import os
+import sys
+from decimal import Decimal, getcontext
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -13,6 +19,7 @@ from sweeping_fd_weno_amd import detonation as D
 from sweeping_fd_weno_amd.build import ROOT
 
 GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_ex7_detonation.json")))
+getcontext().prec = 80
 
 
 def _sha(a):
@@ -34,3 +41,76 @@ def test_detonation_bitwise_to_reference_run(key, total):
     assert (s["total_neg"], s["max_neg"]) == (st["total negative pairs"], st["max negative pairs"])
     assert (s["total_pos"], s["max_pos"]) == (st["total positive neighbors"], st["max positive neighbors"])
     assert (s["den_count"], s["pre_count"]) == (st["den_count"], st["pre_count"])
+
+
+@pytest.mark.gpu
+def test_sweep_on_crafted_states_equals_oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle as O
+    from sweep7_cases import CASES, crafted7
+    for total, kind, seed in CASES:
+        print("case", total, kind, seed, flush=True)
+        u = crafted7(total, kind, seed)
+        ref = O.sweep7(u, total)
+        s = D.Solver7(D.det_config(total))
+        s.upload(u)
+        code = s.sweep()
+        got = s.download()
+        st = s.stats()
+        trk = s.track()
+        s.close()
+        assert code == ref["code"], (kind, seed)
+        assert (st["sweep_count"], st["den_count"], st["pre_count"]) == (
+            ref["sweep_count"], ref["den_count"], ref["pre_count"]), (kind, seed)
+        want = [ref["q"], ref["neg"], ref["pos"]] if ref["code"] == 0 else [0, 0, 0]
+        assert trk[-1].tolist() == want, (kind, seed)
+        assert np.array_equal(got.view(np.int64), ref["u"].view(np.int64)), (kind, seed)
+
+
+def _nearest_q128(d: Fraction) -> Fraction:
+    """d > 0 rounded to the nearest binary128 value (ties to even)."""
+    e = d.numerator.bit_length() - d.denominator.bit_length()
+    while Fraction(2) ** e > d:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= d:
+        e += 1
+    ulp = Fraction(2) ** (e - 112)
+    q = d / ulp
+    n = q.numerator // q.denominator
+    rem = q - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return n * ulp
+
+
+def _percent(t: float, T: float) -> str:
+    """time/time_terminal*100d0 as the program prints it: the REAL(8) quotient times the
+    REAL(16) 100 (exact: 60 bits), written with the fewest significant digits that read
+    back as the same REAL(16) value (the runtime's list-directed output)."""
+    v = Fraction(float(np.float64(t) / np.float64(T))) * 100
+    exact = Decimal(v.numerator) / Decimal(v.denominator)
+    for digits in range(1, 41):
+        txt = format(exact, f".{digits}g")
+        if _nearest_q128(Fraction(Decimal(txt))) == v:
+            return txt
+    raise AssertionError(v)
+
+
+@pytest.mark.gpu
+def test_shipped_grid_progress_and_stop_match_reference_run():
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_ex7_detonation400.json")))
+    cfg = D.det_config(80)
+    s = D.Solver7(cfg)
+    s.upload(D.init_detonation(cfg))
+    s.bc()
+    t, k, seen = 0.0, 0, []
+    while True:
+        t, n, rc = s.advance(t, 0.6, 50)
+        k += n
+        if rc != 0 or n < 50:
+            break
+        seen.append([k, _percent(t, 0.6)])
+    s.close()
+    print("stopped after", k, "steps at t =", t, "code", rc, flush=True)
+    assert seen == g["progress"]
+    assert rc == 3 and any("Sweep check failed" in x for x in g["stop"])

@@ -7,6 +7,8 @@ tests/golden/ref_ex5_jet160x40.json.
   bitwise (logarithms by the C library, as the program's), and every printed statistic.
 - fortran/swf_jet, the ex5 program over the C-ABI, writes den.dat / pre.dat / m1.dat
   byte for byte and prints the same text (progress percentages, statistics).
+- The same two at the program's own 800 x 200 grid to T = 5e-5
+  (tests/golden/ref_ex5_jet800x200.json).
 """
 import hashlib
 import json
@@ -53,6 +55,42 @@ def test_jet160x40_exact_matches_reference_run():
 def test_jet_driver_reproduces_reference_output_files(tmp_path):
     g = json.load(open(GOLD))["full"]
     res = subprocess.run([DRIVER, "160", "40"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout + res.stderr
+    for fn, digest in g["file_sha256"].items():
+        assert hashlib.sha256((tmp_path / fn).read_bytes()).hexdigest() == digest, fn
+    assert hashlib.sha256(res.stdout.encode()).hexdigest() == g["stdout_sha256"]
+
+
+GOLD_SHIP = os.path.join(ROOT, "tests", "golden", "ref_ex5_jet800x200.json")
+
+
+def test_jet_shipped_grid_exact_matches_reference_run():
+    """The program's own 800 x 200 grid (cfl 0.25), to T = 5e-5 (time_terminal set in a
+    build copy): fields bitwise and every statistic."""
+    g = json.load(open(GOLD_SHIP))["ship"]
+    cfg = S.jet_config(800, 200, "fp64_exact")
+    with S.Solver(cfg) as s:
+        s.upload(S.init_jet(cfg))
+        s.bc()
+        t, k = s.advance(0.0, 5e-5)
+        u = s.download()
+        st = s.stats()
+    print("steps", k, "sweeps", st["sweep_count"], flush=True)
+    for name, v in jet_log_fields(u, 800, 200).items():
+        assert _digest(v) == g["sha256"][name], name
+    for key, name in (("total sweep", "total_sweep"), ("max sweep", "max_sweep"), ("total negative pairs", "total_neg"),
+                      ("max negative pairs", "max_neg"), ("total positive neighbors", "total_pos"),
+                      ("max positive neighbors", "max_pos"), ("den_count", "den_count"), ("pre_count", "pre_count")):
+        assert st[name] == g["stats"][key], key
+    assert k // 20 * 20 == g["last_progress_step"]
+
+
+def test_jet_driver_shipped_grid_output_files(tmp_path):
+    """fortran/swf_jet at the program's grid, T = 5e-5: den.dat / pre.dat / m1.dat and the
+    printout byte for byte."""
+    g = json.load(open(GOLD_SHIP))["ship"]
+    res = subprocess.run([DRIVER, "800", "200", "0", "0.00005"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=300)
     assert res.returncode == 0, res.stdout + res.stderr
     for fn, digest in g["file_sha256"].items():
         assert hashlib.sha256((tmp_path / fn).read_bytes()).hexdigest() == digest, fn
