@@ -106,7 +106,7 @@ struct Bufs {
   long long *trk;                   // 3 * track_cap: q, neg, pos per sweep trigger
   double *apart;                    // alpha max of each update workgroup
   double *upart;                    // 4 per update workgroup: sum of weight*u over its cells
-  double *wrec;                     // sweep walk records: SW_REC rows x 4 x SW_T slots
+  double *wrec;                     // sweep walk records: SW_REC rows x SW_T slots x 4
   unsigned *wpos;                   // their branch bits: SW_REC/32 words x SW_T slots
   int *zs;                          // flux limiter: (den, pre) counts per workgroup of the
                                     // x-flux, stage and edge kernels (zs_pairs)
@@ -715,6 +715,13 @@ SWF_HD double sweep_pres(const KCfg &k, double v0, double v1, double v2, double 
 #pragma clang fp contract(off)
   if (k.fast_div) return k.gamma1 * (v3 - (0.5 * (v1 * v1 + v2 * v2)) * rcp64(v0));
   return sweep_pres(k.gamma1, v0, v1, v2, v3);
+}
+// The same with the mode fixed at compile time (the sweep's speculative walks).
+template <bool FAST>
+SWF_HD double sweep_pres_t(double g1, double v0, double v1, double v2, double v3) {
+#pragma clang fp contract(off)
+  if (FAST) return g1 * (v3 - (0.5 * (v1 * v1 + v2 * v2)) * rcp64(v0));
+  return sweep_pres(g1, v0, v1, v2, v3);
 }
 SWF_HD bool sweep_flag(const KCfg &k, double v0, double v1, double v2, double v3) {
 #pragma clang fp contract(off)

@@ -123,9 +123,11 @@ SWF_D void store_ij(const KCfg &k, const Bufs &b, int i, int j, const double v[4
 // forms each cell's pressure once.
 // fast: the divisions by refined reciprocals and the square roots by sqrt64 (the
 // fast modes' arithmetic; each within 1 ulp), else IEEE operations as the reference.
-SWF_D bool nod_lim(double eps, double p1, double p2, double pb, const double v1[4], const double v2[4],
-                   const double vb[4], double out[4], bool fast = false) {
+template <bool FAST>
+SWF_D bool nod_lim_t(double eps, double p1, double p2, double pb, const double v1[4], const double v2[4],
+                     const double vb[4], double out[4]) {
 #pragma clang fp contract(off)
+  constexpr bool fast = FAST;
   double t;
   bool pos;
   if (p2 > eps) {
@@ -153,21 +155,58 @@ SWF_D bool nod_lim(double eps, double p1, double p2, double pb, const double v1[
   for (int c = 0; c < 4; ++c) out[c] = (1. - t) * v1[c] + t * v2[c];
   return pos;
 }
+SWF_D bool nod_lim(double eps, double p1, double p2, double pb, const double v1[4], const double v2[4],
+                   const double vb[4], double out[4], bool fast = false) {
+  return fast ? nod_lim_t<true>(eps, p1, p2, pb, v1, v2, vb, out) : nod_lim_t<false>(eps, p1, p2, pb, v1, v2, vb, out);
+}
 
-// Pressure-sweep walk from position f in `order`.  FWD: neighbour f+1, else f-1.
-// WRITE == false: speculative, returns the stop position only, and records each
-// step's adjusted state (and branch) in the slot's rows of the walk record, the
-// value handed to the stop position after the last step, up to SW_REC rows.
-// The cells of the path are loaded PF = 2 positions ahead (clamped at the last
-// position); the values used are those of cells the walk reaches, which no other
-// walk writes.  Each cell's pressure is formed once (sweep_pres = nod_lim's form).
-// psum_hit (speculative walks): set when a step fails the step program's pressure-sum
-// check; the failure counts only if the walk turns out to be genuine.
-template <bool FWD, bool WRITE, int NT>
-SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], double pb, unsigned *next_map,
-                long long *npos, long long *nneg, SweepShared<NT> *sh, int slot, bool *psum_hit = nullptr) {
+// A snake cursor that also carries the cell's offset in a plane (cidx), stepped by
+// additions: along a line by dc (1 along a row, the pitch along a column), to the
+// next line by da.
+struct SnakeO {
+  int a, c;
+  unsigned o;
+};
+SWF_D SnakeO snakeo_at(const KCfg &k, int order, int p) {
+  const Snake s = snake_at(k, order, p);
+  int i, j;
+  snake_ij(s, order, i, j);
+  return SnakeO{s.a, s.c, (unsigned)cidx(k, i, j)};
+}
+template <bool FWD>
+SWF_D void snakeo_step(SnakeO &s, int n, unsigned dc, unsigned da) {
+  const bool up = s.a & 1;
+  if (FWD) {
+    if (up ? s.c < n : s.c > 1) {
+      s.c += up ? 1 : -1;
+      s.o = up ? s.o + dc : s.o - dc;
+    } else {
+      ++s.a;
+      s.o += da;
+    }
+  } else {
+    if (up ? s.c > 1 : s.c < n) {
+      s.c += up ? -1 : 1;
+      s.o = up ? s.o - dc : s.o + dc;
+    } else {
+      --s.a;
+      s.o -= da;
+    }
+  }
+}
+
+// Pressure-sweep walk from position f in `order` that writes as it goes (a genuine
+// walk longer than the SW_REC steps the speculative record holds).  FWD: neighbour
+// f+1, else f-1.  Each step adjusts the current cell against its neighbour
+// (nod_lim), stores the adjusted cell, flags it for the next pass when still
+// flagged, counts its branch, and hands the difference to the neighbour; the walk
+// stops at the first neighbour no longer flagged (or at the pass's last position)
+// and stores it.  The path is loaded two cells ahead; each cell's pressure is
+// formed once (sweep_pres = nod_lim's form).  Returns the stop position.
+template <bool FWD, int NT>
+SWF_D int write_walk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], double pb,
+                     unsigned *next_map, long long *npos, long long *nneg, SweepShared<NT> *sh) {
 #pragma clang fp contract(off)
-  static_assert(NT <= SW_T, "walk records hold SW_T slots");
   const int N = k.nx * k.ny;
   const int last = FWD ? N - 1 : 0;
   const int nline = order == 0 ? k.nx : k.ny;
@@ -177,21 +216,13 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
   snake_ij(cur, order, i, j);
   double v[4];
   load_ij(k, b, i, j, v);
-  // cells p+1 .. p+PF (p-1 ... backward) in PF slots used in rotation: the loop
-  // body is unrolled PF times, so no slot is copied while its load is in flight;
-  // the path is stepped by snake cursors, without division
-  // A walk step is a chain of two correctly rounded divisions (about 0.2 us for
-  // one wave), so two cells ahead cover the load latency; deeper prefetch only
-  // added registers (jet 800 x 400 walks: PF 2, 3, 4, 6 -> 195, 207, 201, 238 us
-  // per sweep)
-  constexpr int PF = 2;
   double a0[4], a1[4];
-  int q = p;           // the cell loaded last, PF ahead of p (clamped at `last`)
+  int q = p;  // the cell loaded last, two ahead of p (clamped at `last`)
   Snake pre = cur;
   {
-    double(*slots[PF])[4] = {&a0, &a1};
+    double(*slots[2])[4] = {&a0, &a1};
 #pragma unroll
-    for (int d = 0; d < PF; ++d) {
+    for (int d = 0; d < 2; ++d) {
       if (q != last) {
         snake_step<FWD>(pre, nline);
         q += FWD ? 1 : -1;
@@ -202,9 +233,7 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
     }
   }
   double pv = sweep_pres(k, v[0], v[1], v[2], v[3]);
-  unsigned posw = 0u;  // branch bits of the current 32 steps
-  int s = 0;
-  // one step with the neighbour in `slot_v`, which then receives cell nb+PF;
+  // one step with the neighbour in `slot_v`, which then receives the cell two ahead;
   // returns true at the stop position
   auto step = [&](double(&slot_v)[4]) __attribute__((always_inline)) -> bool {
     const int nb = FWD ? p + 1 : p - 1;
@@ -228,45 +257,124 @@ SWF_D int pwalk(const KCfg &k, const Bufs &b, int order, int f, const double ub[
     // the step program's check of an order-I adjustment (ex6_shock_diffraction/
     // main.f90:1418-1426, :1477-1485): total pressure of the pair must not rise
     if (order == 0 && k.psum_check &&
-        pv + pn > (sweep_pres(k, vt[0], vt[1], vt[2], vt[3]) + sweep_pres(k, vn2[0], vn2[1], vn2[2], vn2[3])) + eps) {
-      if (WRITE)
-        b.g->status = ST_PSUM;
-      else
-        *psum_hit = true;
-    }
-    if (WRITE) {
-      store_ij(k, b, i, j, vt);
-      mark_dirty(k, *sh, i, j);
-      if (sweep_flag(k, vt[0], vt[1], vt[2], vt[3])) set_bit(next_map, p);
-      if (pos)
-        ++*npos;
-      else
-        ++*nneg;
-    } else if (s < SW_REC) {
+        pv + pn > (sweep_pres(k, vt[0], vt[1], vt[2], vt[3]) + sweep_pres(k, vn2[0], vn2[1], vn2[2], vn2[3])) + eps)
+      b.g->status = ST_PSUM;
+    store_ij(k, b, i, j, vt);
+    mark_dirty(k, *sh, i, j);
+    if (sweep_flag(k, vt[0], vt[1], vt[2], vt[3])) set_bit(next_map, p);
+    if (pos)
+      ++*npos;
+    else
+      ++*nneg;
+    p = nb;
+    snake_step<FWD>(cur, nline);
+    snake_ij(cur, order, i, j);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) b.wrec[((long long)s * 4 + c) * NT + slot] = vt[c];
+    for (int c = 0; c < 4; ++c) v[c] = vn2[c];
+    pv = sweep_pres(k, v[0], v[1], v[2], v[3]);
+    const bool edge = FWD ? (p == N - 1) : (p == 0);
+    if (edge || !((pv <= eps) || (v[0] <= eps))) {  // stop at the first cell not flagged
+      store_ij(k, b, i, j, v);
+      mark_dirty(k, *sh, i, j);
+      return true;
+    }
+    return false;
+  };
+  while (true) {
+    if (step(a0)) return p;
+    if (step(a1)) return p;
+  }
+}
+
+// Speculative pressure-sweep walk from position f: the same steps as write_walk
+// (the precision mode FAST and ex6's pressure-sum check PS fixed at compile time),
+// but it writes no cell: it returns the stop position and records each step's
+// adjusted cell and branch (wrec, wpos) and, in the row after the last step, the
+// value handed to the stop position, up to SW_REC rows; psum_hit is set when a step
+// fails ex6's check (it counts only if the walk turns out to be genuine).  The
+// longest walk of a pass is its critical path, and a step is issue-bound (one wave
+// issues its 70-120 instructions in order, 4-8 cycles each: about 0.45 us per step
+// on the Mach 2000 jet's walks), so the step carries nothing the chain does not
+// need: no run-time mode branches, cell offsets stepped by additions, and the
+// record of step s at wrec[(s * NT + slot) * 4 + c], 32 contiguous bytes per slot.
+template <bool FWD, int NT, bool FAST, bool PS>
+SWF_D int spec_walk(const KCfg &k, const Bufs &b, int order, int f, const double ub[4], double pb, int slot,
+                    bool *psum_hit) {
+#pragma clang fp contract(off)
+  static_assert(NT <= SW_T, "walk records hold SW_T slots");
+  const int N = k.nx * k.ny;
+  const int last = FWD ? N - 1 : 0;
+  const int nline = order == 0 ? k.nx : k.ny;
+  const unsigned dc = order == 0 ? 1u : (unsigned)k.P, da = order == 0 ? (unsigned)k.P : 1u;
+  const double g1 = k.gamma1, eps = k.eps, wr = k.weight / k.weight;
+  const double *U0 = b.U, *U1 = U0 + k.plane, *U2 = U1 + k.plane, *U3 = U2 + k.plane;
+  auto load = [&](unsigned o, double v[4]) __attribute__((always_inline)) {
+    v[0] = U0[o];
+    v[1] = U1[o];
+    v[2] = U2[o];
+    v[3] = U3[o];
+  };
+  int p = f;
+  SnakeO pre = snakeo_at(k, order, p);  // the cell loaded last, up to 2 ahead of p
+  double v[4];
+  load(pre.o, v);
+  int q = p;
+  // cells p+1, p+2 (p-1, p-2 backward) in two slots used in rotation (the loop body is
+  // two steps); measured against deeper rings (4, 6, 8 cells, each slot refilled after
+  // its last use), which issued more instructions per step than they saved in waits
+  double a0[4], a1[4];
+  if (q != last) {
+    snakeo_step<FWD>(pre, nline, dc, da);
+    q += FWD ? 1 : -1;
+  }
+  load(pre.o, a0);
+  if (q != last) {
+    snakeo_step<FWD>(pre, nline, dc, da);
+    q += FWD ? 1 : -1;
+  }
+  load(pre.o, a1);
+  double pv = sweep_pres_t<FAST>(g1, v[0], v[1], v[2], v[3]);
+  unsigned posw = 0u;
+  int s = 0;
+  double *rp = b.wrec + 4 * (long long)slot;
+  auto step = [&](double(&slot_v)[4]) __attribute__((always_inline)) -> bool {
+    double vn[4], vt[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) vn[c] = slot_v[c];
+    if (q != last) {
+      snakeo_step<FWD>(pre, nline, dc, da);
+      q += FWD ? 1 : -1;
+    }
+    load(pre.o, slot_v);
+    const double pn = sweep_pres_t<FAST>(g1, vn[0], vn[1], vn[2], vn[3]);
+    const bool pos = nod_lim_t<FAST>(eps, pv, pn, pb, v, vn, ub, vt);
+    double vn2[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) vn2[c] = vn[c] + wr * (v[c] - vt[c]);
+    // ex6's check of an order-I adjustment (ex6_shock_diffraction/main.f90:1418-1426,
+    // :1477-1485): the pair's total pressure must not rise
+    if (PS && pv + pn > (sweep_pres_t<FAST>(g1, vt[0], vt[1], vt[2], vt[3]) +
+                         sweep_pres_t<FAST>(g1, vn2[0], vn2[1], vn2[2], vn2[3])) +
+                            eps)
+      *psum_hit = true;
+    if (s < SW_REC) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) rp[c] = vt[c];
       if (pos) posw |= 1u << (s & 31);
       if ((s & 31) == 31) {
         b.wpos[(s >> 5) * NT + slot] = posw;
         posw = 0u;
       }
     }
-    p = nb;
-    if (WRITE) {
-      snake_step<FWD>(cur, nline);
-      snake_ij(cur, order, i, j);
-    }
+    rp += 4 * NT;
+    p += FWD ? 1 : -1;
 #pragma unroll
     for (int c = 0; c < 4; ++c) v[c] = vn2[c];
-    pv = sweep_pres(k, v[0], v[1], v[2], v[3]);
-    const bool edge = FWD ? (p == N - 1) : (p == 0);
-    if (edge || !((pv <= eps) || (v[0] <= eps))) {  // stop at the first cell not flagged
-      if (WRITE) {
-        store_ij(k, b, i, j, v);
-        mark_dirty(k, *sh, i, j);
-      } else if (s + 1 < SW_REC) {
+    pv = sweep_pres_t<FAST>(g1, v[0], v[1], v[2], v[3]);
+    if (p == last || !((pv <= eps) || (v[0] <= eps))) {  // stop at the first cell not flagged
+      if (s + 1 < SW_REC) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) b.wrec[((long long)(s + 1) * 4 + c) * NT + slot] = v[c];
+        for (int c = 0; c < 4; ++c) rp[c] = v[c];
         b.wpos[((s + 1) >> 5) * NT + slot] = posw;  // the block of steps not yet stored
       }
       return true;
@@ -290,7 +398,7 @@ SWF_D void replay_step(const KCfg &k, const Bufs &b, int order, int f, int st, i
 #pragma clang fp contract(off)
   double v[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) v[c] = b.wrec[((long long)st * 4 + c) * NT + slot];
+  for (int c = 0; c < 4; ++c) v[c] = b.wrec[((long long)st * NT + slot) * 4 + c];
   const unsigned pw = b.wpos[(st >> 5) * NT + slot];
   const int p = FWD ? f + st : f - st;
   int i, j;
@@ -448,6 +556,43 @@ SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, i
   return total;
 }
 
+// The walks the sequential loop really starts (genuine), from the speculative ends
+// (sh.ends, -1 where no walk was computed) of the m = min(NT, n - s0) candidates of
+// this chunk of the ordered list: the first candidate beyond the previous walk's end
+// is genuine, and so is the first candidate beyond each genuine walk's end.  Every
+// slot finds that successor of its own walk by a binary search over the chunk's
+// ordered positions (into sh.woff, free until the replay), then thread 0 follows the
+// chain, one hop per genuine walk, instead of testing every candidate in turn.
+template <bool FWD, int NT>
+SWF_D void select_genuine(SweepShared<NT> &sh, int s0, int n, int tid) {
+  const int m = min(NT, n - s0);
+  auto pos = [&](int q) { return FWD ? sh.list[s0 + q] : sh.list[n - 1 - (s0 + q)]; };
+  auto beyond = [&](int lo, int e) {  // first slot in [lo, m) past position e, or m
+    int hi = m;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const int fm = pos(mid);
+      if (FWD ? fm > e : fm < e)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    return lo;
+  };
+  if (tid < m && sh.ends[tid] >= 0) sh.woff[tid] = beyond(tid + 1, sh.ends[tid]);
+  __syncthreads();
+  if (tid == 0) {
+    int last = sh.last_end, q = beyond(0, last);
+    while (q < m && sh.ends[q] >= 0) {
+      sh.genuine[q] = 1;
+      last = sh.ends[q];
+      q = sh.woff[q];
+    }
+    sh.last_end = last;
+  }
+  __syncthreads();
+}
+
 // Walks of one pass from the n positions in sh.list (ascending; a backward pass
 // takes them from the end), NT candidates at a time.
 template <bool FWD, int NT>
@@ -461,25 +606,21 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
     const int le = sh.last_end;
     const bool cand = (f >= 0) && (FWD ? (f > le && f <= N - 2) : (f < le && f >= 1));
     bool hit = false;
-    sh.ends[tid] =
-        cand ? pwalk<FWD, false, NT>(k, b, order, f, ub, pb, nullptr, nullptr, nullptr, nullptr, tid, &hit) : -1;
+    int e = -1;
+    if (cand) {
+      const bool ps = order == 0 && k.psum_check;
+      if (k.fast_div)
+        e = ps ? spec_walk<FWD, NT, true, true>(k, b, order, f, ub, pb, tid, &hit)
+               : spec_walk<FWD, NT, true, false>(k, b, order, f, ub, pb, tid, &hit);
+      else
+        e = ps ? spec_walk<FWD, NT, false, true>(k, b, order, f, ub, pb, tid, &hit)
+               : spec_walk<FWD, NT, false, false>(k, b, order, f, ub, pb, tid, &hit);
+    }
+    sh.ends[tid] = e;
     sh.psum_bad[tid] = hit ? 1 : 0;
     sh.genuine[tid] = 0;
     __syncthreads();
-    if (tid == 0) {
-      int last = sh.last_end;
-      const int m = min(NT, n - s0);
-      for (int q = 0; q < m; ++q) {
-        const int fq = FWD ? sh.list[s0 + q] : sh.list[n - 1 - (s0 + q)];
-        const bool ok = FWD ? (fq > last && fq <= N - 2) : (fq < last && fq >= 1);
-        if (ok) {
-          sh.genuine[q] = 1;
-          last = sh.ends[q];
-        }
-      }
-      sh.last_end = last;
-    }
-    __syncthreads();
+    select_genuine<FWD, NT>(sh, s0, n, tid);
     // genuine walks: recorded ones are replayed by the whole workgroup (their
     // steps are independent), longer ones are walked again by their thread
     long long np = 0, nn = 0;
@@ -491,7 +632,7 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
       if (adj < SW_REC)
         len = adj + 1;
       else
-        pwalk<FWD, true, NT>(k, b, order, f, ub, pb, next, &np, &nn, &sh, tid);
+        write_walk<FWD, NT>(k, b, order, f, ub, pb, next, &np, &nn, &sh);
     }
     sh.woff[tid] = block_exclusive_scan<NT>(len, sh, &sh.nsteps, tid);
     __syncthreads();
@@ -562,19 +703,7 @@ SWF_D void density_walks(const KCfg &k, const Bufs &b, int n, SweepShared<NT> &s
     sh.ends[tid] = cand ? swalk<false, NT>(k, b, f, +1, nullptr) : -1;
     sh.genuine[tid] = 0;
     __syncthreads();
-    if (tid == 0) {
-      int last = sh.last_end;
-      const int m = min(NT, n - s0);
-      for (int q = 0; q < m; ++q) {
-        const int fq = sh.list[s0 + q];
-        if (fq > last && fq <= N - 2) {
-          sh.genuine[q] = 1;
-          last = sh.ends[q];
-        }
-      }
-      sh.last_end = last;
-    }
-    __syncthreads();
+    select_genuine<true, NT>(sh, s0, n, tid);
     if (sh.genuine[tid]) swalk<true, NT>(k, b, f, +1, &sh);
     __syncthreads();
   }
