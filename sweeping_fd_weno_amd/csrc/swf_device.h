@@ -708,19 +708,22 @@ SWF_HD bool sweep_flag(double g1, double eps, double v0, double v1, double v2, d
   double p = sweep_pres(g1, v0, v1, v2, v3);
   return (p <= eps) || (v0 <= eps);
 }
-// The same for a context: in the fast modes (k.fast_div) the division by rho is a
-// refined reciprocal (within 1 ulp), here and wherever the sweep forms a pressure,
-// so that every flag test of a run uses one function.
+// The fast modes' form: the division by rho a refined reciprocal (within 1 ulp) and
+// the two products fused, g1 * (E - (m1^2 + m2^2) / (2 rho)).
+SWF_HD double sweep_pres_fast(double g1, double v0, double v1, double v2, double v3) {
+  return g1 * fma(-0.5 * fma(v1, v1, v2 * v2), rcp64(v0), v3);
+}
+// The same for a context: the fast form in the fast modes (k.fast_div), here and
+// wherever the sweep forms a pressure, so that every flag test of a run uses one
+// function.
 SWF_HD double sweep_pres(const KCfg &k, double v0, double v1, double v2, double v3) {
-#pragma clang fp contract(off)
-  if (k.fast_div) return k.gamma1 * (v3 - (0.5 * (v1 * v1 + v2 * v2)) * rcp64(v0));
+  if (k.fast_div) return sweep_pres_fast(k.gamma1, v0, v1, v2, v3);
   return sweep_pres(k.gamma1, v0, v1, v2, v3);
 }
 // The same with the mode fixed at compile time (the sweep's speculative walks).
 template <bool FAST>
 SWF_HD double sweep_pres_t(double g1, double v0, double v1, double v2, double v3) {
-#pragma clang fp contract(off)
-  if (FAST) return g1 * (v3 - (0.5 * (v1 * v1 + v2 * v2)) * rcp64(v0));
+  if (FAST) return sweep_pres_fast(g1, v0, v1, v2, v3);
   return sweep_pres(g1, v0, v1, v2, v3);
 }
 SWF_HD bool sweep_flag(const KCfg &k, double v0, double v1, double v2, double v3) {

@@ -134,12 +134,24 @@ SWF_D bool nod_lim_t(double eps, double p1, double p2, double pb, const double v
     t = fast ? (p1 - eps) * rcp64(p1 - p2) : (p1 - eps) / (p1 - p2);
     pos = true;
   } else {
-    const double s12 = (((v2[3] - v1[3]) * (v2[3] - v1[3]) + (v2[2] - v1[2]) * (v2[2] - v1[2])) +
-                        (v2[1] - v1[1]) * (v2[1] - v1[1])) +
-                       (v2[0] - v1[0]) * (v2[0] - v1[0]);
-    const double s1b = (((vb[3] - v1[3]) * (vb[3] - v1[3]) + (vb[2] - v1[2]) * (vb[2] - v1[2])) +
-                        (vb[1] - v1[1]) * (vb[1] - v1[1])) +
-                       (vb[0] - v1[0]) * (vb[0] - v1[0]);
+    double s12, s1b;
+    if (fast) {  // the sums of squares fused, in the same order
+      double d[4], e[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        d[c] = v2[c] - v1[c];
+        e[c] = vb[c] - v1[c];
+      }
+      s12 = fma(d[0], d[0], fma(d[1], d[1], fma(d[2], d[2], d[3] * d[3])));
+      s1b = fma(e[0], e[0], fma(e[1], e[1], fma(e[2], e[2], e[3] * e[3])));
+    } else {
+      s12 = (((v2[3] - v1[3]) * (v2[3] - v1[3]) + (v2[2] - v1[2]) * (v2[2] - v1[2])) +
+             (v2[1] - v1[1]) * (v2[1] - v1[1])) +
+            (v2[0] - v1[0]) * (v2[0] - v1[0]);
+      s1b = (((vb[3] - v1[3]) * (vb[3] - v1[3]) + (vb[2] - v1[2]) * (vb[2] - v1[2])) +
+             (vb[1] - v1[1]) * (vb[1] - v1[1])) +
+            (vb[0] - v1[0]) * (vb[0] - v1[0]);
+    }
     if (fast) {
       const double t_1 = (p1 - eps) * rcp64(p1 - pb);
       t = fmin2((t_1 * sqrt64(s1b)) * rcp64(sqrt64(s12)), 0.25);
@@ -151,8 +163,9 @@ SWF_D bool nod_lim_t(double eps, double p1, double p2, double pb, const double v
     }
     pos = false;
   }
+  const double omt = 1. - t;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) out[c] = (1. - t) * v1[c] + t * v2[c];
+  for (int c = 0; c < 4; ++c) out[c] = fast ? fma(t, v2[c], omt * v1[c]) : omt * v1[c] + t * v2[c];
   return pos;
 }
 SWF_D bool nod_lim(double eps, double p1, double p2, double pb, const double v1[4], const double v2[4],
@@ -306,7 +319,9 @@ SWF_D int spec_walk(const KCfg &k, const Bufs &b, int order, int f, const double
   const int last = FWD ? N - 1 : 0;
   const int nline = order == 0 ? k.nx : k.ny;
   const unsigned dc = order == 0 ? 1u : (unsigned)k.P, da = order == 0 ? (unsigned)k.P : 1u;
-  const double g1 = k.gamma1, eps = k.eps, wr = k.weight / k.weight;
+  // the reference's weight ratio (weight / weight) is exactly 1 for the uniform
+  // weights of every program here, so the hand-over is vn + (v - vt) bit for bit
+  const double g1 = k.gamma1, eps = k.eps;
   const double *U0 = b.U, *U1 = U0 + k.plane, *U2 = U1 + k.plane, *U3 = U2 + k.plane;
   auto load = [&](unsigned o, double v[4]) __attribute__((always_inline)) {
     v[0] = U0[o];
@@ -350,7 +365,7 @@ SWF_D int spec_walk(const KCfg &k, const Bufs &b, int order, int f, const double
     const bool pos = nod_lim_t<FAST>(eps, pv, pn, pb, v, vn, ub, vt);
     double vn2[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) vn2[c] = vn[c] + wr * (v[c] - vt[c]);
+    for (int c = 0; c < 4; ++c) vn2[c] = vn[c] + (v[c] - vt[c]);
     // ex6's check of an order-I adjustment (ex6_shock_diffraction/main.f90:1418-1426,
     // :1477-1485): the pair's total pressure must not rise
     if (PS && pv + pn > (sweep_pres_t<FAST>(g1, vt[0], vt[1], vt[2], vt[3]) +
@@ -358,8 +373,9 @@ SWF_D int spec_walk(const KCfg &k, const Bufs &b, int order, int f, const double
                             eps)
       *psum_hit = true;
     if (s < SW_REC) {
+      double *ra = (double *)__builtin_assume_aligned(rp, 32);  // two 16-byte stores
 #pragma unroll
-      for (int c = 0; c < 4; ++c) rp[c] = vt[c];
+      for (int c = 0; c < 4; ++c) ra[c] = vt[c];
       if (pos) posw |= 1u << (s & 31);
       if ((s & 31) == 31) {
         b.wpos[(s >> 5) * NT + slot] = posw;
