@@ -536,7 +536,7 @@ SWF_D int block_exclusive_scan(int x, SweepShared<NT> &sh, int *total, int tid) 
 // counts gives each thread its place in the list.
 template <int NT>
 SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, int tid) {
-  constexpr int BW = 16;  // words loaded together
+  constexpr int BW = 32;  // words loaded together (one batch for up to 2^19 positions)
   const int wpt = (k.nwords + NT - 1) / NT;
   const int w0 = tid * wpt, w1 = min(w0 + wpt, k.nwords);
   unsigned v[BW];
