@@ -92,6 +92,7 @@ struct SweepShared {
   unsigned char genuine[NT];
   unsigned char psum_bad[NT];  // the slot's speculative walk failed ex6's pressure-sum check
   int nlist, last_end, flag, all_dirty, nsteps;
+  int chain0, chainlast;  // the genuine walks' first and last slot (select_genuine)
   long long tpos, tneg;
   double ubar[4];
   double red[4][NT / 64];
@@ -577,8 +578,10 @@ SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, i
 // this chunk of the ordered list: the first candidate beyond the previous walk's end
 // is genuine, and so is the first candidate beyond each genuine walk's end.  Every
 // slot finds that successor of its own walk by a binary search over the chunk's
-// ordered positions (into sh.woff, free until the replay), then thread 0 follows the
-// chain, one hop per genuine walk, instead of testing every candidate in turn.
+// ordered positions (into sh.woff, free until the replay); the genuine walks are the
+// chain of successors from the first candidate beyond the previous pass's end, marked
+// by pointer doubling (round r marks the successors 2^r links ahead of the marked
+// slots), ceil(log2 m) rounds instead of one dependent hop per genuine walk.
 template <bool FWD, int NT>
 SWF_D void select_genuine(SweepShared<NT> &sh, int s0, int n, int tid) {
   const int m = min(NT, n - s0);
@@ -595,17 +598,34 @@ SWF_D void select_genuine(SweepShared<NT> &sh, int s0, int n, int tid) {
     }
     return lo;
   };
-  if (tid < m && sh.ends[tid] >= 0) sh.woff[tid] = beyond(tid + 1, sh.ends[tid]);
-  __syncthreads();
+  const bool mine = tid < m && sh.ends[tid] >= 0;
+  // the successor link of each slot (m: none; a slot without a walk ends the chain)
+  int nxt = mine ? beyond(tid + 1, sh.ends[tid]) : m;
+  if (tid < m) sh.woff[tid] = nxt;
   if (tid == 0) {
-    int last = sh.last_end, q = beyond(0, last);
-    while (q < m && sh.ends[q] >= 0) {
-      sh.genuine[q] = 1;
-      last = sh.ends[q];
-      q = sh.woff[q];
-    }
-    sh.last_end = last;
+    const int q0 = beyond(0, sh.last_end);
+    sh.chain0 = (q0 < m && sh.ends[q0] >= 0) ? q0 : -1;
+    sh.chainlast = -1;
   }
+  __syncthreads();
+  bool marked = mine && tid == sh.chain0;
+  for (int span = 1; span < m; span <<= 1) {  // marked: within span - 1 links of the start
+    const int nn = (nxt < m) ? sh.woff[nxt] : m;
+    __syncthreads();
+    if (marked && nxt < m) sh.genuine[nxt] = 1;
+    if (tid < m) sh.woff[tid] = nn;
+    __syncthreads();
+    marked = marked || (tid < m && sh.genuine[tid]);
+    nxt = nn;
+  }
+  if (marked && sh.ends[tid] >= 0) {
+    sh.genuine[tid] = 1;
+    atomicMax(&sh.chainlast, tid);
+  } else if (tid < m) {
+    sh.genuine[tid] = 0;
+  }
+  __syncthreads();
+  if (tid == 0 && sh.chainlast >= 0) sh.last_end = sh.ends[sh.chainlast];
   __syncthreads();
 }
 
