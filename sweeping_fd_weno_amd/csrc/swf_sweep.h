@@ -83,7 +83,7 @@ template <int NT>
 struct SweepShared {
   union {
     int list[SW_WCH * 32];  // flagged positions of one scan chunk
-    double prod[4][NT];     // u_bar partial products
+    double prod[2][4][NT];  // u_bar products, two blocks of NT cells (serial u_bar)
   };
   int ends[NT];
   int woff[NT];  // first replay step of each slot's genuine walk
@@ -775,27 +775,55 @@ SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, const double 
   const long long N = (long long)k.nx * k.ny;
   const double w = k.weight;
   if (serial) {
-    double acc = 0.0;  // lane c of wave 0 accumulates component c
-    for (long long c0 = 0; c0 < N; c0 += NT) {
-      const long long t = c0 + tid;
-      if (t < N) {
-        const int i = 1 + (int)(t % k.nx), j = 1 + (int)(t / k.nx);
-        const long long o = cidx(k, i, j);
+    // Lane c of wave 0 adds component c's products in the reference's order, one
+    // block of NT cells at a time from LDS, with the next sixteen values read before
+    // the current sixteen are added; meanwhile the other waves form the next block's
+    // products in the other buffer (one barrier per block).
+    static_assert(NT >= 128 && NT % 64 == 0, "a summing wave and at least one staging wave");
+    const long long nb = (N + NT - 1) / NT;
+    auto stage = [&](long long blk, int buf, int t0, int stride) {
+      for (int t = t0; t < NT; t += stride) {
+        const long long cell = blk * NT + t;
+        if (cell < N) {
+          const int i = 1 + (int)(cell % k.nx), j = 1 + (int)(cell / k.nx);
+          const long long o = cidx(k, i, j);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sh.prod[c][tid] = w * b.U[c * k.plane + o];
-      }
-      __syncthreads();
-      if (tid < 4) {
-        const int m = (int)min((long long)NT, N - c0);
-        int q = 0;
-        for (; q + 8 <= m; q += 8) {  // eight LDS reads in flight, the sums in order
-          double x[8];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) x[e] = sh.prod[tid][q + e];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) acc = acc + x[e];
+          for (int c = 0; c < 4; ++c) sh.prod[buf][c][t] = w * b.U[c * k.plane + o];
         }
-        for (; q < m; ++q) acc = acc + sh.prod[tid][q];
+      }
+    };
+    stage(0, 0, tid, NT);
+    __syncthreads();
+    double acc = 0.0;
+    for (long long blk = 0; blk < nb; ++blk) {
+      const int cur = (int)(blk & 1);
+      if (tid < 64) {
+        if (tid < 4) {
+          const double *pr = sh.prod[cur][tid];
+          const int m = (int)min((long long)NT, N - blk * NT);
+          int q = 0;
+          if (m == NT) {
+            constexpr int G = 16;  // sixteen dependent adds cover one LDS read latency
+            double x[G];
+#pragma unroll
+            for (int e = 0; e < G; ++e) x[e] = pr[e];
+            for (q = G; q < NT; q += G) {
+              double y[G];
+#pragma unroll
+              for (int e = 0; e < G; ++e) y[e] = pr[q + e];
+#pragma unroll
+              for (int e = 0; e < G; ++e) acc = acc + x[e];
+#pragma unroll
+              for (int e = 0; e < G; ++e) x[e] = y[e];
+            }
+#pragma unroll
+            for (int e = 0; e < G; ++e) acc = acc + x[e];
+          } else {
+            for (; q < m; ++q) acc = acc + pr[q];
+          }
+        }
+      } else if (blk + 1 < nb) {
+        stage(blk + 1, 1 - cur, tid - 64, NT - 64);
       }
       __syncthreads();
     }
