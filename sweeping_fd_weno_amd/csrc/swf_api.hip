@@ -366,8 +366,18 @@ int swf_step(swf_ctx *c, double time, double tfinal, double *dt_out) {
   return h.status;
 }
 
+// SWF_NO_GRAPH=1 in the environment: every step launched on the stream, no graph
+// captured or launched (for tools that do not handle graph dispatches); read once.
+static bool no_graph() {
+  static const bool v = [] {
+    const char *e = getenv("SWF_NO_GRAPH");
+    return e && e[0] == '1';
+  }();
+  return v;
+}
+
 static int ensure_graph(swf_ctx *c) {
-  if (c->gexec) return SWF_OK;
+  if (c->gexec || no_graph()) return SWF_OK;
   hipGraph_t graph = nullptr;
   HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   for (int q = 0; q < kGraphSteps; ++q) enqueue_step(c, q & 1);
@@ -391,7 +401,7 @@ int swf_advance(swf_ctx *c, double *time_inout, double tfinal, int64_t max_steps
   int64_t launched = 0, since_check = 0;
   DevScalars h;
   while (launched < max_steps) {
-    if (max_steps - launched >= kGraphSteps) {
+    if (max_steps - launched >= kGraphSteps && !no_graph()) {
       HIPC(hipGraphLaunch(c->gexec, c->stream));
       launched += kGraphSteps;
       since_check += kGraphSteps;
@@ -790,7 +800,7 @@ void batch_enqueue_step(swf_batch *bt, int p) {
 }
 
 int batch_ensure_graph(swf_batch *bt) {
-  if (bt->gexec) return SWF_OK;
+  if (bt->gexec || no_graph()) return SWF_OK;
   hipGraph_t graph = nullptr;
   HIPC(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
   for (int q = 0; q < kGraphSteps; ++q) batch_enqueue_step(bt, q & 1);
@@ -897,7 +907,7 @@ int swf_batch_advance(swf_batch *bt, double *time_inout, const double *tfinal, i
   launch_begin_run_batch(bt->d_inst, n, bt->d_time, bt->d_tfinal, bt->stream);
   int64_t launched = 0, since_check = 0;
   while (launched < max_steps) {
-    if (max_steps - launched >= kGraphSteps) {
+    if (max_steps - launched >= kGraphSteps && !no_graph()) {
       HIPC(hipGraphLaunch(bt->gexec, bt->stream));
       launched += kGraphSteps;
       since_check += kGraphSteps;
