@@ -12,9 +12,10 @@ arithmetic and control flow, not races; the GPU tests check the real device.
 - Vortex 90^2 with the Zhang-Shu flux limiter: bitwise, limiter counts equal.
 - swf_sweep on crafted states with 14, 3000 and 12000 cells of negative
   pressure (the last more than one ordered list of flagged positions holds, so
-  its passes go chunk by chunk), and with a chain of 100 or 300 consecutive
-  flagged snake positions (walks shorter and longer than a walk record):
-  fields bitwise, statistics equal.
+  its passes go chunk by chunk), and with a chain of 100, 300 or 6000 consecutive
+  flagged snake positions (walks replayed from their speculative record, and one
+  longer than the record's 4096 steps, walked again as it writes): fields
+  bitwise, statistics equal.
 - FP64 / FP32-bulk full jet 64x32 with sweeps (u_bar from per-tile sums).
 - A batch of three instances against each run alone: bitwise.
 - After upload of the ambient jet and swf_bc, the first dt sees the Mach 2000
@@ -41,7 +42,7 @@ EMU = os.path.join(ROOT, "tests", "emu")
 # every scenario of this module: run concurrently, one process each, when the module starts
 SCENARIOS = ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter", "crafted_sweep_24_14",
              "crafted_sweep_128_3000", "crafted_sweep_160_12000", "crafted_sweep_64_20_100",
-             "crafted_sweep_64_20_300", "crafted_psum_48_400_1", "crafted_psum_48_400_2", "jet_full_sweeps_fp64",
+             "crafted_sweep_64_20_300", "crafted_sweep_96_20_6000", "crafted_psum_48_400_1", "crafted_psum_48_400_2", "jet_full_sweeps_fp64",
              "jet_full_sweeps_fp32_bulk", "batch_fp64_exact", "batch_fp32_bulk", "first_dt_fp64_exact",
              "first_dt_fp64", "first_dt_fp32_bulk", "line_small", "line_stress_fp64_exact", "line_cfg1",
              "step_fp64_exact_4_40", "ex7_8", "sweep7_crafted"]
@@ -76,7 +77,7 @@ def scenario(name):
 
 @pytest.mark.parametrize("name", ["sedov_sweeps", "jet_full_sweeps", "jet_half", "vortex_flux_limiter",
                                   "crafted_sweep_24_14", "crafted_sweep_128_3000", "crafted_sweep_160_12000",
-                                  "crafted_sweep_64_20_100", "crafted_sweep_64_20_300"])
+                                  "crafted_sweep_64_20_100", "crafted_sweep_64_20_300", "crafted_sweep_96_20_6000"])
 def test_emulated_kernels_match_oracle(emu_lib, name):
     r = scenario(name)
     assert r["rc"] == 0

@@ -145,12 +145,14 @@ def _crafted_sweep_state(cfg, seed):
     return u
 
 
-@pytest.mark.parametrize("n,nbad,chain", [(128, 3000, 0), (160, 12000, 0), (64, 20, 100), (64, 20, 300)])
+@pytest.mark.parametrize("n,nbad,chain", [(128, 3000, 0), (160, 12000, 0), (64, 20, 100), (64, 20, 300),
+                                         (96, 20, 6000)])
 def test_sweep_many_flagged_matches_oracle(n, nbad, chain):
     """Thousands of flagged cells (at 12000 a pass holds more flagged positions
     than its ordered list and goes chunk by chunk), or a chain of consecutive
-    flagged snake positions whose walks are shorter (100) or longer (300) than
-    the record a speculative walk keeps for its writes."""
+    flagged snake positions: walks of 100 and 300 steps, replayed from the
+    speculative record, and of about 6000 steps, longer than the record's 4096,
+    which the genuine walk then makes again as it writes."""
     cfg = S.sedov_config(n, "fp64_exact")
     rng = np.random.default_rng(7)
     u = np.zeros(S.state_shape(cfg))
