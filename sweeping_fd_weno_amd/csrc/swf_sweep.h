@@ -93,6 +93,11 @@ struct SweepShared {
   unsigned char psum_bad[NT];  // the slot's speculative walk failed ex6's pressure-sum check
   int nlist, last_end, flag, all_dirty, nsteps;
   int chain0, chainlast;  // the genuine walks' first and last slot (select_genuine)
+  // the positions a pass leaves flagged, in the pass's order, as its replay writes
+  // them (the next pass's candidates or the convergence test's list); out_ok is 0
+  // when the list is incomplete (a walk that wrote as it went, or overflow)
+  int list2[SW_WCH * 32];
+  int nout, out_ok, nscan;
   long long tpos, tneg;
   double ubar[4];
   double red[4][NT / 64];
@@ -409,9 +414,10 @@ SWF_D int spec_walk(const KCfg &k, const Bufs &b, int order, int f, const double
 // record of slot `slot`: st < adj writes the adjusted cell f+st (f-st backward),
 // flags it for the next pass and counts its branch; st == adj writes the value
 // handed to the stop position.  The same stores, flags and counts as the walk.
+// Returns the position it flags for the next pass, else -1.
 template <bool FWD, int NT>
-SWF_D void replay_step(const KCfg &k, const Bufs &b, int order, int f, int st, int adj, int slot, unsigned *next_map,
-                       long long *npos, long long *nneg, SweepShared<NT> *sh) {
+SWF_D int replay_step(const KCfg &k, const Bufs &b, int order, int f, int st, int adj, int slot, unsigned *next_map,
+                      long long *npos, long long *nneg, SweepShared<NT> *sh) {
 #pragma clang fp contract(off)
   double v[4];
 #pragma unroll
@@ -422,12 +428,14 @@ SWF_D void replay_step(const KCfg &k, const Bufs &b, int order, int f, int st, i
   cell_of(k, order, p, i, j);
   store_ij(k, b, i, j, v);
   mark_dirty(k, *sh, i, j);
-  if (st == adj) return;
-  if (sweep_flag(k, v[0], v[1], v[2], v[3])) set_bit(next_map, p);
+  if (st == adj) return -1;
+  const bool fl = sweep_flag(k, v[0], v[1], v[2], v[3]);
+  if (fl) set_bit(next_map, p);
   if ((pw >> (st & 31)) & 1u)
     ++*npos;
   else
     ++*nneg;
+  return fl ? p : -1;
 }
 
 // Density-sweep walk (scalar_sweep) in order I; dir +1 forward, -1 backward.
@@ -583,9 +591,9 @@ SWF_D int collect_all(const KCfg &k, const unsigned *map, SweepShared<NT> &sh, i
 // by pointer doubling (round r marks the successors 2^r links ahead of the marked
 // slots), ceil(log2 m) rounds instead of one dependent hop per genuine walk.
 template <bool FWD, int NT>
-SWF_D void select_genuine(SweepShared<NT> &sh, int s0, int n, int tid) {
+SWF_D void select_genuine(SweepShared<NT> &sh, const int *cl, int s0, int n, int tid) {
   const int m = min(NT, n - s0);
-  auto pos = [&](int q) { return FWD ? sh.list[s0 + q] : sh.list[n - 1 - (s0 + q)]; };
+  auto pos = [&](int q) { return FWD ? cl[s0 + q] : cl[n - 1 - (s0 + q)]; };
   auto beyond = [&](int lo, int e) {  // first slot in [lo, m) past position e, or m
     int hi = m;
     while (lo < hi) {
@@ -629,16 +637,17 @@ SWF_D void select_genuine(SweepShared<NT> &sh, int s0, int n, int tid) {
   __syncthreads();
 }
 
-// Walks of one pass from the n positions in sh.list (ascending; a backward pass
-// takes them from the end), NT candidates at a time.
+// Walks of one pass from the n positions in cl (ascending; a backward pass takes
+// them from the end), NT candidates at a time.  ol (or null): the positions the
+// pass leaves flagged are appended to it in the pass's order (sh.nout, sh.out_ok).
 template <bool FWD, int NT>
 SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsigned *next, const double ub[4],
-                          double pb, SweepShared<NT> &sh, int tid) {
+                          double pb, SweepShared<NT> &sh, int tid, const int *cl, int *ol) {
   const int N = k.nx * k.ny;
   for (int s0 = 0; s0 < n; s0 += NT) {
     const int idx = s0 + tid;
     int f = -1;
-    if (idx < n) f = FWD ? sh.list[idx] : sh.list[n - 1 - idx];
+    if (idx < n) f = FWD ? cl[idx] : cl[n - 1 - idx];
     const int le = sh.last_end;
     const bool cand = (f >= 0) && (FWD ? (f > le && f <= N - 2) : (f < le && f >= 1));
     bool hit = false;
@@ -656,7 +665,7 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
     sh.psum_bad[tid] = hit ? 1 : 0;
     sh.genuine[tid] = 0;
     __syncthreads();
-    select_genuine<FWD, NT>(sh, s0, n, tid);
+    select_genuine<FWD, NT>(sh, cl, s0, n, tid);
     // genuine walks: recorded ones are replayed by the whole workgroup (their
     // steps are independent), longer ones are walked again by their thread
     long long np = 0, nn = 0;
@@ -665,29 +674,50 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
     if (sh.genuine[tid]) {
       const int end = sh.ends[tid];
       const int adj = FWD ? end - f : f - end;
-      if (adj < SW_REC)
+      if (adj < SW_REC) {
         len = adj + 1;
-      else
+      } else {
         write_walk<FWD, NT>(k, b, order, f, ub, pb, next, &np, &nn, &sh);
+        sh.out_ok = 0;  // its flags are in the bitmap only
+      }
     }
     sh.woff[tid] = block_exclusive_scan<NT>(len, sh, &sh.nsteps, tid);
     __syncthreads();
     const int total = sh.nsteps;
     const int m = min(NT, n - s0);
-    for (int x = tid; x < total; x += NT) {
-      int lo = 0, hi = m - 1;  // the last slot whose replay starts at or before x
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (sh.woff[mid] <= x)
-          lo = mid;
-        else
-          hi = mid - 1;
+    // the replay in rounds of NT steps; step x is the x-th written cell of the chunk
+    // in the pass's order, so the flagged ones, compacted round by round, extend ol
+    for (int x0 = 0; x0 < total; x0 += NT) {
+      const int x = x0 + tid;
+      int fp = -1;
+      if (x < total) {
+        int lo = 0, hi = m - 1;  // the last slot whose replay starts at or before x
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sh.woff[mid] <= x)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        const int w = lo, st = x - sh.woff[w];
+        const int fw = FWD ? cl[s0 + w] : cl[n - 1 - (s0 + w)];
+        const int endw = sh.ends[w];
+        const int adj = FWD ? endw - fw : fw - endw;
+        fp = replay_step<FWD, NT>(k, b, order, fw, st, adj, w, next, &np, &nn, &sh);
       }
-      const int w = lo, st = x - sh.woff[w];
-      const int fw = FWD ? sh.list[s0 + w] : sh.list[n - 1 - (s0 + w)];
-      const int endw = sh.ends[w];
-      const int adj = FWD ? endw - fw : fw - endw;
-      replay_step<FWD, NT>(k, b, order, fw, st, adj, w, next, &np, &nn, &sh);
+      if (ol) {
+        const int base = sh.nout;
+        const int r = block_exclusive_scan<NT>(fp >= 0 ? 1 : 0, sh, &sh.nscan, tid);
+        if (fp >= 0) {
+          if (base + r < SW_WCH * 32)
+            ol[base + r] = fp;
+          else
+            sh.out_ok = 0;
+        }
+        __syncthreads();
+        if (tid == 0) sh.nout = base + sh.nscan;
+        __syncthreads();
+      }
     }
     if (np) atomicAdd((unsigned long long *)&sh.tpos, (unsigned long long)np);
     if (nn) atomicAdd((unsigned long long *)&sh.tneg, (unsigned long long)nn);
@@ -698,31 +728,49 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
 // One directional pass of the pressure sweep over the flagged positions in
 // `map` (order `order`).  Adjusted-but-still-flagged cells are recorded in `next`.
 // More flagged positions than the list holds: chunk by chunk of SW_WCH words.
+// cl_in (or null): the flagged positions of `map`, n_in of them, ascending (the
+// previous pass's output list), used instead of scanning `map`.  ol (or null): see
+// pressure_walks; it must not be sh.list when the pass scans `map` into sh.list.
 template <bool FWD, int NT>
 SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned *map, unsigned *next,
-                         const double ub[4], double pb, SweepShared<NT> &sh, int tid) {
+                         const double ub[4], double pb, SweepShared<NT> &sh, int tid, const int *cl_in, int n_in,
+                         int *ol) {
   const int N = k.nx * k.ny;
-  if (tid == 0) sh.last_end = FWD ? -1 : N;
+  if (tid == 0) {
+    sh.last_end = FWD ? -1 : N;
+    sh.nout = 0;
+    sh.out_ok = ol != nullptr;
+  }
   __syncthreads();
-  const int n = collect_all<NT>(k, map, sh, tid);
+  const int n = cl_in ? n_in : collect_all<NT>(k, map, sh, tid);
   if (n >= 0) {
-    pressure_walks<FWD, NT>(k, b, order, n, next, ub, pb, sh, tid);
+    pressure_walks<FWD, NT>(k, b, order, n, next, ub, pb, sh, tid, cl_in ? cl_in : sh.list, ol);
   } else {
+    if (tid == 0) sh.out_ok = 0;
     const int nchunks = (k.nwords + SW_WCH - 1) / SW_WCH;
     for (int ci = 0; ci < nchunks; ++ci) {
       const int c = FWD ? ci : (nchunks - 1 - ci);
       collect_chunk<NT>(k, map, c * SW_WCH, sh, tid);
-      pressure_walks<FWD, NT>(k, b, order, sh.nlist, next, ub, pb, sh, tid);
+      pressure_walks<FWD, NT>(k, b, order, sh.nlist, next, ub, pb, sh, tid, sh.list, nullptr);
     }
   }
-  // the position the pass never examines (N-1 forward, 0 backward)
+  // the position the pass never examines (N-1 forward, 0 backward), the last of the
+  // pass's order
   if (tid == 0) {
     const int pe = FWD ? N - 1 : 0;
     int i, j;
     cell_of(k, order, pe, i, j);
     double v[4];
     load_ij(k, b, i, j, v);
-    if (sweep_flag(k, v[0], v[1], v[2], v[3])) set_bit(next, pe);
+    if (sweep_flag(k, v[0], v[1], v[2], v[3])) {
+      set_bit(next, pe);
+      if (ol) {
+        if (sh.nout < SW_WCH * 32)
+          ol[sh.nout++] = pe;
+        else
+          sh.out_ok = 0;
+      }
+    }
   }
   __syncthreads();
 }
@@ -739,7 +787,7 @@ SWF_D void density_walks(const KCfg &k, const Bufs &b, int n, SweepShared<NT> &s
     sh.ends[tid] = cand ? swalk<false, NT>(k, b, f, +1, nullptr) : -1;
     sh.genuine[tid] = 0;
     __syncthreads();
-    select_genuine<true, NT>(sh, s0, n, tid);
+    select_genuine<true, NT>(sh, sh.list, s0, n, tid);
     if (sh.genuine[tid]) swalk<true, NT>(k, b, f, +1, &sh);
     __syncthreads();
   }
@@ -925,9 +973,18 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
   for (int q = 1; q <= k.swp_num; ++q) {
     const int order = (q & 1) ? 0 : 1;
     clear_map<NT>(k, B, tid);
-    pressure_pass<true, NT>(k, b, order, A, B, ub, pb, sh, tid);
+    // the forward pass lists what it leaves flagged (ascending) in sh.list2; the
+    // backward pass takes its candidates from there and lists what it leaves flagged
+    // in sh.list for the convergence test and the next order (the bitmaps are kept
+    // as well, for the passes whose lists are incomplete)
+    pressure_pass<true, NT>(k, b, order, A, B, ub, pb, sh, tid, nullptr, 0, sh.list2);
+    const bool fwd_list = sh.out_ok;
+    const int n2 = sh.nout;
     clear_map<NT>(k, A, tid);
-    pressure_pass<false, NT>(k, b, order, B, A, ub, pb, sh, tid);
+    pressure_pass<false, NT>(k, b, order, B, A, ub, pb, sh, tid, fwd_list ? sh.list2 : nullptr, n2,
+                             fwd_list ? sh.list : nullptr);
+    const bool bwd_list = fwd_list && sh.out_ok;
+    const int n3 = sh.nout;
     // convergence test (:1714-1744): no cell left with P <= eps or rho <= eps
     // (ex5: P < eps or rho < eps)
     if (tid == 0) {
@@ -936,7 +993,7 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
     }
     __syncthreads();
     // the flagged positions once, as a list when they fit
-    const int nf = collect_all<NT>(k, A, sh, tid);
+    const int nf = bwd_list ? n3 : collect_all<NT>(k, A, sh, tid);
     if (nf >= 0) {
       for (int x = tid; x < nf; x += NT)
         if (!converged_at(k, b, order, sh.list[x])) atomicOr(&sh.flag, 1);
