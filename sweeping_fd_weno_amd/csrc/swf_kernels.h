@@ -403,24 +403,52 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
   const int gx = ub_gx(k), nblk = gx * ub_gy(k);
   const int nw = sh.all_dirty ? (nblk + 31) / 32 : min(SW_DIRTY_WORDS, (nblk + 31) / 32);
   const int grp = tid >> 5, lt = tid & 31;
-  // the dirty-tile bitmap in order; every thread walks it alike (uniform)
+  // the changed tiles as a list in sh.list (free after the sweep): each thread counts
+  // the set bits of its bitmap words, one block scan places them
+  int cnt = 0;
+  for (int w = tid; w < nw; w += NT) {
+    unsigned word = sh.all_dirty ? 0xffffffffu : sh.dirty[w];
+    if (w * 32 + 32 > nblk) word &= (nblk - w * 32 >= 32) ? 0xffffffffu : ((1u << (nblk - w * 32)) - 1u);
+    cnt += __popc(word);
+  }
+  const int first = block_exclusive_scan<NT>(cnt, sh, &sh.nscan, tid);
+  const int ntiles = sh.nscan;
+  if (ntiles <= SW_WCH * 32 && cnt) {
+    int o = first;
+    for (int w = tid; w < nw; w += NT) {
+      unsigned word = sh.all_dirty ? 0xffffffffu : sh.dirty[w];
+      if (w * 32 + 32 > nblk) word &= (nblk - w * 32 >= 32) ? 0xffffffffu : ((1u << (nblk - w * 32)) - 1u);
+      while (word) {
+        sh.list[o++] = w * 32 + (__ffs(word) - 1);
+        word &= word - 1;
+      }
+    }
+  }
+  __syncthreads();
+  // beyond the list's capacity (whole-grid sweeps of very large grids): the bitmap in
+  // order, every thread walking it alike
   int w = 0;
   unsigned word = (nw > 0) ? (sh.all_dirty ? 0xffffffffu : sh.dirty[0]) : 0u;
-  while (true) {
+  for (int r0 = 0;; r0 += RT) {
     int myblk = -1;
     bool any = false;
+    if (ntiles <= SW_WCH * 32) {
+      any = r0 < ntiles;
+      if (r0 + grp < ntiles) myblk = sh.list[r0 + grp];
+    } else {
 #pragma unroll
-    for (int q = 0; q < RT; ++q) {
-      while (!word && w + 1 < nw) {
-        ++w;
-        word = sh.all_dirty ? 0xffffffffu : sh.dirty[w];
-      }
-      if (word) {
-        const int blk = w * 32 + (__ffs(word) - 1);
-        word &= word - 1;
-        if (blk < nblk) {
-          any = true;
-          if (q == grp) myblk = blk;
+      for (int q = 0; q < RT; ++q) {
+        while (!word && w + 1 < nw) {
+          ++w;
+          word = sh.all_dirty ? 0xffffffffu : sh.dirty[w];
+        }
+        if (word) {
+          const int blk = w * 32 + (__ffs(word) - 1);
+          word &= word - 1;
+          if (blk < nblk) {
+            any = true;
+            if (q == grp) myblk = blk;
+          }
         }
       }
     }
