@@ -391,9 +391,10 @@ SWF_D double block_max(double a, double *red, int tid) {
 }
 
 // After a sweep: alpha partials of the update tiles whose cells changed, and
-// the edge ghosts of the boundary band.  One workgroup of NT threads (NT >= 512).
-// The changed tiles go NT / 32 at a time, 32 threads per tile, each thread's
-// cells loaded together; the tile's max by shuffles within its 32 lanes.
+// the edge ghosts of their cells in the boundary band.  One workgroup of NT
+// threads (NT >= 512).  The changed tiles go NT / 32 at a time, 32 threads per
+// tile, each thread's cells loaded together; the tile's max by shuffles within
+// its 32 lanes.
 template <int NT>
 SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double *red, int tid) {
   static_assert(NT % 64 == 0 && NT >= 512, "whole waves, 16 tiles per round at least");
@@ -467,6 +468,12 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
 #pragma unroll
       for (int r = 0; r < PER; ++r)
         if (ok[r]) a = fmax2(a, cell_alpha(k, v[r][0], v[r][1], v[r][2], v[r][3]));
+      // the ghost images of the tile's cells in the boundary band (3 cells deep)
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const int c = lt + 32 * r, i = i0 + c % UB_W, j = j0 + c / UB_W;
+        if (ok[r] && (i <= 3 || i >= k.nx - 2 || j <= 3 || j >= k.ny - 2)) write_ghosts(k, b, i, j, v[r]);
+      }
     }
 #pragma unroll
     for (int off = 16; off >= 1; off >>= 1) {
@@ -474,30 +481,6 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
       a = (o > a) ? o : a;
     }
     if (lt == 0 && myblk >= 0) b.apart[myblk] = a;
-  }
-  // ghosts of the boundary band
-  const int nb = 6 * k.ny + 6 * k.nx;
-  for (int t = tid; t < nb; t += NT) {
-    int i, j;
-    if (t < 6 * k.ny) {
-      const int c = t % 6;
-      j = 1 + t / 6;
-      i = (c < 3) ? 1 + c : k.nx - 5 + c;
-    } else {
-      const int s = t - 6 * k.ny, c = s % 6;
-      i = 1 + s / 6;
-      j = (c < 3) ? 1 + c : k.ny - 5 + c;
-    }
-    if (i >= 1 && i <= k.nx && j >= 1 && j <= k.ny) {
-      // only cells of tiles the sweep changed have new images
-      const int blk = ub_of(k, i, j);
-      const bool dirty = sh.all_dirty || (blk < SW_DIRTY_WORDS * 32 && ((sh.dirty[blk >> 5] >> (blk & 31)) & 1u));
-      if (dirty) {
-        double v[4];
-        load_ij(k, b, i, j, v);
-        write_ghosts(k, b, i, j, v);
-      }
-    }
   }
   __syncthreads();
 }

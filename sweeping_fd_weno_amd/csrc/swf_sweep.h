@@ -512,17 +512,28 @@ SWF_D void collect_chunk(const KCfg &k, const unsigned *map, int w0, SweepShared
   __syncthreads();
 }
 
+// Inclusive prefix sum of x over the 64 lanes of a wave, every lane active:
+// row shifts by 1, 2, 4 and 8 scan each row of 16 lanes; lane 15 broadcast to
+// rows 1 and 3, then lane 31 to rows 2 and 3, carry the rows' totals.  (Six DPP
+// adds; the same scan by lane shuffles took 1.7 times as long per block scan.)
+SWF_D int wave_inclusive_scan(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return x;
+}
+
 // Exclusive prefix sum of one int per thread over the workgroup, in thread
-// order; the total goes to *total (in LDS).  Uses sh.wscan.
+// order; the total goes to *total (in LDS).  Uses sh.wscan.  Called by every
+// thread of the workgroup.
 template <int NT>
 SWF_D int block_exclusive_scan(int x, SweepShared<NT> &sh, int *total, int tid) {
   static_assert(NT / 64 <= SW_WCH, "wave totals live in sh.wscan");
-  int incl = x;  // inclusive scan within the wave
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int v = __shfl_up(incl, off, 64);
-    if ((tid & 63) >= off) incl += v;
-  }
+  static_assert(NT % 64 == 0, "whole waves");
+  const int incl = wave_inclusive_scan(x);
   if ((tid & 63) == 63) sh.wscan[tid >> 6] = incl;
   __syncthreads();
   if (tid == 0) {
@@ -728,8 +739,9 @@ SWF_D void pressure_walks(const KCfg &k, const Bufs &b, int order, int n, unsign
 // One directional pass of the pressure sweep over the flagged positions in
 // `map` (order `order`).  Adjusted-but-still-flagged cells are recorded in `next`.
 // More flagged positions than the list holds: chunk by chunk of SW_WCH words.
-// cl_in (or null): the flagged positions of `map`, n_in of them, ascending (the
-// previous pass's output list), used instead of scanning `map`.  ol (or null): see
+// cl_in (or null): the pass's flagged positions, n_in of them, ascending (the
+// previous pass's output list, or the previous order's set remapped), used instead
+// of scanning `map`, which is then not read.  ol (or null): see
 // pressure_walks; it must not be sh.list when the pass scans `map` into sh.list.
 template <bool FWD, int NT>
 SWF_D void pressure_pass(const KCfg &k, const Bufs &b, int order, const unsigned *map, unsigned *next,
@@ -911,6 +923,32 @@ SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, const double 
 // The sweep of one stage (sweep_lim incl. scalar_sweep), run by one workgroup of
 // NT threads.  Marks the update blocks whose cells it changed in sh.dirty.
 // upart: the stage kernel's per-tile sums of weight*u of the state, or null.
+// The set of positions of `order` in bitmap A (listed in sh.list when nf >= 0) as
+// positions of the other order, set in bitmap B (cleared beforehand).
+template <int NT>
+SWF_D void remap_bits(const KCfg &k, int order, const unsigned *A, unsigned *B, int nf, SweepShared<NT> &sh,
+                      int tid) {
+  if (nf >= 0) {
+    for (int x = tid; x < nf; x += NT) {
+      int i, j;
+      cell_of(k, order, sh.list[x], i, j);
+      set_bit(B, pos_of(k, 1 - order, i, j));
+    }
+  } else {
+    for (int w = tid; w < k.nwords; w += NT) {
+      unsigned word = A[w];
+      while (word) {
+        const int bit = __ffs(word) - 1;
+        word &= word - 1;
+        int i, j;
+        cell_of(k, order, w * 32 + bit, i, j);
+        set_bit(B, pos_of(k, 1 - order, i, j));
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <int NT>
 SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const double *upart, SweepShared<NT> &sh,
                       int tid) {
@@ -970,6 +1008,7 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
   }
   unsigned *A = b.bmP, *B = b.bmA;
   int qdone = 0;
+  int nin = -1;  // the forward pass's candidates are sh.list[0, nin) when nin >= 0, else map A
   for (int q = 1; q <= k.swp_num; ++q) {
     const int order = (q & 1) ? 0 : 1;
     clear_map<NT>(k, B, tid);
@@ -977,7 +1016,7 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
     // backward pass takes its candidates from there and lists what it leaves flagged
     // in sh.list for the convergence test and the next order (the bitmaps are kept
     // as well, for the passes whose lists are incomplete)
-    pressure_pass<true, NT>(k, b, order, A, B, ub, pb, sh, tid, nullptr, 0, sh.list2);
+    pressure_pass<true, NT>(k, b, order, A, B, ub, pb, sh, tid, nin >= 0 ? sh.list : nullptr, nin, sh.list2);
     const bool fwd_list = sh.out_ok;
     const int n2 = sh.nout;
     clear_map<NT>(k, A, tid);
@@ -1013,27 +1052,30 @@ SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const doub
       break;
     }
     if (q == k.swp_num) break;
-    // flagged set of the next pass, in the other snake order
-    clear_map<NT>(k, B, tid);
-    if (nf >= 0) {
-      for (int x = tid; x < nf; x += NT) {
+    // flagged set of the next pass, in the other snake order: up to NT positions as
+    // an ascending list in sh.list (each thread places its position by counting the
+    // smaller ones; map A is then not read), more as a bitmap
+    if (nf >= 0 && nf <= NT) {
+      int key = 0;
+      if (tid < nf) {
         int i, j;
-        cell_of(k, order, sh.list[x], i, j);
-        set_bit(B, pos_of(k, 1 - order, i, j));
+        cell_of(k, order, sh.list[tid], i, j);
+        key = pos_of(k, 1 - order, i, j);
+        sh.list2[tid] = key;
       }
+      __syncthreads();
+      if (tid < nf) {
+        int r = 0;
+        for (int y = 0; y < nf; ++y) r += sh.list2[y] < key ? 1 : 0;
+        sh.list[r] = key;
+      }
+      __syncthreads();
+      nin = nf;
     } else {
-      for (int w = tid; w < k.nwords; w += NT) {
-        unsigned word = A[w];
-        while (word) {
-          const int bit = __ffs(word) - 1;
-          word &= word - 1;
-          int i, j;
-          cell_of(k, order, w * 32 + bit, i, j);
-          set_bit(B, pos_of(k, 1 - order, i, j));
-        }
-      }
+      nin = -1;
+      clear_map<NT>(k, B, tid);
+      remap_bits<NT>(k, order, A, B, nf, sh, tid);
     }
-    __syncthreads();
     unsigned *t = A;
     A = B;
     B = t;

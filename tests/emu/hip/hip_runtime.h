@@ -12,6 +12,7 @@
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #include <math.h>
 #include <functional>
@@ -134,6 +135,38 @@ T __shfl_down(T v, unsigned d, int width = 64) {
   T out;
   memcpy(&out, &r, sizeof(T));
   return out;
+}
+
+// DPP move of an int (the controls this build uses: row_shr:1-15, row_bcast:15,
+// row_bcast:31); rows outside row_mask, and lanes whose source lies outside the
+// row (shift) or before row 1 / row 2 (broadcasts), take `old` (bound_ctrl off)
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask,
+                                       bool bound_ctrl) {
+  (void)bank_mask;
+  const int t = emu::lin_tid();
+  emu::shfl_buf[t] = (unsigned long long)(unsigned)src;
+  emu::wave_sync();
+  const int lane = t & 63, row = lane >> 4, w0 = t & ~63;
+  int s = -1;
+  if (ctrl > 0x110 && ctrl < 0x120) {
+    const int n = ctrl - 0x110;
+    if ((lane & 15) >= n) s = lane - n;
+  } else if (ctrl == 0x142) {
+    if (row >= 1) s = row * 16 - 1;
+  } else if (ctrl == 0x143) {
+    if (row >= 2) s = 31;
+  } else {
+    abort();
+  }
+  int r = old;
+  if ((row_mask >> row) & 1) {
+    if (s >= 0)
+      r = (int)(unsigned)emu::shfl_buf[w0 + s];
+    else if (bound_ctrl)
+      r = 0;
+  }
+  emu::wave_sync();
+  return r;
 }
 
 // the value of the wave's first lane (every lane active in this build's uses)
