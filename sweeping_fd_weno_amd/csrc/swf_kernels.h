@@ -467,7 +467,9 @@ SWF_D void after_sweep(const KCfg &k, const Bufs &b, SweepShared<NT> &sh, double
       }
 #pragma unroll
       for (int r = 0; r < PER; ++r)
-        if (ok[r]) a = fmax2(a, cell_alpha(k, v[r][0], v[r][1], v[r][2], v[r][3]));
+        if (ok[r])  // the stage kernel's function for the mode (fast_div: every mode but the exact one)
+          a = fmax2(a, k.fast_div ? cell_alpha_fast(k, v[r][0], v[r][1], v[r][2], v[r][3])
+                                  : cell_alpha(k, v[r][0], v[r][1], v[r][2], v[r][3]));
       // the ghost images of the tile's cells in the boundary band (3 cells deep)
 #pragma unroll
       for (int r = 0; r < PER; ++r) {
