@@ -920,9 +920,6 @@ SWF_D void compute_ubar(const KCfg &k, const Bufs &b, bool serial, const double 
   __syncthreads();
 }
 
-// The sweep of one stage (sweep_lim incl. scalar_sweep), run by one workgroup of
-// NT threads.  Marks the update blocks whose cells it changed in sh.dirty.
-// upart: the stage kernel's per-tile sums of weight*u of the state, or null.
 // The set of positions of `order` in bitmap A (listed in sh.list when nf >= 0) as
 // positions of the other order, set in bitmap B (cleared beforehand).
 template <int NT>
@@ -949,6 +946,9 @@ SWF_D void remap_bits(const KCfg &k, int order, const unsigned *A, unsigned *B, 
   __syncthreads();
 }
 
+// The sweep of one stage (sweep_lim incl. scalar_sweep), run by one workgroup of
+// NT threads.  Marks the update blocks whose cells it changed in sh.dirty.
+// upart: the stage kernel's per-tile sums of weight*u of the state, or null.
 template <int NT>
 SWF_D void sweep_body(const KCfg &k, const Bufs &b, bool serial_ubar, const double *upart, SweepShared<NT> &sh,
                       int tid) {
